@@ -1,0 +1,100 @@
+/*
+ * GPU-native interface of the matchgate target / gradient / Hessian-vector
+ * product path (no reference counterpart: the reference evaluates targets through
+ * a per-column host callback, src/matchgate_target.h:10, and has no HVP).
+ *
+ * Targets live on the device; gate programs generalise the reference's tied
+ * brickwall (one gate per layer, src/matchgate_target.c:308-324) to any map from
+ * gate instances to parameter gates, e.g. the open-boundary brickwall with an
+ * independent gate per instance.  Sweeps run in FP64 or FP32 (MG_FP32); traces,
+ * gradients and second derivatives are always accumulated in FP64 and returned as
+ * FP64.  Results are deterministic run to run.
+ *
+ * All functions return 0 on success and -1 on error (message on stderr and in
+ * mg_last_error()).  The library is used from one host thread at a time.
+ */
+#pragma once
+#include "matchgate.h"
+#include "matchgate_target.h"
+#include "trust_region.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mg_target mg_target;
+typedef struct mg_program mg_program;
+
+enum { MG_FP64 = 0, MG_FP32 = 1 };
+enum { MG_EVAL_F = 0, MG_EVAL_F_GRAD = 1, MG_EVAL_F_GRAD_HVP = 2 };
+
+const char* mg_last_error(void);
+
+/* ---- targets U (f = -Tr(U^dagger W)) ---- */
+/* dense U, row-major 2^n x 2^n (the zgemv 'ufunc' of examples/hubbard1d/hubbard1d_opt.c:41-52) */
+mg_target* mg_target_dense(const numeric* U, int nqubits);
+/* matrix-free brickwall U (the 'ufunc_matfree' of examples/hubbard1d/hubbard1d_opt.c:23-39) */
+mg_target* mg_target_brickwall(const struct matchgate ulist[], int ulayers, const int* uperms[], int nqubits);
+/* sequential circuit U: gate k on wires (wires[2k], wires[2k+1]) */
+mg_target* mg_target_circuit(const struct matchgate gates[], int ngates, const int wires[], int nqubits);
+/* any linear map, materialised once by 2^n calls of ufunc (reentrant, called from several threads) */
+mg_target* mg_target_from_linear_func(linear_func ufunc, void* udata, int nqubits);
+void mg_target_free(mg_target* t);
+
+/* ---- gate programs ---- */
+/* ninst instances; instance g applies parameter gate param[g] to (wires[2g], wires[2g+1]) */
+mg_program* mg_program_create(int nqubits, int nparams, int ninst, const int* param, const int* wires);
+/* the reference's tied brickwall: nlayers parameter gates */
+mg_program* mg_program_brickwall(int nqubits, int nlayers, const int* perms[]);
+void mg_program_free(mg_program* p);
+int mg_program_nparams(const mg_program* p);
+int mg_program_ninst(const mg_program* p);
+
+/* ---- evaluation over all 2^n basis columns ----
+ * what = MG_EVAL_F: f;  MG_EVAL_F_GRAD: + complex gradient dgrad[P] (df/dV, no conjugation,
+ * src/gate.c:718-722);  MG_EVAL_F_GRAD_HVP: + dhz[P] = D^2 f[Z] for the complex direction Z[P]. */
+int mg_evaluate(const mg_target* t, const mg_program* p, const struct matchgate gates[], const struct matchgate Z[],
+                int what, int precision, numeric* f, struct matchgate dgrad[], struct matchgate dhz[]);
+
+/* the same over the basis columns [b0, b1) only: partial sums that add up to the full
+ * trace (one GPU's share when the columns are split across devices) */
+int mg_evaluate_columns(const mg_target* t, const mg_program* p, const struct matchgate gates[],
+                        const struct matchgate Z[], int what, int precision, long b0, long b1, numeric* f,
+                        struct matchgate dgrad[], struct matchgate dhz[]);
+
+/* Riemannian gradient (8P reals, src/matchgate_target.c:903-908) and, if v != NULL,
+ * the Riemannian Hessian-vector product H v (8P reals), one sweep */
+int mg_riemannian_gradient_hvp(const mg_target* t, const mg_program* p, const struct matchgate gates[], const double* v,
+                               int precision, numeric* f, double* grad_vec, double* hvp);
+
+/* complex Hessian (8P)^2 (layout of src/matchgate_brickwall.c:135) from 8P directional sweeps */
+int mg_hessian(const mg_target* t, const mg_program* p, const struct matchgate gates[], int precision, numeric* f,
+               struct matchgate dgrad[], numeric* hess);
+
+/* real gradient and Riemannian Hessian matrix (src/matchgate_target.c:877-973) */
+int mg_gradient_vector_hessian_matrix(const mg_target* t, const mg_program* p, const struct matchgate gates[],
+                                      int precision, numeric* f, double* grad_vec, double* H);
+
+/* ---- Riemannian trust-region optimisation on the GPU path ----
+ * hvp != 0: truncated CG on GPU Hessian-vector products; hvp == 0: explicit Hessian matrix.
+ * precision_switch: the last precision_switch iterations (and the final f) run in FP64
+ * when precision == MG_FP32.  nhvp (nullable) receives the number of HVP sweeps. */
+int mg_optimize(const mg_target* t, const mg_program* p, const struct matchgate gates_start[],
+                const struct rtr_params* params, int niter, int hvp, int precision, int precision_switch,
+                double* f_iter, numeric* f_citer, struct matchgate gates_opt[], int* nhvp);
+
+/* ---- tuning and measurement ---- */
+/* force checkpointed (1) or uncomputed (0) reverse sweeps; -1 restores automatic choice */
+void mg_set_checkpointing(int mode);
+/* force LDS-resident (1) or global-scratch (0) states; -1 automatic */
+void mg_set_lds_states(int mode);
+/* record HIP events around every sweep kernel on the library's stream */
+void mg_set_kernel_timing(int on);
+/* duration of the last sweep kernel in ms (needs mg_set_kernel_timing(1)) */
+float mg_last_kernel_ms(void);
+/* details of the last sweep: states in LDS, checkpointing, grid size, dynamic LDS bytes */
+void mg_last_sweep_info(int* lds_states, int* ckpt, int* grid, long* lds_bytes);
+
+#ifdef __cplusplus
+}
+#endif

@@ -1,0 +1,92 @@
+// Host runtime of the GPU evaluation path: device context, device-resident
+// targets, gate programs and the f / gradient / D^2 f[Z] evaluator.
+#pragma once
+#include <complex>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "mg_device.hpp"
+#include "../../include/config.h"
+
+namespace mg {
+
+using zd = std::complex<double>;
+
+void set_error(const std::string& msg);
+const char* last_error();
+
+// grow-only device allocation owned by the context
+struct DevBuf {
+    void* p = nullptr;
+    size_t n = 0;
+    hipError_t ensure(size_t bytes);
+    void release();
+    ~DevBuf() { release(); }
+};
+
+// A gate program: G instances; instance g applies parameter gate param[g]
+// to wires (wires[2g], wires[2g+1]).
+struct Program {
+    int L = 0, P = 0, G = 0;
+    std::vector<int> param, wires;
+    bool validate(std::string* why) const;
+};
+
+// tied brickwall of the reference: layer i applies gate i to the pairs
+// (perms[i][2k], perms[i][2k+1]) (reference src/matchgate_target.c:308-324)
+Program brickwall_program(int L, int nlayers, const int* const* perms);
+
+// Device-resident target U.  Dense: the seed matrix row b = -conj(U e_b) in FP64
+// (and FP32 on first use).  Program: a gate program (with its gates) applied to
+// e_b inside the sweep kernel.
+struct Target {
+    int kind = 0;  // 0 dense, 1 program
+    int L = 0;
+    DevBuf seed64, seed32;
+    int Gt = 0;
+    DevBuf tinst, tcoef64, tcoef32;
+    std::vector<Inst> tinst_h;
+    std::vector<cpx<double>> tcoef_h;
+    bool seed32_ready = false, tcoef32_ready = false;
+};
+
+// U row-major (S x S)
+Target* target_dense(const zd* U, int L);
+// rows already in seed layout: seed[b*S + a] = -conj(U[a][b])
+Target* target_dense_seed(std::vector<cpx<double>>&& seed, int L);
+Target* target_program(const Program& prog, const zd* gates);
+
+enum Mode { kF = 0, kFGrad = 1, kFGradHvp = 2 };
+enum Precision { kFP64 = 0, kFP32 = 1 };
+
+struct EvalOut {
+    zd f;
+    std::vector<zd> grad;  // [P*8]
+    std::vector<zd> dhz;   // [P*8]
+};
+
+struct EvalStats {
+    bool lds_states = false, ckpt = false;
+    int grid = 0;
+    size_t lds_bytes = 0;
+    float kernel_ms = 0.f;  // sweep kernel duration (HIP events on the launch stream)
+};
+
+// gates, Z: P*8 complex each (Z may be null when mode < 2).  Column range [b0, b1).
+int evaluate(const Target& t, const Program& p, const zd* gates, const zd* Z, int mode, int precision,
+             EvalOut* out, long b0 = 0, long b1 = -1, EvalStats* stats = nullptr);
+
+// configuration knobs (mostly for tests and tuning)
+struct Knobs {
+    int force_ckpt = -1;      // -1 auto, 0 never, 1 always (when mode >= 1)
+    int force_lds = -1;       // -1 auto, 0 global-scratch states, 1 LDS states (if they fit)
+    int grid_per_cu = 0;      // 0 auto
+    bool time_kernels = false;
+};
+Knobs& knobs();
+
+hipStream_t stream();
+int device_cu_count();
+
+}  // namespace mg

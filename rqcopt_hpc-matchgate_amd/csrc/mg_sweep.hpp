@@ -1,0 +1,41 @@
+// Launch interface of the column-sweep kernels (mg_sweep.hip).
+#pragma once
+#include "mg_device.hpp"
+
+namespace mg {
+
+template <typename R>
+struct SweepArgs {
+    int L;                  // qubits; S = 2^L amplitudes per column
+    int G;                  // gate instances of the program
+    int P;                  // parameter gates (gradient slots)
+    const Inst* inst;       // [G]
+    const cpx<R>* coef;     // [G][kCoefPerInst]: V, V^{-1}, Z (sorted basis)
+    int target_kind;        // 0: dense seed rows, 1: target program applied in-kernel
+    const cpx<R>* seed;     // [S][S]: row b = -conj(U e_b)
+    int Gt;                 // target program instances
+    const Inst* tinst;      // [Gt]
+    const cpx<R>* tcoef;    // [Gt][kCoefPerInst] (V used)
+    long b0, b1;            // column range [b0, b1)
+    double* partial;        // [gridDim.x][2 + 32 P]: f (re, im), then per parameter 16 complex
+    cpx<R>* scratch;        // global state slices [gridDim.x][nstate][S] (when states are not in LDS)
+    cpx<R>* ckpt;           // checkpoints [gridDim.x][G+1][2][S] (checkpoint mode)
+    int nstate;             // state vectors per workgroup
+    int lds_state_offset;   // byte offset of the states inside dynamic LDS (LDS mode)
+};
+
+// doubles of LDS bookkeeping in front of the states: acc [P][32], stage [2][NW][32], f [NW][2]
+inline size_t sweep_lds_head_bytes(int P, int nwaves)
+{
+    size_t b = sizeof(double) * (32 * (size_t)P + 2 * nwaves * 32 + 2 * nwaves);
+    return (b + 255) & ~(size_t)255;
+}
+
+template <typename R>
+hipError_t launch_sweep(const SweepArgs<R>& a, int mode, bool lds_states, bool ckpt, int grid, size_t lds, hipStream_t s);
+
+hipError_t launch_reduce(const double* partial, int nblk, int stride, double* out, hipStream_t s);
+
+constexpr int kSweepThreads = 256;
+
+}  // namespace mg

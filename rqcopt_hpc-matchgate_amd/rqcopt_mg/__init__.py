@@ -1,0 +1,346 @@
+"""ctypes binding of librqcopt_mg.so, the MI355X implementation of the
+matchgate-brickwall target / gradient / Hessian-vector-product path.
+
+The C-ABI (include/*.h) is the product; this module mirrors it for Python callers,
+tests and the benchmark.  Gates are numpy complex128 arrays of shape (P, 8)
+(center block entries 0-3, corner block 4-7, each 2x2 row-major), byte-identical
+to ``struct matchgate[P]``.
+
+Importing fails loudly if the HIP library has not been built (no CPU fallback).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(_DIR, "lib", "librqcopt_mg.so")
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(f"{LIB_PATH} is missing: build it with `make -C {_DIR}` (or __graft_entry__.build())")
+
+lib = ctypes.CDLL(LIB_PATH)
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_D = ctypes.c_double
+_L = ctypes.c_long
+
+MG_FP64, MG_FP32 = 0, 1
+EVAL_F, EVAL_F_GRAD, EVAL_F_GRAD_HVP = 0, 1, 2
+
+# typedef int (*linear_func)(const struct statevector*, void*, struct statevector*)
+LINEAR_FUNC = ctypes.CFUNCTYPE(ctypes.c_int, _P, _P, _P)
+
+
+class Statevector(ctypes.Structure):
+    _fields_ = [("data", _P), ("nqubits", ctypes.c_int)]
+
+
+class StatevectorArray(ctypes.Structure):
+    _fields_ = [("data", _P), ("nqubits", ctypes.c_int), ("nstates", ctypes.c_int)]
+
+
+class QuantumCircuitCache(ctypes.Structure):
+    _fields_ = [("psi_list", ctypes.POINTER(Statevector)), ("nqubits", ctypes.c_int), ("ngates", ctypes.c_int)]
+
+
+class TruncatedCGParams(ctypes.Structure):
+    _fields_ = [("maxiter", ctypes.c_int), ("abstol", _D), ("reltol", _D)]
+
+
+class RTRParams(ctypes.Structure):
+    _fields_ = [("tcg_params", TruncatedCGParams), ("rho_trust", _D), ("radius_init", _D), ("maxradius", _D),
+                ("g_func", _P), ("g_data", _P), ("g_iter", _P)]
+
+
+def rtr_default_params(n):
+    """set_rtr_default_params (reference src/trust_region.h:69-78)"""
+    return RTRParams(TruncatedCGParams(2 * n, 1e-8, 1e-6), 0.125, 0.01, 0.1, None, None, None)
+
+
+def _sig(name, args, res=ctypes.c_int):
+    fn = getattr(lib, name)
+    fn.argtypes = args
+    fn.restype = res
+    return fn
+
+
+_sig("mg_last_error", [], ctypes.c_char_p)
+_sig("mg_target_dense", [_P, _I], _P)
+_sig("mg_target_brickwall", [_P, _I, _P, _I], _P)
+_sig("mg_target_circuit", [_P, _I, _P, _I], _P)
+_sig("mg_target_from_linear_func", [LINEAR_FUNC, _P, _I], _P)
+_sig("mg_target_free", [_P], None)
+_sig("mg_program_create", [_I, _I, _I, _P, _P], _P)
+_sig("mg_program_brickwall", [_I, _I, _P], _P)
+_sig("mg_program_free", [_P], None)
+_sig("mg_program_nparams", [_P])
+_sig("mg_program_ninst", [_P])
+_sig("mg_evaluate", [_P, _P, _P, _P, _I, _I, _P, _P, _P])
+_sig("mg_evaluate_columns", [_P, _P, _P, _P, _I, _I, _L, _L, _P, _P, _P])
+_sig("mg_riemannian_gradient_hvp", [_P, _P, _P, _P, _I, _P, _P, _P])
+_sig("mg_hessian", [_P, _P, _P, _I, _P, _P, _P])
+_sig("mg_gradient_vector_hessian_matrix", [_P, _P, _P, _I, _P, _P, _P])
+_sig("mg_optimize", [_P, _P, _P, ctypes.POINTER(RTRParams), _I, _I, _I, _I, _P, _P, _P, _P])
+_sig("mg_set_checkpointing", [_I], None)
+_sig("mg_set_lds_states", [_I], None)
+_sig("mg_set_kernel_timing", [_I], None)
+_sig("mg_last_kernel_ms", [], ctypes.c_float)
+_sig("mg_last_sweep_info", [_P, _P, _P, _P], None)
+_sig("mg_target_cache_clear", [], None)
+# reference entry points (include/matchgate_target.h, mg_brickwall_opt.h)
+_sig("matchgate_circuit_unitary_target", [LINEAR_FUNC, _P, _P, _I, _P, _I, _P])
+_sig("parallel_matchgate_circuit_unitary_target", [LINEAR_FUNC, _P, _P, _I, _P, _I, _P])
+_sig("matchgate_circuit_unitary_target_and_gradient", [LINEAR_FUNC, _P, _P, _I, _P, _I, _P, _P])
+_sig("matchgate_brickwall_unitary_target", [LINEAR_FUNC, _P, _P, _I, _I, _P, _P])
+_sig("matchgate_brickwall_unitary_target_and_gradient", [LINEAR_FUNC, _P, _P, _I, _I, _P, _P, _P])
+_sig("matchgate_brickwall_unitary_target_gradient_hessian", [LINEAR_FUNC, _P, _P, _I, _I, _P, _P, _P, _P])
+_sig("parallel_matchgate_brickwall_unitary_target_gradient_hessian", [LINEAR_FUNC, _P, _P, _I, _I, _P, _P, _P, _P])
+_sig("matchgate_ti_brickwall_unitary_target_gradient_hessian", [LINEAR_FUNC, _P, _P, _I, _I, _P, _P, _P, _P])
+_sig("matchgate_brickwall_unitary_target_gradient_vector_hessian_matrix", [LINEAR_FUNC, _P, _P, _I, _I, _P, _P, _P, _P])
+_sig("optimize_matchgate_brickwall_circuit_hmat", [LINEAR_FUNC, _P, _P, _I, _I, _P, ctypes.POINTER(RTRParams), _I, _P, _P, _P], None)
+_sig("optimize_matchgate_brickwall_circuit_hvp", [LINEAR_FUNC, _P, _P, _I, _I, _P, ctypes.POINTER(RTRParams), _I, _P, _P, _P], None)
+# single-column kernels (include/gate.h, quantum_circuit.h, matchgate_brickwall.h)
+_SV = ctypes.POINTER(Statevector)
+_SA = ctypes.POINTER(StatevectorArray)
+_QC = ctypes.POINTER(QuantumCircuitCache)
+_sig("apply_matchgate", [_P, _I, _I, _SV, _SV], None)
+_sig("apply_matchgate_backward", [_P, _I, _I, _SV, _SV, _SV, _P], None)
+_sig("apply_matchgate_to_array", [_P, _I, _I, _SA, _SA], None)
+_sig("apply_matchgate_backward_array", [_P, _I, _I, _SA, _SV, _P], None)
+_sig("apply_matchgate_placeholder", [_I, _I, _SV, _SA], None)
+_sig("apply_quantum_matchgate_circuit", [_P, _I, _P, _SV, _SV])
+_sig("quantum_matchgate_circuit_forward", [_P, _I, _P, _SV, _QC, _SV])
+_sig("quantum_matchgate_circuit_backward", [_P, _I, _P, _QC, _SV, _SV, _P])
+_sig("allocate_quantum_circuit_cache", [_I, _I, _QC])
+_sig("free_quantum_circuit_cache", [_QC], None)
+_sig("apply_matchgate_brickwall_unitary", [_P, _I, _P, _SV, _SV])
+_sig("matchgate_brickwall_unitary_forward", [_P, _I, _P, _SV, _QC, _SV])
+_sig("matchgate_brickwall_unitary_backward", [_P, _I, _P, _QC, _SV, _SV, _P])
+_sig("matchgate_brickwall_unitary_backward_hessian", [_P, _I, _P, _QC, _SV, _SV, _P, _P])
+_sig("matchgate_ti_brickwall_unitary_backward_hessian", [_P, _I, _P, _QC, _SV, _SV, _P, _P])
+_sig("matchgate_ti_brickwall_unitary_backward", [_P, _I, _P, _QC, _SV, _SV, _P])
+# host algebra and trust region
+for _n in ("multiply_matchgates", "real_to_tangent_matchgate", "project_tangent_matchgate"):
+    _sig(_n, [_P, _P, _P], None)
+_sig("tangent_matchgate_to_real", [_P, _P, _P], None)
+_sig("polar_matchgate_factor", [_P, _P], None)
+_sig("retract_matchgate", [_P, _P, _P], None)
+_sig("inverse_mat2x2", [_P, _P])
+_sig("truncated_cg_hmat", [_P, _P, _I, _D, ctypes.POINTER(TruncatedCGParams), _P], ctypes.c_bool)
+
+
+class GPUError(RuntimeError):
+    pass
+
+
+def _check(ret, what):
+    if ret != 0:
+        raise GPUError(f"{what} failed ({ret}): {lib.mg_last_error().decode(errors='replace')}")
+
+
+def _c(a):
+    return np.ascontiguousarray(a, dtype=np.complex128)
+
+
+def _i(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _p(a):
+    return a.ctypes.data_as(_P)
+
+
+def _perm_ptrs(perms):
+    """const int* perms[] from an (nlayers, L) int array (keeps the buffers alive)."""
+    perms = _i(perms)
+    arr = (_P * perms.shape[0])(*[perms[k].ctypes.data for k in range(perms.shape[0])])
+    return perms, arr
+
+
+# ---------------- device targets and programs ----------------
+
+class Target:
+    def __init__(self, handle, keep=()):
+        if not handle:
+            raise GPUError("target creation failed: " + lib.mg_last_error().decode(errors="replace"))
+        self.h = handle
+        self._keep = keep
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib.mg_target_free(self.h)
+            self.h = None
+
+    @classmethod
+    def dense(cls, U):
+        U = _c(U)
+        L = int(np.log2(U.shape[0]))
+        return cls(lib.mg_target_dense(_p(U), L))
+
+    @classmethod
+    def brickwall(cls, ulist, uperms, L):
+        ulist = _c(ulist)
+        perms, arr = _perm_ptrs(uperms)
+        return cls(lib.mg_target_brickwall(_p(ulist), perms.shape[0], ctypes.cast(arr, _P), L))
+
+    @classmethod
+    def circuit(cls, gates, wires, L):
+        gates, wires = _c(gates), _i(wires)
+        return cls(lib.mg_target_circuit(_p(gates), gates.shape[0], _p(wires), L))
+
+
+class Program:
+    def __init__(self, handle, L):
+        if not handle:
+            raise GPUError("program creation failed: " + lib.mg_last_error().decode(errors="replace"))
+        self.h = handle
+        self.L = L
+        self.P = lib.mg_program_nparams(handle)
+        self.G = lib.mg_program_ninst(handle)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib.mg_program_free(self.h)
+            self.h = None
+
+    @classmethod
+    def create(cls, L, P, param, wires):
+        param, wires = _i(param), _i(wires)
+        return cls(lib.mg_program_create(L, P, len(param), _p(param), _p(wires)), L)
+
+    @classmethod
+    def brickwall(cls, L, perms):
+        perms, arr = _perm_ptrs(perms)
+        return cls(lib.mg_program_brickwall(L, perms.shape[0], ctypes.cast(arr, _P)), L)
+
+
+def evaluate(target, program, gates, Z=None, what=EVAL_F_GRAD_HVP, precision=MG_FP64):
+    """f, complex gradient (P,8), D^2 f[Z] (P,8) over all basis columns."""
+    gates = _c(gates)
+    P = program.P
+    f = np.zeros(1, np.complex128)
+    dg = np.zeros((P, 8), np.complex128)
+    dh = np.zeros((P, 8), np.complex128)
+    Zc = _c(Z) if Z is not None else None
+    _check(lib.mg_evaluate(target.h, program.h, _p(gates), _p(Zc) if Zc is not None else None, what, precision,
+                           _p(f), _p(dg), _p(dh)), "mg_evaluate")
+    return f[0], dg, dh
+
+
+def evaluate_columns(target, program, gates, Z, what, precision, b0, b1):
+    """Partial sums over the basis columns [b0, b1)."""
+    gates = _c(gates)
+    P = program.P
+    f = np.zeros(1, np.complex128)
+    dg = np.zeros((P, 8), np.complex128)
+    dh = np.zeros((P, 8), np.complex128)
+    Zc = _c(Z) if Z is not None else None
+    _check(lib.mg_evaluate_columns(target.h, program.h, _p(gates), _p(Zc) if Zc is not None else None, what,
+                                   precision, b0, b1, _p(f), _p(dg), _p(dh)), "mg_evaluate_columns")
+    return f[0], dg, dh
+
+
+def riemannian_gradient_hvp(target, program, gates, v=None, precision=MG_FP64):
+    gates = _c(gates)
+    P = program.P
+    f = np.zeros(1, np.complex128)
+    g = np.zeros(8 * P)
+    hv = np.zeros(8 * P)
+    vv = np.ascontiguousarray(v, dtype=np.float64) if v is not None else None
+    _check(lib.mg_riemannian_gradient_hvp(target.h, program.h, _p(gates), _p(vv) if vv is not None else None,
+                                          precision, _p(f), _p(g), _p(hv)), "mg_riemannian_gradient_hvp")
+    return f[0], g, (hv if v is not None else None)
+
+
+def hessian(target, program, gates, precision=MG_FP64):
+    gates = _c(gates)
+    m = 8 * program.P
+    f = np.zeros(1, np.complex128)
+    dg = np.zeros((program.P, 8), np.complex128)
+    H = np.zeros((m, m), np.complex128)
+    _check(lib.mg_hessian(target.h, program.h, _p(gates), precision, _p(f), _p(dg), _p(H)), "mg_hessian")
+    return f[0], dg, H
+
+
+def gradient_vector_hessian_matrix(target, program, gates, precision=MG_FP64):
+    gates = _c(gates)
+    m = 8 * program.P
+    f = np.zeros(1, np.complex128)
+    g = np.zeros(m)
+    H = np.zeros((m, m))
+    _check(lib.mg_gradient_vector_hessian_matrix(target.h, program.h, _p(gates), precision, _p(f), _p(g), _p(H)),
+           "mg_gradient_vector_hessian_matrix")
+    return f[0], g, H
+
+
+def optimize(target, program, gates, niter, hvp=True, precision=MG_FP64, precision_switch=0, params=None):
+    gates = _c(gates)
+    params = params if params is not None else rtr_default_params(16 * program.P)
+    f_iter = np.zeros(niter + 1)
+    f_citer = np.zeros(niter + 1, np.complex128)
+    out = np.zeros_like(gates)
+    nh = np.zeros(1, np.int32)
+    _check(lib.mg_optimize(target.h, program.h, _p(gates), ctypes.byref(params), niter, int(hvp), precision,
+                           precision_switch, _p(f_iter), _p(f_citer), _p(out), _p(nh)), "mg_optimize")
+    return f_iter, f_citer, out, int(nh[0])
+
+
+def set_checkpointing(mode):
+    lib.mg_set_checkpointing(int(mode))
+
+
+def set_lds_states(mode):
+    lib.mg_set_lds_states(int(mode))
+
+
+def set_kernel_timing(on):
+    lib.mg_set_kernel_timing(int(on))
+
+
+def last_kernel_ms():
+    return float(lib.mg_last_kernel_ms())
+
+
+def last_sweep_info():
+    a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    d = ctypes.c_long()
+    lib.mg_last_sweep_info(ctypes.byref(a), ctypes.byref(b), ctypes.byref(c), ctypes.byref(d))
+    return {"lds_states": bool(a.value), "ckpt": bool(b.value), "grid": c.value, "lds_bytes": d.value}
+
+
+# ---------------- reference-API helpers ----------------
+
+def dense_linear_func(U):
+    """A linear_func callback applying the dense matrix U (the zgemv 'ufunc' of
+    examples/hubbard1d/hubbard1d_opt.c:41-52)."""
+    U = _c(U)
+    S = U.shape[0]
+
+    def fn(psi_p, udata, out_p):
+        psi = ctypes.cast(psi_p, ctypes.POINTER(Statevector)).contents
+        out = ctypes.cast(out_p, ctypes.POINTER(Statevector)).contents
+        x = np.ctypeslib.as_array(ctypes.cast(psi.data, ctypes.POINTER(ctypes.c_double)), shape=(2 * S,)).view(np.complex128)
+        y = np.ctypeslib.as_array(ctypes.cast(out.data, ctypes.POINTER(ctypes.c_double)), shape=(2 * S,)).view(np.complex128)
+        y[:] = U @ x
+        return 0
+
+    cb = LINEAR_FUNC(fn)
+    cb._U = U
+    return cb
+
+
+def statevector(data):
+    data = _c(data)
+    sv = Statevector(data.ctypes.data, int(np.log2(data.shape[0])))
+    sv._keep = data
+    return sv
+
+
+def statevector_array(data):
+    """data: (2^L, nstates) interleaved"""
+    data = _c(data)
+    sa = StatevectorArray(data.ctypes.data, int(np.log2(data.shape[0])), data.shape[1])
+    sa._keep = data
+    return sa

@@ -1,0 +1,81 @@
+"""Synthetic workloads of the matchgate-brickwall optimisation: seeded Haar
+matchgates, brickwall layouts and the 1D Fermi-Hubbard target exp(-iHt).
+
+Randomness comes from numpy's PCG64 generator (``np.random.default_rng(seed)``),
+so every workload is reproducible from its seed.
+"""
+import numpy as np
+
+
+def haar_u2(rng):
+    """Haar-random 2x2 unitary: QR of a complex Gaussian matrix with the phases of
+    R's diagonal moved into Q (Mezzadri, Notices AMS 54, 592 (2007))."""
+    z = (rng.standard_normal((2, 2)) + 1j * rng.standard_normal((2, 2))) / np.sqrt(2)
+    q, r = np.linalg.qr(z)
+    d = np.diagonal(r)
+    return (q * (d / np.abs(d))).reshape(-1)
+
+
+def haar_matchgates(rng, n):
+    """n matchgates with independent Haar U(2) center (odd parity) and corner (even parity) blocks."""
+    return np.stack([np.concatenate([haar_u2(rng), haar_u2(rng)]) for _ in range(n)])
+
+
+def open_brickwall(L, depth):
+    """Open-boundary brickwall with one independent gate per instance: layer k acts on
+    (0,1),(2,3),... for even k and on (1,2),(3,4),... for odd k.
+    Returns (param, wires, ngates) with param[g] = g."""
+    wires = []
+    for k in range(depth):
+        for a in range(k % 2, L - 1, 2):
+            wires += [a, a + 1]
+    G = len(wires) // 2
+    return np.arange(G, dtype=np.int32), np.asarray(wires, dtype=np.int32), G
+
+
+def fermi_hubbard_1d_unitary(nsites, J=1.0, U=4.0, t=0.25):
+    """exp(-i H t), dense complex128, for the open-chain Fermi-Hubbard model
+        H = -J sum_{j,s} (c+_{j,s} c_{j+1,s} + h.c.) + U sum_j n_{j,up} n_{j,down}
+    with spin-up of site j on qubit 2j and spin-down on qubit 2j+1 (qubit 0 = most
+    significant bit), Jordan-Wigner in qubit order: the same-spin hop between qubits
+    2j+s and 2j+2+s carries a Z on qubit 2j+1+s.  H conserves (N_up, N_down), so it is
+    exponentiated block by block with a Hermitian eigendecomposition."""
+    L = 2 * nsites
+    S = 1 << L
+    idx = np.arange(S)
+
+    def bit(s, q):
+        return (s >> (L - 1 - q)) & 1
+
+    nup = sum(bit(idx, 2 * j) for j in range(nsites))
+    ndn = sum(bit(idx, 2 * j + 1) for j in range(nsites))
+    diag = U * sum(bit(idx, 2 * j) * bit(idx, 2 * j + 1) for j in range(nsites)).astype(np.float64)
+    W = np.zeros((S, S), np.complex128)
+    for nu in range(nsites + 1):
+        for nd in range(nsites + 1):
+            states = idx[(nup == nu) & (ndn == nd)]
+            pos = {int(s): k for k, s in enumerate(states)}
+            n = len(states)
+            H = np.diag(diag[states])
+            for k, s in enumerate(states):
+                s = int(s)
+                for j in range(nsites - 1):
+                    for sp in range(2):
+                        a, b = 2 * j + sp, 2 * j + 2 + sp
+                        if bit(s, a) != bit(s, b):
+                            s2 = s ^ (1 << (L - 1 - a)) ^ (1 << (L - 1 - b))
+                            sign = -1.0 if bit(s, 2 * j + 1 + sp) else 1.0
+                            H[pos[s2], k] += -J * sign
+            w, v = np.linalg.eigh(H)
+            blk = (v * np.exp(-1j * t * w)) @ v.conj().T
+            W[np.ix_(states, states)] = blk
+    return W
+
+
+def sweep_flops(L, ngates, ntarget_gates=0):
+    """Algorithmic FP64/FP32 flops of one f + gradient + HVP evaluation over all
+    2^L basis columns: 18 complex MACs per amplitude per gate (forward 2, gradient 4,
+    tangent forward/adjoint and the two outer products 12), 1 for the trace and 2 per
+    matrix-free target gate; 8 real flops per complex MAC."""
+    S = 1 << L
+    return 8.0 * S * S * (18 * ngates + 2 * ntarget_gates + 1)
