@@ -7,8 +7,9 @@
  * needs U on the device, the library materialises the callback once as a dense
  * matrix (2^n host calls, from several host threads, so the callback must be
  * reentrant exactly as under the reference's OpenMP loop) and caches it keyed by
- * (ufunc, udata, nqubits); call mg_target_cache_clear() after changing what a
- * cached callback computes.  Callers that know their target should use the
+ * (ufunc, udata, nqubits).  Before a cached matrix is reused the callback is run on
+ * four probe columns whose results must match the cached ones bitwise, else U is
+ * materialised again; mg_target_cache_clear() drops every cached matrix.  Callers that know their target should use the
  * descriptor API of mg_gpu.h instead (dense or brickwall targets on the device).
  *
  * Return codes as in the reference: 0 ok, -1 allocation/device error, -2 ufunc

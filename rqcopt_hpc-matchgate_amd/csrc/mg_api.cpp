@@ -37,10 +37,51 @@ zd* Zp(struct matchgate* g) { return reinterpret_cast<zd*>(g); }
 mg::EvalStats g_last_stats;
 
 // ---- materialised linear_func targets, cached per (ufunc, udata, nqubits) ----
-std::map<std::tuple<void*, void*, int>, std::unique_ptr<mg::Target>>& target_cache()
+// A cache entry also keeps a few probe columns of U.  Before an entry is reused the
+// callback is called again on those basis vectors and the results must match
+// bitwise; otherwise (a function pointer reused for another callback, or udata whose
+// contents changed) the target is materialised again.
+struct CachedTarget {
+    std::unique_ptr<mg::Target> t;
+    std::vector<size_t> probe_cols;
+    std::vector<numeric> probe_vals;
+};
+
+std::map<std::tuple<void*, void*, int>, CachedTarget>& target_cache()
 {
-    static std::map<std::tuple<void*, void*, int>, std::unique_ptr<mg::Target>> c;
+    static std::map<std::tuple<void*, void*, int>, CachedTarget> c;
     return c;
+}
+
+std::vector<size_t> probe_columns(int L)
+{
+    const size_t S = (size_t)1 << L;
+    std::vector<size_t> cols = {0, S - 1, (S / 3) | 1, (2 * S / 3) & ~(size_t)1};
+    std::sort(cols.begin(), cols.end());
+    cols.erase(std::unique(cols.begin(), cols.end()), cols.end());
+    return cols;
+}
+
+// U e_b for the probe columns; false if ufunc fails
+bool probe(linear_func ufunc, void* udata, int L, const std::vector<size_t>& cols, std::vector<numeric>* vals)
+{
+    const size_t S = (size_t)1 << L;
+    statevector psi{}, out{};
+    if (allocate_statevector(L, &psi) < 0 || allocate_statevector(L, &out) < 0) return false;
+    vals->clear();
+    bool ok = true;
+    for (size_t b : cols) {
+        std::memset((void*)psi.data, 0, S * sizeof(numeric));
+        psi.data[b] = 1.0;
+        if (ufunc(&psi, udata, &out) < 0) {
+            ok = false;
+            break;
+        }
+        vals->insert(vals->end(), out.data, out.data + S);
+    }
+    free_statevector(&psi);
+    free_statevector(&out);
+    return ok;
 }
 
 // Calls ufunc on every basis vector (several host threads, as the reference's
@@ -91,11 +132,27 @@ mg::Target* cached_target(linear_func ufunc, void* udata, int L, int* err)
     *err = 0;
     auto key = std::make_tuple((void*)ufunc, udata, L);
     auto& c = target_cache();
+    const std::vector<size_t> cols = probe_columns(L);
+    std::vector<numeric> vals;
+    if (!probe(ufunc, udata, L, cols, &vals)) {
+        std::fprintf(stderr, "call of 'ufunc' failed\n");
+        *err = -2;
+        return nullptr;
+    }
     auto it = c.find(key);
-    if (it != c.end()) return it->second.get();
+    if (it != c.end()) {
+        const CachedTarget& e = it->second;
+        if (e.probe_vals.size() == vals.size() &&
+            std::memcmp(e.probe_vals.data(), vals.data(), vals.size() * sizeof(numeric)) == 0)
+            return e.t.get();
+        c.erase(it);
+    }
     mg::Target* t = materialise(ufunc, udata, L, err);
     if (!t) return nullptr;
-    c[key].reset(t);
+    CachedTarget& e = c[key];
+    e.t.reset(t);
+    e.probe_cols = cols;
+    e.probe_vals = std::move(vals);
     return t;
 }
 

@@ -307,13 +307,24 @@ __global__ __launch_bounds__(T) void sweep_kernel(SweepArgs<R> a)
     for (int k = tid; k < 32 * a.P; k += T) out[2 + k] = acc[k];
 }
 
-__global__ void reduce_partials(const double* __restrict__ partial, int nblk, int stride, double* __restrict__ out)
+// out[k] = sum over workgroups w of partial[w][k], in a fixed order: thread row ty
+// sums w = ty, ty + 16, ... and row 0 then adds the 16 row sums in order.
+__global__ __launch_bounds__(1024) void reduce_partials(const double* __restrict__ partial, int nblk, int stride,
+                                                        double* __restrict__ out)
 {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= stride) return;
+    __shared__ double rows[16][64];
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    const int k = blockIdx.x * 64 + tx;
     double s = 0.0;
-    for (int w = 0; w < nblk; w++) s += partial[(size_t)w * stride + k];
-    out[k] = s;
+    if (k < stride)
+        for (int w = ty; w < nblk; w += 16) s += partial[(size_t)w * stride + k];
+    rows[ty][tx] = s;
+    __syncthreads();
+    if (ty == 0 && k < stride) {
+        double t = 0.0;
+        for (int r = 0; r < 16; r++) t += rows[r][tx];
+        out[k] = t;
+    }
 }
 
 // ---- host-side launch ----------------------------------------------------------------
@@ -347,7 +358,7 @@ template hipError_t launch_sweep<float>(const SweepArgs<float>&, int, bool, bool
 
 hipError_t launch_reduce(const double* partial, int nblk, int stride, double* out, hipStream_t s)
 {
-    hipLaunchKernelGGL(reduce_partials, dim3((stride + 255) / 256), dim3(256), 0, s, partial, nblk, stride, out);
+    hipLaunchKernelGGL(reduce_partials, dim3((stride + 63) / 64), dim3(1024), 0, s, partial, nblk, stride, out);
     return hipGetLastError();
 }
 
