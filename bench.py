@@ -69,6 +69,32 @@ def cpu_baseline(W, V, v, param, wires, L, budget_s=12.0):
     }
 
 
+def copy_calibration(mib):
+    """hipMemcpyDtoD of a known size: its FETCH_SIZE / WRITE_SIZE calibrate the sweep kernel's counters."""
+    import ctypes
+
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = mib << 20
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(n)) == 0 and hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(n)) == 0
+    for _ in range(5):
+        assert hip.hipMemcpyDtoD(b, a, ctypes.c_size_t(n)) == 0
+    assert hip.hipDeviceSynchronize() == 0
+    print(json.dumps({"copy_bytes": n, "copies": 5}))
+
+
+def time_steps(mg, t, p, V, v, prec, steps, warmup):
+    for _ in range(warmup):
+        mg.riemannian_gradient_hvp(t, p, V, v, prec)
+    kms = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        out = mg.riemannian_gradient_hvp(t, p, V, v, prec)
+        kms.append(mg.last_kernel_ms())
+    dt = time.perf_counter() - t0
+    return out, dt, float(np.mean(kms)), mg.last_sweep_info()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -77,7 +103,13 @@ def main():
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--rtr-iters", type=int, default=5, help="full trust-region iterations timed for the extra field")
+    ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-sweep variant")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device-to-device 5 times (reference kernel for HBM byte counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the column split across GPUs is a design note, "
               "not a measured configuration", file=sys.stderr)
@@ -94,37 +126,33 @@ def main():
     p = mg.Program.create(L, G, param, wires)
     mg.set_kernel_timing(True)
 
-    log(f"warmup {args.warmup} steps ({args.precision}) ...")
-    for _ in range(args.warmup):
-        mg.riemannian_gradient_hvp(t, p, V, v, prec)
-    info = mg.last_sweep_info()
-    log(f"sweep config: {info}")
-
-    kms = []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        f, g, hv = mg.riemannian_gradient_hvp(t, p, V, v, prec)
-        kms.append(mg.last_kernel_ms())
-    dt = time.perf_counter() - t0
+    log(f"{args.warmup} warmup + {args.steps} timed steps ({args.precision}) ...")
+    (f, g, hv), dt, kernel_ms, info = time_steps(mg, t, p, V, v, prec, args.steps, args.warmup)
     ms_per_step = 1e3 * dt / args.steps
     value = args.steps / dt
-    log(f"timed {args.steps} steps: {ms_per_step:.3f} ms/step, kernel {np.mean(kms):.3f} ms")
+    log(f"{ms_per_step:.3f} ms/step, sweep kernel {kernel_ms:.3f} ms, launch {info}")
 
     flops = sweep_flops(L, G)
-    kernel_s = float(np.mean(kms)) * 1e-3
-    achieved = flops / kernel_s / 1e12
+    achieved = flops / (kernel_ms * 1e-3) / 1e12
     roofline = {"bound": "mfma", "achieved": achieved, "peak": PEAK_TFLOPS[dtype], "unit": "TFLOP/s",
                 "frac": achieved / PEAK_TFLOPS[dtype], "traffic": None,
-                "kernel": "sweep_kernel (f+grad+HVP)", "kernel_ms": float(np.mean(kms)),
-                "flops_per_launch": flops}
+                "kernel": "sweep_reg_kernel" if info["reg_threads"] else "sweep_kernel",
+                "kernel_ms": kernel_ms, "flops_per_launch": flops,
+                "note": "VALU FP64/FP32 FMA bound; MI355X FP64 vector peak equals its FP64 matrix peak"}
 
-    # error of this evaluation against the FP64 path (for FP32 sweeps)
     extra = {}
-    if prec == mg.MG_FP32:
-        f64, g64, hv64 = mg.riemannian_gradient_hvp(t, p, V, v, mg.MG_FP64)
-        extra["rel_err_vs_fp64"] = {"f": float(abs(f - f64) / abs(f64)),
-                                    "grad": float(np.max(np.abs(g - g64)) / np.max(np.abs(g64))),
-                                    "hvp": float(np.max(np.abs(hv - hv64)) / np.max(np.abs(hv64)))}
+    if prec == mg.MG_FP64 and not args.no_fp32:
+        log("FP32-sweep variant ...")
+        (f32, g32, hv32), dt32, k32, info32 = time_steps(mg, t, p, V, v, mg.MG_FP32, args.steps, args.warmup)
+        a32 = flops / (k32 * 1e-3) / 1e12
+        extra["fp32_sweeps"] = {
+            "value": args.steps / dt32, "ms_per_step": 1e3 * dt32 / args.steps, "kernel_ms": k32,
+            "achieved_tflops": a32, "frac_of_157.3": a32 / PEAK_TFLOPS["f32"],
+            "rel_err_vs_fp64": {"f": float(abs(f32 - f) / abs(f)),
+                                "grad": float(np.max(np.abs(g32 - g)) / np.max(np.abs(g))),
+                                "hvp": float(np.max(np.abs(hv32 - hv)) / np.max(np.abs(hv)))},
+            "launch": info32}
+        log(f"FP32 sweeps: {1e3 * dt32 / args.steps:.3f} ms/step, kernel {k32:.3f} ms")
 
     # full trust-region iterations (f+grad, tCG HVPs, f at the candidate)
     if args.rtr_iters > 0:

@@ -88,6 +88,16 @@ int mg_optimize(const mg_target* t, const mg_program* p, const struct matchgate 
 void mg_set_checkpointing(int mode);
 /* force LDS-resident (1) or global-scratch (0) states; -1 automatic */
 void mg_set_lds_states(int mode);
+/* register-resident sweep kernel (states in VGPRs, LDS as re-layout buffer): mode -1 automatic
+ * (used when the states do not fit LDS), 0 never, 1 whenever a configuration exists;
+ * threads 0 automatic, 512 or 1024 */
+void mg_set_register_kernel(int mode, int threads);
+/* threads per workgroup of the LDS/global-state sweep kernel: 0 automatic, 256, 512 or 1024 */
+void mg_set_sweep_threads(int threads);
+/* workgroups per CU of the sweep launch (0 automatic) */
+void mg_set_grid_per_cu(int n);
+/* threads per workgroup of the last register-resident sweep, 0 if the last sweep used the LDS/global kernel */
+int mg_last_sweep_threads(void);
 /* record HIP events around every sweep kernel on the library's stream */
 void mg_set_kernel_timing(int on);
 /* duration of the last sweep kernel in ms (needs mg_set_kernel_timing(1)) */

@@ -604,6 +604,14 @@ int mg_gradient_vector_hessian_matrix(const mg_target* t, const mg_program* p, c
 void mg_set_checkpointing(int mode) { mg::knobs().force_ckpt = mode; }
 void mg_set_lds_states(int mode) { mg::knobs().force_lds = mode; }
 void mg_set_kernel_timing(int on) { mg::knobs().time_kernels = on != 0; }
+void mg_set_register_kernel(int mode, int threads)
+{
+    mg::knobs().force_reg = mode;
+    mg::knobs().reg_threads = threads;
+}
+void mg_set_grid_per_cu(int n) { mg::knobs().grid_per_cu = n; }
+void mg_set_sweep_threads(int threads) { mg::knobs().lds_threads = threads; }
+int mg_last_sweep_threads(void) { return g_last_stats.reg_threads; }
 float mg_last_kernel_ms(void) { return g_last_stats.kernel_ms; }
 void mg_last_sweep_info(int* lds_states, int* ckpt, int* grid, long* lds_bytes)
 {

@@ -1,0 +1,294 @@
+// Device helpers shared by the sweep kernels: quadruple gate kernels (the 4
+// amplitudes a two-qubit matchgate couples) and wave-level FP64 reductions.
+#pragma once
+#include "mg_device.hpp"
+
+namespace mg {
+
+// ---- wave-level transposing reduction -------------------------------------------
+// 32-bit exchange with lane ^ O.  Offsets 1 and 2 are DPP quad permutations
+// ([1,0,3,2] = 0xB1, [2,3,0,1] = 0x4E), offsets 4..16 ds_swizzle in bit-mask mode
+// (and 0x1F, or 0, xor O), 32 ds_bpermute.  DPP and ds_swizzle take the pattern as an
+// immediate, so no per-lane address register stays live across the sweep.
+template <int O>
+__device__ inline uint32_t xlane32(uint32_t x)
+{
+    if constexpr (O == 1) return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0xB1, 0xF, 0xF, true);
+    else if constexpr (O == 2) return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x4E, 0xF, 0xF, true);
+    else if constexpr (O < 32) return (uint32_t)__builtin_amdgcn_ds_swizzle((int)x, (O << 10) | 0x1F);
+    else return (uint32_t)__shfl_xor((int)x, O);
+}
+
+template <int O>
+__device__ inline float xlane(float x) { return __builtin_bit_cast(float, xlane32<O>(__builtin_bit_cast(uint32_t, x))); }
+
+template <int O>
+__device__ inline double xlane(double x)
+{
+    const uint64_t u = __builtin_bit_cast(uint64_t, x);
+    const uint64_t lo = xlane32<O>((uint32_t)u), hi = xlane32<O>((uint32_t)(u >> 32));
+    return __builtin_bit_cast(double, lo | (hi << 32));
+}
+
+template <typename T>
+struct Bits;
+template <>
+struct Bits<float> {
+    static __device__ inline uint32_t lo(float x) { return __builtin_bit_cast(uint32_t, x); }
+};
+
+// One halving step at offset O for the value pair (a, b) = (v[k], v[k+H]):
+// lanes with bit O clear return a + partner's a, lanes with it set b + partner's b.
+#ifndef MG_PERMLANE_BUTTERFLY
+#define MG_PERMLANE_BUTTERFLY 1
+#endif
+#ifndef MG_SGPR_COEF
+#define MG_SGPR_COEF 1
+#endif
+
+template <int O>
+__device__ inline float halve_pair(float a, float b, int lane)
+{
+    const bool up = (lane & O) != 0;
+    if constexpr (MG_PERMLANE_BUTTERFLY && (O == 32 || O == 16)) {
+        const uint32_t ua = __builtin_bit_cast(uint32_t, a), ub = __builtin_bit_cast(uint32_t, b);
+        uint32_t na, nb;
+        if constexpr (O == 32) {
+            auto r = __builtin_amdgcn_permlane32_swap(ua, ub, false, false);
+            na = r[0];
+            nb = r[1];
+        } else {
+            auto r = __builtin_amdgcn_permlane16_swap(ua, ub, false, false);
+            na = r[0];
+            nb = r[1];
+        }
+        return up ? b + __builtin_bit_cast(float, na) : a + __builtin_bit_cast(float, nb);
+    } else {
+        const float send = up ? a : b;
+        const float keep = up ? b : a;
+        return keep + xlane<O>(send);
+    }
+}
+
+template <int O>
+__device__ inline double halve_pair(double a, double b, int lane)
+{
+    const bool up = (lane & O) != 0;
+    if constexpr (MG_PERMLANE_BUTTERFLY && (O == 32 || O == 16)) {
+        const uint64_t ua = __builtin_bit_cast(uint64_t, a), ub = __builtin_bit_cast(uint64_t, b);
+        const uint32_t al = (uint32_t)ua, ah = (uint32_t)(ua >> 32), bl = (uint32_t)ub, bh = (uint32_t)(ub >> 32);
+        uint32_t nal, nbl, nah, nbh;
+        if constexpr (O == 32) {
+            auto r0 = __builtin_amdgcn_permlane32_swap(al, bl, false, false);
+            auto r1 = __builtin_amdgcn_permlane32_swap(ah, bh, false, false);
+            nal = r0[0]; nbl = r0[1]; nah = r1[0]; nbh = r1[1];
+        } else {
+            auto r0 = __builtin_amdgcn_permlane16_swap(al, bl, false, false);
+            auto r1 = __builtin_amdgcn_permlane16_swap(ah, bh, false, false);
+            nal = r0[0]; nbl = r0[1]; nah = r1[0]; nbh = r1[1];
+        }
+        const double na = __builtin_bit_cast(double, (uint64_t)nal | ((uint64_t)nah << 32));
+        const double nb = __builtin_bit_cast(double, (uint64_t)nbl | ((uint64_t)nbh << 32));
+        return up ? b + na : a + nb;
+    } else {
+        const double send = up ? a : b;
+        const double keep = up ? b : a;
+        return keep + xlane<O>(send);
+    }
+}
+
+// Transposing butterfly: on return lane l holds the 64-lane sum of value
+// (l >> (6 - log2 N)); N values per lane, T = float or double.
+template <int N, int O, typename T>
+__device__ inline void wave_halve(T* v, int lane)
+{
+    if constexpr (N > 1) {
+        constexpr int H = N / 2;
+#pragma unroll
+        for (int k = 0; k < H; k++) v[k] = halve_pair<O>(v[k], v[k + H], lane);
+        wave_halve<H, O / 2, T>(v, lane);
+    } else if constexpr (O >= 1) {
+        v[0] += xlane<O>(v[0]);
+        wave_halve<1, O / 2, T>(v, lane);
+    }
+}
+
+// broadcast a value that is uniform across the wave into scalar registers
+__device__ inline float uniform(float x)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, x)));
+}
+__device__ inline double uniform(double x)
+{
+    const uint64_t u = __builtin_bit_cast(uint64_t, x);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)u);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(u >> 32));
+    return __builtin_bit_cast(double, (uint64_t)lo | ((uint64_t)hi << 32));
+}
+
+// n gate coefficients (wave-uniform) loaded into scalar registers
+template <typename R, int N>
+__device__ inline void load_coef(const cpx<R>* src, cpx<R>* dst)
+{
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+        // measured on MI355X: scalar coefficients relieve the FP64 kernels' VGPR pressure but
+        // slow the FP32 kernels, which have VGPRs to spare
+        if constexpr (MG_SGPR_COEF && sizeof(R) == 8) dst[k] = {uniform(src[k].re), uniform(src[k].im)};
+        else dst[k] = src[k];
+    }
+}
+
+template <int N>
+constexpr int log2c() { return N <= 1 ? 0 : 1 + log2c<N / 2>(); }
+
+__device__ inline double wave_sum(double x)
+{
+    x += xlane<32>(x);
+    x += xlane<16>(x);
+    x += xlane<8>(x);
+    x += xlane<4>(x);
+    x += xlane<2>(x);
+    x += xlane<1>(x);
+    return x;
+}
+
+// ---- quadruple kernels -----------------------------------------------------------
+template <typename R>
+struct Quad {
+    cpx<R> x, y, z, w;
+};
+
+// LDS position of amplitude a.  A wave's LDS access is served in groups of lanes
+// that together cover all banks once (MG_LDS_BANK_BYTES per cycle): 8-byte (FP32
+// complex) elements fill a group with BANK/8 lanes, 16-byte (FP64 complex) ones with
+// BANK/16.  A quadruple access with a low wire bit keeps one of the group's index
+// bits constant across the lanes and would collide.  XOR-ing the two index bits just
+// above the group (bits B, B+1) into the low B bits with the masks below makes every
+// nearest-neighbour pair (k, k+1) conflict-free: consecutive rows of the 2-column
+// mask matrix are linearly independent and row B-1 has its second bit set.  Only
+// the low B bits change, so the map permutes every aligned 2^(B+2)-element block.
+#ifndef MG_LDS_BANK_BYTES
+#define MG_LDS_BANK_BYTES 128
+#endif
+template <int B>
+struct SwzMasks;
+template <>
+struct SwzMasks<3> { static constexpr uint32_t m0 = 5u, m1 = 6u; };
+template <>
+struct SwzMasks<4> { static constexpr uint32_t m0 = 5u, m1 = 10u; };
+template <>
+struct SwzMasks<5> { static constexpr uint32_t m0 = 21u, m1 = 26u; };
+
+template <typename R>
+__device__ inline uint32_t lds_swz(uint32_t a)
+{
+    constexpr int lanes = MG_LDS_BANK_BYTES / (int)sizeof(cpx<R>);
+    constexpr int B = lanes == 8 ? 3 : (lanes == 16 ? 4 : 5);
+    return a ^ (((a >> B) & 1u) * SwzMasks<B>::m0) ^ (((a >> (B + 1)) & 1u) * SwzMasks<B>::m1);
+}
+
+// SWZ: the state lives in LDS with the lds_swz layout (else plain global memory)
+template <typename R, bool SWZ = false>
+__device__ inline cpx<R>& sel(cpx<R>* s, uint32_t a)
+{
+    if constexpr (SWZ) return s[lds_swz<R>(a)];
+    else return s[a];
+}
+
+template <typename R, bool SWZ = false>
+__device__ inline const cpx<R>& sel(const cpx<R>* s, uint32_t a)
+{
+    if constexpr (SWZ) return s[lds_swz<R>(a)];
+    else return s[a];
+}
+
+template <typename R, bool SWZ = false>
+__device__ inline Quad<R> load_quad(const cpx<R>* s, uint32_t b, int lo, int hi)
+{
+    return {sel<R, SWZ>(s, b), sel<R, SWZ>(s, b | (1u << lo)), sel<R, SWZ>(s, b | (1u << hi)),
+            sel<R, SWZ>(s, b | (1u << lo) | (1u << hi))};
+}
+
+template <typename R, bool SWZ = false>
+__device__ inline void store_quad(cpx<R>* s, uint32_t b, int lo, int hi, const Quad<R>& q)
+{
+    sel<R, SWZ>(s, b) = q.x;
+    sel<R, SWZ>(s, b | (1u << lo)) = q.y;
+    sel<R, SWZ>(s, b | (1u << hi)) = q.z;
+    sel<R, SWZ>(s, b | (1u << lo) | (1u << hi)) = q.w;
+}
+
+// V q  (reference src/gate.c:641-646)
+template <typename R>
+__device__ inline Quad<R> gapply(const cpx<R>* g, const Quad<R>& q)
+{
+    return {cfma(g[4], q.x, cmul(g[5], q.w)), cfma(g[0], q.y, cmul(g[1], q.z)),
+            cfma(g[2], q.y, cmul(g[3], q.z)), cfma(g[6], q.x, cmul(g[7], q.w))};
+}
+
+// V q + acc
+template <typename R>
+__device__ inline Quad<R> gapply_add(const cpx<R>* g, const Quad<R>& q, const Quad<R>& a)
+{
+    return {cfma(g[4], q.x, cfma(g[5], q.w, a.x)), cfma(g[0], q.y, cfma(g[1], q.z, a.y)),
+            cfma(g[2], q.y, cfma(g[3], q.z, a.z)), cfma(g[6], q.x, cfma(g[7], q.w, a.w))};
+}
+
+// V^T q  (reference src/gate.c:760-764)
+template <typename R>
+__device__ inline Quad<R> gapplyT(const cpx<R>* g, const Quad<R>& q)
+{
+    return {cfma(g[4], q.x, cmul(g[6], q.w)), cfma(g[0], q.y, cmul(g[2], q.z)),
+            cfma(g[1], q.y, cmul(g[3], q.z)), cfma(g[5], q.x, cmul(g[7], q.w))};
+}
+
+template <typename R>
+__device__ inline Quad<R> gapplyT_add(const cpx<R>* g, const Quad<R>& q, const Quad<R>& a)
+{
+    return {cfma(g[4], q.x, cfma(g[6], q.w, a.x)), cfma(g[0], q.y, cfma(g[2], q.z, a.y)),
+            cfma(g[1], q.y, cfma(g[3], q.z, a.z)), cfma(g[5], q.x, cfma(g[7], q.w, a.w))};
+}
+
+// conj(a) b + c
+template <typename R>
+__host__ __device__ inline cpx<R> cfma_conj(cpx<R> a, cpx<R> b, cpx<R> c)
+{
+    return {rfma(a.re, b.re, rfma(a.im, b.im, c.re)), rfma(a.re, b.im, rfma(-a.im, b.re, c.im))};
+}
+
+template <typename R>
+__host__ __device__ inline cpx<R> cmul_conj(cpx<R> a, cpx<R> b)
+{
+    return {a.re * b.re + a.im * b.im, a.re * b.im - a.im * b.re};
+}
+
+// V^dagger q = conj(V)^T q: the inverse of a unitary V
+template <typename R>
+__device__ inline Quad<R> gapplyH(const cpx<R>* g, const Quad<R>& q)
+{
+    return {cfma_conj(g[4], q.x, cmul_conj(g[6], q.w)), cfma_conj(g[0], q.y, cmul_conj(g[2], q.z)),
+            cfma_conj(g[1], q.y, cmul_conj(g[3], q.z)), cfma_conj(g[5], q.x, cmul_conj(g[7], q.w))};
+}
+
+template <typename R>
+__device__ inline Quad<R> qsub(const Quad<R>& a, const Quad<R>& b)
+{
+    return {csub(a.x, b.x), csub(a.y, b.y), csub(a.z, b.z), csub(a.w, b.w)};
+}
+
+// acc[0..7] += d (x) p  (reference src/gate.c:753-757): center then corner entries
+template <typename R>
+__device__ inline void outer_acc(cpx<R>* acc, const Quad<R>& d, const Quad<R>& p)
+{
+    acc[0] = cfma(d.y, p.y, acc[0]);
+    acc[1] = cfma(d.y, p.z, acc[1]);
+    acc[2] = cfma(d.z, p.y, acc[2]);
+    acc[3] = cfma(d.z, p.z, acc[3]);
+    acc[4] = cfma(d.x, p.x, acc[4]);
+    acc[5] = cfma(d.x, p.w, acc[5]);
+    acc[6] = cfma(d.w, p.x, acc[6]);
+    acc[7] = cfma(d.w, p.w, acc[7]);
+}
+
+}  // namespace mg

@@ -296,7 +296,16 @@ static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, in
     hipStream_t s = c->s;
     const int L = p.L;
     const size_t S = (size_t)1 << L;
-    const int NW = kSweepThreads / 64;
+    const Knobs& kn = knobs();
+    // threads of the LDS/global kernel: more waves per CU when one workgroup fills the LDS
+    int lds_T = kn.lds_threads;
+    {
+        // 512 threads when the states fill the LDS (one workgroup per CU), else 256
+        const int nst0 = (mode == 2 ? 4 : 2);
+        const bool fits = sweep_lds_head_bytes(p.P, 8) + (size_t)nst0 * S * sizeof(cpx<R>) <= c->lds_limit;
+        if (lds_T == 0) lds_T = fits && S >= 2048 ? 512 : 256;
+    }
+    const int NW = lds_T / 64;
 
     // coefficients in precision R
     std::vector<cpx<R>> coefR(b.coef.size());
@@ -307,7 +316,6 @@ static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, in
     MG_HIP(hipMemcpyAsync(c->coef.p, coefR.data(), coefR.size() * sizeof(cpx<R>), hipMemcpyHostToDevice, s));
 
     // checkpointing: needed when psi_g cannot be recovered by V^{-1} (non-unitary gates)
-    const Knobs& kn = knobs();
     bool ckpt = false;
     if (mode >= 1) {
         if (kn.force_ckpt == 1) ckpt = true;
@@ -320,14 +328,41 @@ static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, in
     if (head > c->lds_limit) { set_error("too many parameter gates for the LDS accumulator"); return -1; }
     bool lds_states = head + state_bytes <= c->lds_limit;
     if (kn.force_lds == 0) lds_states = false;
-    const size_t lds = lds_states ? head + state_bytes : head;
+
+    // register-resident kernel: states in VGPRs, LDS as the re-layout buffer; needs gates
+    // unitary to 1e-12 (it uncomputes with V^dagger) and threads x quads = 2^L / 4
+    int reg_T = 0, reg_qpt = 0;
+    if (!ckpt && b.defect <= 1e-12 && kn.force_reg != 0 && (!lds_states || kn.force_reg == 1)) {
+        const size_t Q = S / 4;
+        const int prefT = kn.reg_threads ? kn.reg_threads : (Q <= 1024 ? 512 : 1024);
+        for (int T : {prefT, 512, 1024}) {
+            if (Q % T) continue;
+            const size_t qpt = Q / T;
+            if ((T == 512 && (qpt == 1 || qpt == 2)) || (T == 1024 && (qpt == 1 || qpt == 2))) {
+                reg_T = T;
+                reg_qpt = (int)qpt;
+                break;
+            }
+        }
+        if (reg_T && head + S * sizeof(cpx<R>) > c->lds_limit) reg_T = 0;
+    }
+    const int nw_reg = reg_T / 64;
+    const size_t head_reg = reg_T ? sweep_lds_head_bytes(p.P, nw_reg) : 0;
+    int reg_nst = 0;
+    if (reg_T) {
+        reg_nst = (int)std::min<size_t>(mode == 2 ? 4 : 2, (c->lds_limit - head_reg) / (S * sizeof(cpx<R>)));
+        if (reg_nst == 3) reg_nst = 2;  // instantiated: 1, 2, 4 states per round
+        if (reg_nst < 1) reg_T = 0;
+    }
+    const size_t lds = reg_T ? head_reg + (size_t)reg_nst * S * sizeof(cpx<R>) : (lds_states ? head + state_bytes : head);
 
     const long ncols = b1 - b0;
     int per_cu = kn.grid_per_cu > 0 ? kn.grid_per_cu : std::max<int>(1, std::min<size_t>(4, (160 * 1024) / std::max<size_t>(lds, 1)));
+    if (reg_T && kn.grid_per_cu <= 0) per_cu = std::max<int>(1, std::min<size_t>(2, (160 * 1024) / lds));
     long grid = std::min<long>(ncols, (long)c->ncu * per_cu);
     // bound the global scratch and checkpoint footprint (2 GiB each)
     const size_t lim = (size_t)2 << 30;
-    if (!lds_states) grid = std::min<long>(grid, std::max<long>(1, lim / state_bytes));
+    if (!lds_states && !reg_T) grid = std::min<long>(grid, std::max<long>(1, lim / state_bytes));
     const size_t ck_per = ckpt ? (size_t)2 * (p.G + 1) * S * sizeof(cpx<R>) : 0;
     if (ckpt) grid = std::min<long>(grid, std::max<long>(1, lim / ck_per));
     grid = std::max<long>(grid, 1);
@@ -335,7 +370,7 @@ static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, in
     const int stride = 2 + 32 * p.P;
     MG_HIP(c->partial.ensure((size_t)grid * stride * sizeof(double)));
     MG_HIP(c->out.ensure((size_t)stride * sizeof(double)));
-    if (!lds_states) MG_HIP(c->scratch.ensure((size_t)grid * state_bytes));
+    if (!lds_states && !reg_T) MG_HIP(c->scratch.ensure((size_t)grid * state_bytes));
     if (ckpt) MG_HIP(c->ckpt.ensure((size_t)grid * ck_per));
 
     SweepArgs<R> a{};
@@ -357,11 +392,14 @@ static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, in
     a.partial = (double*)c->partial.p;
     a.scratch = (cpx<R>*)c->scratch.p;
     a.ckpt = (cpx<R>*)c->ckpt.p;
-    a.nstate = nstate;
-    a.lds_state_offset = (int)head;
+    a.nstate = reg_T ? reg_nst : nstate;
+    a.lds_state_offset = (int)(reg_T ? head_reg : head);
 
     if (kn.time_kernels) MG_HIP(hipEventRecord(c->ev0, s));
-    MG_HIP(launch_sweep<R>(a, mode, lds_states, ckpt, (int)grid, lds, s));
+    if (reg_T)
+        MG_HIP(launch_sweep_reg<R>(a, mode, reg_T, reg_qpt, (int)grid, lds, s));
+    else
+        MG_HIP(launch_sweep<R>(a, mode, lds_states, ckpt, lds_T, (int)grid, lds, s));
     if (kn.time_kernels) MG_HIP(hipEventRecord(c->ev1, s));
     MG_HIP(launch_reduce((const double*)c->partial.p, (int)grid, stride, (double*)c->out.p, s));
     c->out_h.resize(stride);
@@ -378,8 +416,10 @@ static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, in
             out->dhz[8 * q + e] = zd(o[2 + 32 * q + 16 + 2 * e], o[2 + 32 * q + 16 + 2 * e + 1]);
         }
     if (st) {
-        st->lds_states = lds_states;
+        st->lds_states = lds_states && !reg_T;
         st->ckpt = ckpt;
+        st->reg_threads = reg_T;
+        st->threads = reg_T ? reg_T : lds_T;
         st->grid = (int)grid;
         st->lds_bytes = lds;
         st->kernel_ms = 0.f;

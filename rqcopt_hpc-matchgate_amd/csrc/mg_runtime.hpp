@@ -68,6 +68,8 @@ struct EvalOut {
 
 struct EvalStats {
     bool lds_states = false, ckpt = false;
+    int reg_threads = 0;  // > 0: register-resident kernel with this many threads per workgroup
+    int threads = 0;      // threads per workgroup of the launched sweep
     int grid = 0;
     size_t lds_bytes = 0;
     float kernel_ms = 0.f;  // sweep kernel duration (HIP events on the launch stream)
@@ -82,6 +84,9 @@ struct Knobs {
     int force_ckpt = -1;      // -1 auto, 0 never, 1 always (when mode >= 1)
     int force_lds = -1;       // -1 auto, 0 global-scratch states, 1 LDS states (if they fit)
     int grid_per_cu = 0;      // 0 auto
+    int force_reg = -1;       // -1 auto, 0 never, 1 whenever a register-resident configuration exists
+    int reg_threads = 0;      // 0 auto, else 512 or 1024 threads for the register-resident kernel
+    int lds_threads = 0;      // 0 auto, else 256, 512 or 1024 threads for the LDS/global kernel
     bool time_kernels = false;
 };
 Knobs& knobs();

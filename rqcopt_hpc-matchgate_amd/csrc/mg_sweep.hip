@@ -18,112 +18,10 @@
 // Workgroup partials are summed in block order by reduce_partials: results are
 // bitwise reproducible run to run.
 #include "mg_device.hpp"
+#include "mg_quad.hpp"
 #include "mg_sweep.hpp"
 
 namespace mg {
-
-// ---- wave-level transposing reduction -------------------------------------------
-// On return lane l holds the 64-lane sum of value (l >> (6 - log2 N)).
-template <int N, int O>
-__device__ inline void wave_halve(double* v, int lane)
-{
-    if constexpr (N > 1) {
-        constexpr int H = N / 2;
-        const bool up = (lane & O) != 0;
-#pragma unroll
-        for (int k = 0; k < H; k++) {
-            const double send = up ? v[k] : v[k + H];
-            const double keep = up ? v[k + H] : v[k];
-            v[k] = keep + __shfl_xor(send, O);
-        }
-        wave_halve<H, O / 2>(v, lane);
-    } else {
-#pragma unroll
-        for (int o = O; o >= 1; o >>= 1) v[0] += __shfl_xor(v[0], o);
-    }
-}
-
-template <int N>
-constexpr int log2c() { return N <= 1 ? 0 : 1 + log2c<N / 2>(); }
-
-__device__ inline double wave_sum(double x)
-{
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) x += __shfl_xor(x, o);
-    return x;
-}
-
-// ---- quadruple kernels -----------------------------------------------------------
-template <typename R>
-struct Quad {
-    cpx<R> x, y, z, w;
-};
-
-template <typename R>
-__device__ inline Quad<R> load_quad(const cpx<R>* s, uint32_t b, int lo, int hi)
-{
-    return {s[b], s[b | (1u << lo)], s[b | (1u << hi)], s[b | (1u << lo) | (1u << hi)]};
-}
-
-template <typename R>
-__device__ inline void store_quad(cpx<R>* s, uint32_t b, int lo, int hi, const Quad<R>& q)
-{
-    s[b] = q.x;
-    s[b | (1u << lo)] = q.y;
-    s[b | (1u << hi)] = q.z;
-    s[b | (1u << lo) | (1u << hi)] = q.w;
-}
-
-// V q  (reference src/gate.c:641-646)
-template <typename R>
-__device__ inline Quad<R> gapply(const cpx<R>* g, const Quad<R>& q)
-{
-    return {cfma(g[4], q.x, cmul(g[5], q.w)), cfma(g[0], q.y, cmul(g[1], q.z)),
-            cfma(g[2], q.y, cmul(g[3], q.z)), cfma(g[6], q.x, cmul(g[7], q.w))};
-}
-
-// V q + acc
-template <typename R>
-__device__ inline Quad<R> gapply_add(const cpx<R>* g, const Quad<R>& q, const Quad<R>& a)
-{
-    return {cfma(g[4], q.x, cfma(g[5], q.w, a.x)), cfma(g[0], q.y, cfma(g[1], q.z, a.y)),
-            cfma(g[2], q.y, cfma(g[3], q.z, a.z)), cfma(g[6], q.x, cfma(g[7], q.w, a.w))};
-}
-
-// V^T q  (reference src/gate.c:760-764)
-template <typename R>
-__device__ inline Quad<R> gapplyT(const cpx<R>* g, const Quad<R>& q)
-{
-    return {cfma(g[4], q.x, cmul(g[6], q.w)), cfma(g[0], q.y, cmul(g[2], q.z)),
-            cfma(g[1], q.y, cmul(g[3], q.z)), cfma(g[5], q.x, cmul(g[7], q.w))};
-}
-
-template <typename R>
-__device__ inline Quad<R> gapplyT_add(const cpx<R>* g, const Quad<R>& q, const Quad<R>& a)
-{
-    return {cfma(g[4], q.x, cfma(g[6], q.w, a.x)), cfma(g[0], q.y, cfma(g[2], q.z, a.y)),
-            cfma(g[1], q.y, cfma(g[3], q.z, a.z)), cfma(g[5], q.x, cfma(g[7], q.w, a.w))};
-}
-
-template <typename R>
-__device__ inline Quad<R> qsub(const Quad<R>& a, const Quad<R>& b)
-{
-    return {csub(a.x, b.x), csub(a.y, b.y), csub(a.z, b.z), csub(a.w, b.w)};
-}
-
-// acc[0..7] += d (x) p  (reference src/gate.c:753-757): center then corner entries
-template <typename R>
-__device__ inline void outer_acc(cpx<R>* acc, const Quad<R>& d, const Quad<R>& p)
-{
-    acc[0] = cfma(d.y, p.y, acc[0]);
-    acc[1] = cfma(d.y, p.z, acc[1]);
-    acc[2] = cfma(d.z, p.y, acc[2]);
-    acc[3] = cfma(d.z, p.z, acc[3]);
-    acc[4] = cfma(d.x, p.x, acc[4]);
-    acc[5] = cfma(d.x, p.w, acc[5]);
-    acc[6] = cfma(d.w, p.x, acc[6]);
-    acc[7] = cfma(d.w, p.w, acc[7]);
-}
 
 // ---- the sweep kernel ----------------------------------------------------------------
 // MODE 0: f;  MODE 1: f + gradient;  MODE 2: f + gradient + D^2 f[Z]
@@ -165,44 +63,50 @@ __global__ __launch_bounds__(T) void sweep_kernel(SweepArgs<R> a)
         // ---- seed: lam = -conj(U e_b) --------------------------------------------------
         if (a.target_kind == 0) {
             const cpx<R>* row = a.seed + (size_t)b * S;
-            for (uint32_t i = tid; i < S; i += T) lam[i] = row[i];
+            for (uint32_t i = tid; i < S; i += T) sel<R, LDS>(lam, i) = row[i];
         } else {
-            for (uint32_t i = tid; i < S; i += T) lam[i] = {R(i == (uint32_t)b), R(0)};
+            for (uint32_t i = tid; i < S; i += T) sel<R, LDS>(lam, i) = {R(i == (uint32_t)b), R(0)};
             __syncthreads();
             for (int g = 0; g < a.Gt; g++) {
                 const Inst in = a.tinst[g];
-                const cpx<R>* V = a.tcoef + (size_t)g * kCoefPerInst;
+                cpx<R> V[8];
+                load_coef<R, 8>(a.tcoef + (size_t)g * kCoefPerInst, V);
                 for (uint32_t q = tid; q < Q; q += T) {
                     const uint32_t base = quad_base(q, in.lo, in.hi);
-                    store_quad(lam, base, in.lo, in.hi, gapply(V, load_quad(lam, base, in.lo, in.hi)));
+                    store_quad<R, LDS>(lam, base, in.lo, in.hi, gapply(V, load_quad<R, LDS>(lam, base, in.lo, in.hi)));
                 }
                 __syncthreads();
             }
-            for (uint32_t i = tid; i < S; i += T) lam[i] = {-lam[i].re, lam[i].im};
+            for (uint32_t i = tid; i < S; i += T) {
+                cpx<R>& v = sel<R, LDS>(lam, i);
+                v = {-v.re, v.im};
+            }
         }
         // ---- forward sweep ---------------------------------------------------------------
         cpx<R>* p0 = CKPT ? ck : psi;
         cpx<R>* d0 = CKPT ? ck + S : dpsi;
         for (uint32_t i = tid; i < S; i += T) {
-            p0[i] = {R(i == (uint32_t)b), R(0)};
-            if constexpr (MODE == 2) d0[i] = {R(0), R(0)};
+            sel<R, LDS && !CKPT>(p0, i) = {R(i == (uint32_t)b), R(0)};
+            if constexpr (MODE == 2) sel<R, LDS && !CKPT>(d0, i) = {R(0), R(0)};
         }
         __syncthreads();
         for (int g = 0; g < a.G; g++) {
             const Inst in = a.inst[g];
-            const cpx<R>* cf = a.coef + (size_t)g * kCoefPerInst;
+            cpx<R> cf[24];
+            load_coef<R, 8>(a.coef + (size_t)g * kCoefPerInst, cf);
+            if constexpr (MODE == 2) load_coef<R, 8>(a.coef + (size_t)g * kCoefPerInst + 16, cf + 16);
             const cpx<R>* pin = CKPT ? ck + (size_t)2 * g * S : psi;
             cpx<R>* pout = CKPT ? ck + (size_t)2 * (g + 1) * S : psi;
             const cpx<R>* din = CKPT ? ck + (size_t)(2 * g + 1) * S : dpsi;
             cpx<R>* dout = CKPT ? ck + (size_t)(2 * g + 3) * S : dpsi;
             for (uint32_t q = tid; q < Q; q += T) {
                 const uint32_t base = quad_base(q, in.lo, in.hi);
-                const Quad<R> p = load_quad(pin, base, in.lo, in.hi);
+                const Quad<R> p = load_quad<R, LDS && !CKPT>(pin, base, in.lo, in.hi);
                 if constexpr (MODE == 2) {
-                    const Quad<R> d = load_quad(din, base, in.lo, in.hi);
-                    store_quad(dout, base, in.lo, in.hi, gapply_add(cf, d, gapply(cf + 16, p)));
+                    const Quad<R> d = load_quad<R, LDS && !CKPT>(din, base, in.lo, in.hi);
+                    store_quad<R, LDS && !CKPT>(dout, base, in.lo, in.hi, gapply_add(cf, d, gapply(cf + 16, p)));
                 }
-                store_quad(pout, base, in.lo, in.hi, gapply(cf, p));
+                store_quad<R, LDS && !CKPT>(pout, base, in.lo, in.hi, gapply(cf, p));
             }
             __syncthreads();
         }
@@ -211,7 +115,7 @@ __global__ __launch_bounds__(T) void sweep_kernel(SweepArgs<R> a)
             const cpx<R>* pG = CKPT ? ck + (size_t)2 * a.G * S : psi;
             double sr = 0.0, si = 0.0;
             for (uint32_t i = tid; i < S; i += T) {
-                const cpx<R> t = cmul(lam[i], pG[i]);
+                const cpx<R> t = cmul(sel<R, LDS>(lam, i), sel<R, LDS && !CKPT>(pG, i));
                 sr += (double)t.re;
                 si += (double)t.im;
             }
@@ -235,12 +139,13 @@ __global__ __launch_bounds__(T) void sweep_kernel(SweepArgs<R> a)
         if constexpr (MODE >= 1) {
             // ---- reverse sweep ---------------------------------------------------------
             if constexpr (MODE == 2) {
-                for (uint32_t i = tid; i < S; i += T) dlam[i] = {R(0), R(0)};
+                for (uint32_t i = tid; i < S; i += T) sel<R, LDS>(dlam, i) = {R(0), R(0)};
             }
             __syncthreads();
             for (int g = a.G - 1; g >= 0; g--) {
                 const Inst in = a.inst[g];
-                const cpx<R>* cf = a.coef + (size_t)g * kCoefPerInst;
+                cpx<R> cf[24];
+                load_coef<R, MODE == 2 ? 24 : 16>(a.coef + (size_t)g * kCoefPerInst, cf);
                 cpx<R> ga[8], ha[8];
 #pragma unroll
                 for (int k = 0; k < 8; k++) {
@@ -249,42 +154,43 @@ __global__ __launch_bounds__(T) void sweep_kernel(SweepArgs<R> a)
                 }
                 for (uint32_t q = tid; q < Q; q += T) {
                     const uint32_t base = quad_base(q, in.lo, in.hi);
-                    const Quad<R> l = load_quad(lam, base, in.lo, in.hi);
+                    const Quad<R> l = load_quad<R, LDS>(lam, base, in.lo, in.hi);
                     Quad<R> p, dp, dl;
                     if constexpr (CKPT) {
                         p = load_quad(ck + (size_t)2 * g * S, base, in.lo, in.hi);
                         if constexpr (MODE == 2) dp = load_quad(ck + (size_t)(2 * g + 1) * S, base, in.lo, in.hi);
                     } else {
-                        p = gapply(cf + 8, load_quad(psi, base, in.lo, in.hi));
-                        store_quad(psi, base, in.lo, in.hi, p);
+                        p = gapply(cf + 8, load_quad<R, LDS>(psi, base, in.lo, in.hi));
+                        store_quad<R, LDS>(psi, base, in.lo, in.hi, p);
                         if constexpr (MODE == 2) {
-                            const Quad<R> dn = load_quad(dpsi, base, in.lo, in.hi);
+                            const Quad<R> dn = load_quad<R, LDS>(dpsi, base, in.lo, in.hi);
                             dp = gapply(cf + 8, qsub(dn, gapply(cf + 16, p)));
-                            store_quad(dpsi, base, in.lo, in.hi, dp);
+                            store_quad<R, LDS>(dpsi, base, in.lo, in.hi, dp);
                         }
                     }
                     outer_acc(ga, l, p);
                     if constexpr (MODE == 2) {
-                        dl = load_quad(dlam, base, in.lo, in.hi);
+                        dl = load_quad<R, LDS>(dlam, base, in.lo, in.hi);
                         outer_acc(ha, dl, p);
                         outer_acc(ha, l, dp);
-                        store_quad(dlam, base, in.lo, in.hi, gapplyT_add(cf, dl, gapplyT(cf + 16, l)));
+                        store_quad<R, LDS>(dlam, base, in.lo, in.hi, gapplyT_add(cf, dl, gapplyT(cf + 16, l)));
                     }
-                    store_quad(lam, base, in.lo, in.hi, gapplyT(cf, l));
+                    store_quad<R, LDS>(lam, base, in.lo, in.hi, gapplyT(cf, l));
                 }
-                double v[NV];
+                // lane sums in the sweep precision, then FP64 across waves, columns and workgroups
+                R v[NV];
 #pragma unroll
                 for (int k = 0; k < 8; k++) {
-                    v[2 * k] = (double)ga[k].re;
-                    v[2 * k + 1] = (double)ga[k].im;
+                    v[2 * k] = ga[k].re;
+                    v[2 * k + 1] = ga[k].im;
                     if constexpr (MODE == 2) {
-                        v[16 + 2 * k] = (double)ha[k].re;
-                        v[16 + 2 * k + 1] = (double)ha[k].im;
+                        v[16 + 2 * k] = ha[k].re;
+                        v[16 + 2 * k + 1] = ha[k].im;
                     }
                 }
-                wave_halve<NV, 32>(v, lane);
+                wave_halve<NV, 32, R>(v, lane);
                 const int buf = g & 1;
-                if ((lane & ((1 << SHIFT) - 1)) == 0) stage[(buf * NW + wv) * NV + (lane >> SHIFT)] = v[0];
+                if ((lane & ((1 << SHIFT) - 1)) == 0) stage[(buf * NW + wv) * NV + (lane >> SHIFT)] = (double)v[0];
                 __syncthreads();
                 if (wv == g % NW && lane < NV) {
                     double s = 0.0;
@@ -328,10 +234,9 @@ __global__ __launch_bounds__(1024) void reduce_partials(const double* __restrict
 }
 
 // ---- host-side launch ----------------------------------------------------------------
-template <typename R, int MODE, bool LDS, bool CKPT>
+template <typename R, int MODE, bool LDS, bool CKPT, int T>
 static hipError_t launch_t(const SweepArgs<R>& a, int grid, size_t lds, hipStream_t s)
 {
-    constexpr int T = 256;
     auto k = sweep_kernel<R, MODE, LDS, CKPT, T>;
     if (lds > 65536) {
         hipError_t e = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -341,11 +246,11 @@ static hipError_t launch_t(const SweepArgs<R>& a, int grid, size_t lds, hipStrea
     return hipGetLastError();
 }
 
-template <typename R>
-hipError_t launch_sweep(const SweepArgs<R>& a, int mode, bool lds_states, bool ckpt, int grid, size_t lds, hipStream_t s)
+template <typename R, int T>
+static hipError_t launch_T(const SweepArgs<R>& a, int mode, bool lds_states, bool ckpt, int grid, size_t lds, hipStream_t s)
 {
 #define MG_CASE(M, LD, CK) \
-    if (mode == M && lds_states == LD && ckpt == CK) return launch_t<R, M, LD, CK>(a, grid, lds, s);
+    if (mode == M && lds_states == LD && ckpt == CK) return launch_t<R, M, LD, CK, T>(a, grid, lds, s);
     MG_CASE(0, true, false) MG_CASE(0, false, false)
     MG_CASE(1, true, false) MG_CASE(1, false, false) MG_CASE(1, true, true) MG_CASE(1, false, true)
     MG_CASE(2, true, false) MG_CASE(2, false, false) MG_CASE(2, true, true) MG_CASE(2, false, true)
@@ -353,8 +258,18 @@ hipError_t launch_sweep(const SweepArgs<R>& a, int mode, bool lds_states, bool c
     return hipErrorInvalidValue;
 }
 
-template hipError_t launch_sweep<double>(const SweepArgs<double>&, int, bool, bool, int, size_t, hipStream_t);
-template hipError_t launch_sweep<float>(const SweepArgs<float>&, int, bool, bool, int, size_t, hipStream_t);
+template <typename R>
+hipError_t launch_sweep(const SweepArgs<R>& a, int mode, bool lds_states, bool ckpt, int threads, int grid, size_t lds,
+                        hipStream_t s)
+{
+    if (threads == 256) return launch_T<R, 256>(a, mode, lds_states, ckpt, grid, lds, s);
+    if (threads == 512) return launch_T<R, 512>(a, mode, lds_states, ckpt, grid, lds, s);
+    if (threads == 1024) return launch_T<R, 1024>(a, mode, lds_states, ckpt, grid, lds, s);
+    return hipErrorInvalidValue;
+}
+
+template hipError_t launch_sweep<double>(const SweepArgs<double>&, int, bool, bool, int, int, size_t, hipStream_t);
+template hipError_t launch_sweep<float>(const SweepArgs<float>&, int, bool, bool, int, int, size_t, hipStream_t);
 
 hipError_t launch_reduce(const double* partial, int nblk, int stride, double* out, hipStream_t s)
 {

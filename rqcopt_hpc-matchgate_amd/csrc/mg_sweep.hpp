@@ -32,10 +32,15 @@ inline size_t sweep_lds_head_bytes(int P, int nwaves)
 }
 
 template <typename R>
-hipError_t launch_sweep(const SweepArgs<R>& a, int mode, bool lds_states, bool ckpt, int grid, size_t lds, hipStream_t s);
+hipError_t launch_sweep(const SweepArgs<R>& a, int mode, bool lds_states, bool ckpt, int threads, int grid, size_t lds,
+                        hipStream_t s);
+
+// register-resident variant (mg_sweep_reg.hip): threads x qpt quadruples = 2^L / 4;
+// a.nstate = states per LDS re-layout round
+template <typename R>
+hipError_t launch_sweep_reg(const SweepArgs<R>& a, int mode, int threads, int qpt, int grid, size_t lds, hipStream_t s);
 
 hipError_t launch_reduce(const double* partial, int nblk, int stride, double* out, hipStream_t s);
 
-constexpr int kSweepThreads = 256;
 
 }  // namespace mg

@@ -14,7 +14,7 @@ import os
 import numpy as np
 
 _DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB_PATH = os.path.join(_DIR, "lib", "librqcopt_mg.so")
+LIB_PATH = os.environ.get("RQCOPT_MG_LIB") or os.path.join(_DIR, "lib", "librqcopt_mg.so")
 
 if not os.path.exists(LIB_PATH):
     raise ImportError(f"{LIB_PATH} is missing: build it with `make -C {_DIR}` (or __graft_entry__.build())")
@@ -86,6 +86,10 @@ _sig("mg_optimize", [_P, _P, _P, ctypes.POINTER(RTRParams), _I, _I, _I, _I, _P, 
 _sig("mg_set_checkpointing", [_I], None)
 _sig("mg_set_lds_states", [_I], None)
 _sig("mg_set_kernel_timing", [_I], None)
+_sig("mg_set_register_kernel", [_I, _I], None)
+_sig("mg_set_grid_per_cu", [_I], None)
+_sig("mg_set_sweep_threads", [_I], None)
+_sig("mg_last_sweep_threads", [])
 _sig("mg_last_kernel_ms", [], ctypes.c_float)
 _sig("mg_last_sweep_info", [_P, _P, _P, _P], None)
 _sig("mg_target_cache_clear", [], None)
@@ -295,6 +299,18 @@ def set_lds_states(mode):
     lib.mg_set_lds_states(int(mode))
 
 
+def set_register_kernel(mode, threads=0):
+    lib.mg_set_register_kernel(int(mode), int(threads))
+
+
+def set_sweep_threads(n):
+    lib.mg_set_sweep_threads(int(n))
+
+
+def set_grid_per_cu(n):
+    lib.mg_set_grid_per_cu(int(n))
+
+
 def set_kernel_timing(on):
     lib.mg_set_kernel_timing(int(on))
 
@@ -307,7 +323,8 @@ def last_sweep_info():
     a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
     d = ctypes.c_long()
     lib.mg_last_sweep_info(ctypes.byref(a), ctypes.byref(b), ctypes.byref(c), ctypes.byref(d))
-    return {"lds_states": bool(a.value), "ckpt": bool(b.value), "grid": c.value, "lds_bytes": d.value}
+    return {"lds_states": bool(a.value), "ckpt": bool(b.value), "grid": c.value, "lds_bytes": d.value,
+            "reg_threads": int(lib.mg_last_sweep_threads())}
 
 
 # ---------------- reference-API helpers ----------------

@@ -183,6 +183,60 @@ def test_evaluate_fp64_matches_oracle(mg, L, layout, depth, ckpt, lds):
     assert rel(dg1, dgo) < 1e-11
 
 
+@pytest.mark.parametrize("L,layout,depth", [(11, "open", 6), (12, "tied", 3)])
+@pytest.mark.parametrize("threads", [512, 1024])
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_register_kernel_matches_oracle(mg, L, layout, depth, threads, prec):
+    """The register-resident kernel (states in VGPRs, LDS re-layouts) on a column subset."""
+    say(f"case: register kernel L={L} {layout} depth={depth} threads={threads} {prec}")
+    if L == 11 and threads == 1024:
+        pytest.skip("2^11/4 quadruples do not split over 1024 threads with an instantiated count")
+    V, Z, param, wires, P, U = build_case(L, layout, depth, seed=L * 7 + depth)
+    t = mg.Target.dense(U)
+    p = mg.Program.create(L, P, param, wires)
+    precision = mg.MG_FP64 if prec == "fp64" else mg.MG_FP32
+    mg.set_register_kernel(1, threads)
+    try:
+        r = mg.evaluate_columns(t, p, V, Z, mg.EVAL_F_GRAD_HVP, precision, 5, 69)
+        info = mg.last_sweep_info()
+        r1 = mg.evaluate_columns(t, p, V, None, mg.EVAL_F_GRAD, precision, 5, 69)
+        r0 = mg.evaluate_columns(t, p, V, None, mg.EVAL_F, precision, 5, 69)
+    finally:
+        mg.set_register_kernel(-1, 0)
+    assert info["reg_threads"] == threads
+    fo, dgo, dho = orc.target_gradient_hvp(U, V, Z, param, wires, L, 5, 69)
+    tol = 1e-11 if prec == "fp64" else 2e-5
+    assert abs(r[0] - fo) / abs(fo) < tol and rel(r[1], dgo) < tol and rel(r[2], dho) < tol
+    assert abs(r1[0] - fo) / abs(fo) < tol and rel(r1[1], dgo) < tol and abs(r0[0] - fo) / abs(fo) < tol
+
+
+def test_register_kernel_brickwall_target(mg):
+    """Register kernel with an in-kernel (matrix-free) brickwall target at L=12."""
+    say("case: register kernel with brickwall target, L=12")
+    rng = np.random.default_rng(21)
+    L = 12
+    up = random_perms(rng, L, 3)
+    Ug = haar_matchgates(rng, 3)
+    pm = random_perms(rng, L, 3)
+    V = haar_matchgates(rng, 3)
+    Z = rng.standard_normal((3, 8)) + 1j * rng.standard_normal((3, 8))
+    p = mg.Program.brickwall(L, pm)
+    t = mg.Target.brickwall(Ug, up, L)
+    mg.set_register_kernel(1, 512)
+    try:
+        r = mg.evaluate_columns(t, p, V, Z, mg.EVAL_F_GRAD_HVP, mg.MG_FP64, 0, 40)
+    finally:
+        mg.set_register_kernel(-1, 0)
+    tp, tw = orc.brickwall_program(L, up)
+    S = 1 << L
+    Ud = np.zeros((S, S), np.complex128)
+    for b in range(40):
+        Ud[:, b] = orc.apply_program(Ug, tp, tw, L, np.eye(1, S, b).ravel())
+    param, wires = orc.brickwall_program(L, pm)
+    fo, dgo, dho = orc.target_gradient_hvp(Ud, V, Z, param, wires, L, 0, 40)
+    assert abs(r[0] - fo) / abs(fo) < 1e-11 and rel(r[1], dgo) < 1e-11 and rel(r[2], dho) < 1e-11
+
+
 @pytest.mark.parametrize("L,layout,depth", CASES)
 def test_evaluate_fp32_matches_oracle(mg, L, layout, depth):
     say(f"case: evaluate FP32 L={L} {layout} depth={depth}")
