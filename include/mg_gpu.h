@@ -83,6 +83,80 @@ int mg_optimize(const mg_target* t, const mg_program* p, const struct matchgate 
                 const struct rtr_params* params, int niter, int hvp, int precision, int precision_switch,
                 double* f_iter, numeric* f_citer, struct matchgate gates_opt[], int* nhvp);
 
+/* ---- batched evaluation ----
+ * Requests of any targets, programs, qubit counts (<= 16) and modes, merged into one
+ * persistent launch per precision.  Outputs as mg_riemannian_gradient_hvp.  A request's
+ * results are the same bits whether it is evaluated alone or with others (they depend
+ * on the batch shape of its qubit count, mg_batch_set_shape, only). */
+typedef struct mg_request {
+    const mg_target* target;
+    const mg_program* program;
+    const struct matchgate* gates; /* [nparams] */
+    const double* v;               /* real tangent direction [8 nparams] (what == MG_EVAL_F_GRAD_HVP) */
+    int what;                      /* MG_EVAL_F, MG_EVAL_F_GRAD or MG_EVAL_F_GRAD_HVP */
+    int precision;                 /* MG_FP64 or MG_FP32 sweeps */
+    numeric* f;                    /* outputs, each nullable */
+    double* grad_vec;              /* Riemannian gradient [8 nparams] (what >= MG_EVAL_F_GRAD) */
+    double* hvp;                   /* Riemannian Hessian-vector product [8 nparams] */
+} mg_request;
+int mg_evaluate_batch(mg_request* reqs, int n);
+/* kernel variant of the batched launch per precision: threads per workgroup and gate-window
+ * bits (each thread applies gates to 2^window_bits amplitudes per state in registers).
+ * Compiled: FP32 (256, 2..4), (512, 2..3); FP64 (256, 2..3), (512, 2).  Returns -1 for others. */
+int mg_batch_set_kernel(int precision, int threads, int window_bits);
+void mg_batch_get_kernel(int precision, int* threads, int* window_bits);
+/* workgroups per CU of the batched launch (0 automatic) */
+void mg_batch_set_grid_per_cu(int n);
+/* batch shape for nqubits in the given precision: threads per column (64 .. threads, a power
+ * of two) and basis columns per work item; 0 restores the default */
+void mg_batch_set_shape(int precision, int nqubits, int threads_per_column, int columns_per_item);
+void mg_batch_get_shape(int precision, int nqubits, int* threads_per_column, int* columns_per_item);
+/* duration of the last mg_evaluate_batch kernels in ms (needs mg_set_kernel_timing(1)) */
+float mg_batch_last_kernel_ms(void);
+
+/* ---- serving: a stream of trust-region optimisation jobs ----
+ * Each job runs riemannian_trust_region_optimize_hvp with set_rtr_default_params(16 nparams)
+ * on its circuit and target.  batched = 1: jobs are admitted as they arrive (and as slots and
+ * device memory allow) and all admitted jobs' evaluations are merged into shared launches;
+ * batched = 0: jobs run one at a time, one launch per evaluation.  The target is created on
+ * the device at admission and freed when the job ends. */
+typedef struct mg_job {
+    int nqubits;
+    const mg_program* program;           /* circuit to optimise */
+    const struct matchgate* gates_start; /* [nparams] */
+    const numeric* U;                    /* dense target (row-major 2^n x 2^n), or NULL for a circuit target: */
+    int target_ngates;                   /*   gate k on (target_wires[2k], target_wires[2k+1]) */
+    const struct matchgate* target_gates;
+    const int* target_wires;
+    int niter;
+    double arrival;                      /* seconds after the start of mg_serve */
+    double* f_iter;                      /* out: [niter + 1] as riemannian_trust_region_optimize_hvp */
+    struct matchgate* gates_opt;         /* out: [nparams] */
+    double t_admit, t_done;              /* out: seconds after the start */
+    int nhvp;                            /* out: HVP evaluations */
+    int status;                          /* out: 0 ok */
+} mg_job;
+
+typedef struct mg_serve_options {
+    int batched;          /* 1: continuous batching, 0: one job at a time */
+    int precision;        /* sweep precision (MG_FP32 or MG_FP64) */
+    int precision_switch; /* with MG_FP32: the last iterations run in FP64 */
+    int max_active;       /* admitted jobs at most (0: no limit) */
+    double hbm_budget;    /* device bytes the admitted jobs may hold (0: no limit) */
+    int realtime;         /* 1: admit a job no earlier than its arrival time; 0: all arrive at 0 */
+} mg_serve_options;
+
+typedef struct mg_serve_stats {
+    double wall_s;         /* from start to the last job's end */
+    long rounds;           /* launches (batched) or evaluations (one job at a time) */
+    long requests;         /* evaluations */
+    double kernel_ms;      /* sweep kernel time (with mg_set_kernel_timing(1)) */
+    int max_active;        /* peak admitted jobs */
+    double peak_footprint; /* peak device bytes held by admitted jobs */
+} mg_serve_stats;
+
+int mg_serve(mg_job* jobs, int njobs, const mg_serve_options* opt, mg_serve_stats* stats);
+
 /* ---- tuning and measurement ---- */
 /* force checkpointed (1) or uncomputed (0) reverse sweeps; -1 restores automatic choice */
 void mg_set_checkpointing(int mode);

@@ -18,16 +18,44 @@
 #include "../../include/mg_brickwall_opt.h"
 #include "../../include/mg_gpu.h"
 #include "../../include/trust_region.h"
+#include "mg_host.hpp"
 #include "mg_runtime.hpp"
 
 using mg::zd;
 
-struct mg_target {
-    std::unique_ptr<mg::Target> t;
-};
-struct mg_program {
-    mg::Program p;
-};
+namespace mg {
+
+// ---- Riemannian projection (reference src/matchgate_target.c:903-967) ----
+
+void riemannian_gradient(const struct matchgate* V, int P, const zd* dgrad, double* grad_vec)
+{
+    for (int i = 0; i < P; i++) {
+        struct matchgate g;
+        std::memcpy(&g, dgrad + 8 * i, sizeof(g));
+        conjugate_matchgate(&g);
+        tangent_matchgate_to_real(&V[i], &g, grad_vec + 8 * i);
+    }
+}
+
+// G = P_V(conj(hz)) - (Z g^T V + V g^T Z)/2 for one parameter gate
+void weingarten_block(const struct matchgate* V, const zd* g, const struct matchgate* Z, const zd* hz, double* r)
+{
+    struct matchgate G, Gp, gc, gh, T;
+    std::memcpy(&G, hz, sizeof(G));
+    conjugate_matchgate(&G);
+    project_tangent_matchgate(V, &G, &Gp);
+    std::memcpy(&gc, g, sizeof(gc));
+    conjugate_matchgate(&gc);
+    adjoint_matchgate(&gc, &gh);
+    symmetric_triple_matchgate_product(Z, &gh, V, &T);
+    sub_matchgates(&Gp, &T, &G);
+    tangent_matchgate_to_real(V, &G, r);
+}
+
+}  // namespace mg
+
+using mg::riemannian_gradient;
+using mg::weingarten_block;
 
 namespace {
 
@@ -163,33 +191,6 @@ bool check_brickwall(int nlayers, int nqubits, const int* const* perms)
         return false;
     }
     return true;
-}
-
-// ---- Riemannian projection (reference src/matchgate_target.c:903-967) ----
-
-void riemannian_gradient(const struct matchgate* V, int P, const zd* dgrad, double* grad_vec)
-{
-    for (int i = 0; i < P; i++) {
-        struct matchgate g;
-        std::memcpy(&g, dgrad + 8 * i, sizeof(g));
-        conjugate_matchgate(&g);
-        tangent_matchgate_to_real(&V[i], &g, grad_vec + 8 * i);
-    }
-}
-
-// G = P_V(conj(hz)) - (Z g^T V + V g^T Z)/2 for one parameter gate
-void weingarten_block(const struct matchgate* V, const zd* g, const struct matchgate* Z, const zd* hz, double* r)
-{
-    struct matchgate G, Gp, gc, gh, T;
-    std::memcpy(&G, hz, sizeof(G));
-    conjugate_matchgate(&G);
-    project_tangent_matchgate(V, &G, &Gp);
-    std::memcpy(&gc, g, sizeof(gc));
-    conjugate_matchgate(&gc);
-    adjoint_matchgate(&gc, &gh);
-    symmetric_triple_matchgate_product(Z, &gh, V, &T);
-    sub_matchgates(&Gp, &T, &G);
-    tangent_matchgate_to_real(V, &G, r);
 }
 
 int full_hessian(const mg::Target& t, const mg::Program& p, const zd* gates, int precision, zd* f, zd* dgrad, zd* hess)

@@ -1,0 +1,521 @@
+// Batched column sweep: one persistent launch over the basis columns of many
+// evaluation requests of different qubit counts, depths, modes and targets, with
+// the gates of each request fused into W-qubit windows.
+//
+// Work split.  Each request ("instance") is cut into work items of a fixed number
+// of columns.  Workgroups take items from an atomic counter (largest items first,
+// as ordered by the host) until the list is exhausted.  Within an item, the T
+// threads form T / cgw column groups of cgw threads; every group sweeps its own
+// column of the same instance, so all groups pass the same gate sequence and share
+// the workgroup barriers (a group may be narrower than a wave).
+//
+// Gate windows.  The host regroups the instance's gates into blocks whose gates all
+// act inside one window of W qubits (mg_window.cpp).  A column's state splits into
+// 2^(L-W) tuples of 2^W amplitudes that differ only in the window bits; per block a
+// thread loads its tuples' amplitudes of every live state into registers, applies
+// all of the block's gates there and stores them back: one LDS (or memory) pass and
+// one barrier per block instead of per gate.  The per-gate outer products of the
+// reverse sweep are reduced per tuple by a transposing wave butterfly and added
+// into per-gate LDS slots (double-buffered by block parity); after the block's
+// barrier one wave per gate adds the slots, in wave order, into the accumulator.
+//
+// The per-column arithmetic is that of sweep_kernel (mg_sweep.hip): seed
+// lam = -conj(U e_b), forward psi (and dpsi), f_b = lam . psi_G, reverse sweep
+// uncomputing psi_g with V^{-1}.  An item's partial sums depend only on its instance,
+// its columns and its batch shape, and each instance's items are summed in item
+// order by batch_reduce: a request evaluates to the same bits alone and in any mix.
+//
+// Reference: per-column loop src/matchgate_target.c:788-837, forward
+// src/matchgate_brickwall.c:52-78, backward kernel src/gate.c:659-772.
+#include "mg_batch.hpp"
+#include "mg_quad.hpp"
+
+namespace mg {
+
+// Read-only descriptor data (gate instances, coefficients, window blocks) is read
+// through the constant address space: with a wave-uniform address the loads become
+// scalar loads into SGPRs, which the gate kernels then use as scalar operands.
+#define MG_CONST __attribute__((address_space(4)))
+template <typename T>
+__device__ inline const MG_CONST T* cptr(const T* p)
+{
+    return (const MG_CONST T*)p;
+}
+
+template <typename R, int N>
+__device__ inline void load_coef_c(const cpx<R>* src, cpx<R>* dst)
+{
+    const MG_CONST R* p = cptr(reinterpret_cast<const R*>(src));
+#pragma unroll
+    for (int k = 0; k < N; k++) dst[k] = cpx<R>{p[2 * k], p[2 * k + 1]};
+}
+
+// a window block's positions and amplitude offsets, in scalar registers
+template <int W>
+struct BlockRegs {
+    int g0, g1;
+    int pos[W];
+    uint32_t off[1 << W];
+};
+
+template <int W>
+__device__ inline BlockRegs<W> load_block(const WBlock* b)
+{
+    const MG_CONST WBlock* p = cptr(b);
+    BlockRegs<W> r;
+    r.g0 = p->g0;
+    r.g1 = p->g1;
+#pragma unroll
+    for (int i = 0; i < W; i++) r.pos[i] = p->pos[i];
+#pragma unroll
+    for (int j = 0; j < (1 << W); j++) r.off[j] = p->off[j];
+    return r;
+}
+
+template <typename R, bool SWZ>
+__device__ inline uint32_t lin(uint32_t i)
+{
+    if constexpr (SWZ) return win_swz<batch_bank_bits<R>()>(i);
+    else return i;
+}
+
+// tuple u -> address with zero bits at the W window positions (ascending)
+template <int W>
+__device__ inline uint32_t deposit(uint32_t u, const int* pos)
+{
+#pragma unroll
+    for (int i = 0; i < W; i++) {
+        const int p = pos[i];
+        u = ((u >> p) << (p + 1)) | (u & ((1u << p) - 1));
+    }
+    return u;
+}
+
+// compile-time quadruples of a window-local gate: local bits LA > LB hold the gate's
+// bit positions hi > lo; the quadruple order is that of load_quad (00, lo set, hi set, both)
+template <int W, int LA, int LB>
+struct WQuads {
+    static constexpr int N = 1 << (W - 2);
+    static constexpr int idx(int r)
+    {
+        int x = ((r >> LB) << (LB + 1)) | (r & ((1 << LB) - 1));
+        return ((x >> LA) << (LA + 1)) | (x & ((1 << LA) - 1));
+    }
+};
+
+template <typename R, int W>
+struct Win {
+    cpx<R> a[1 << W];
+};
+
+template <typename R, int W, int LA, int LB>
+__device__ inline Quad<R> wq_get(const Win<R, W>& s, int r)
+{
+    const int x = WQuads<W, LA, LB>::idx(r);
+    return {s.a[x], s.a[x | (1 << LB)], s.a[x | (1 << LA)], s.a[x | (1 << LA) | (1 << LB)]};
+}
+
+template <typename R, int W, int LA, int LB>
+__device__ inline void wq_put(Win<R, W>& s, int r, const Quad<R>& q)
+{
+    const int x = WQuads<W, LA, LB>::idx(r);
+    s.a[x] = q.x;
+    s.a[x | (1 << LB)] = q.y;
+    s.a[x | (1 << LA)] = q.z;
+    s.a[x | (1 << LA) | (1 << LB)] = q.w;
+}
+
+// ---- per-gate operations on a window tuple ----------------------------------------------
+// forward: P <- V P  (and D <- V D + Z P)
+template <typename R, int W, int LA, int LB, int MODE>
+__device__ inline void wg_forward(const cpx<R>* cf, Win<R, W>& P, Win<R, W>& D)
+{
+#pragma unroll
+    for (int r = 0; r < WQuads<W, LA, LB>::N; r++) {
+        const Quad<R> p = wq_get<R, W, LA, LB>(P, r);
+        if constexpr (MODE == 2) wq_put<R, W, LA, LB>(D, r, gapply_add(cf, wq_get<R, W, LA, LB>(D, r), gapply(cf + 16, p)));
+        wq_put<R, W, LA, LB>(P, r, gapply(cf, p));
+    }
+}
+
+// reverse: psi_g = V^{-1} psi_{g+1} (and dpsi), outer products, lam <- V^T lam (and dlam)
+template <typename R, int W, int LA, int LB, int MODE>
+__device__ inline void wg_backward(const cpx<R>* cf, Win<R, W>& Lm, Win<R, W>& P, Win<R, W>& DL, Win<R, W>& D,
+                                   cpx<R>* ga, cpx<R>* ha)
+{
+#pragma unroll
+    for (int r = 0; r < WQuads<W, LA, LB>::N; r++) {
+        const Quad<R> l = wq_get<R, W, LA, LB>(Lm, r);
+        const Quad<R> p = gapply(cf + 8, wq_get<R, W, LA, LB>(P, r));
+        wq_put<R, W, LA, LB>(P, r, p);
+        outer_acc(ga, l, p);
+        if constexpr (MODE == 2) {
+            const Quad<R> dp = gapply(cf + 8, qsub(wq_get<R, W, LA, LB>(D, r), gapply(cf + 16, p)));
+            wq_put<R, W, LA, LB>(D, r, dp);
+            const Quad<R> dl = wq_get<R, W, LA, LB>(DL, r);
+            outer_acc(ha, dl, p);
+            outer_acc(ha, l, dp);
+            wq_put<R, W, LA, LB>(DL, r, gapplyT_add(cf, dl, gapplyT(cf + 16, l)));
+        }
+        wq_put<R, W, LA, LB>(Lm, r, gapplyT(cf, l));
+    }
+}
+
+// dispatch on the window-local pair index (la, lb): (1,0) (2,0) (2,1) (3,0) (3,1) (3,2)
+#define MG_WCASES(W, CALL)                                   \
+    switch (wcase) {                                         \
+        case 0: CALL(W, 1, 0); break;                        \
+        case 1: if constexpr (W >= 3) { CALL(W, 2, 0); } break; \
+        case 2: if constexpr (W >= 3) { CALL(W, 2, 1); } break; \
+        case 3: if constexpr (W >= 4) { CALL(W, 3, 0); } break; \
+        case 4: if constexpr (W >= 4) { CALL(W, 3, 1); } break; \
+        default: if constexpr (W >= 4) { CALL(W, 3, 2); } break; \
+    }
+
+template <typename R, int W, bool SWZ>
+__device__ inline void win_load(const cpx<R>* st, uint32_t sb, const uint32_t* off, Win<R, W>& s)
+{
+#pragma unroll
+    for (int j = 0; j < (1 << W); j++) s.a[j] = st[sb ^ off[j]];
+}
+
+template <typename R, int W, bool SWZ>
+__device__ inline void win_store(cpx<R>* st, uint32_t sb, const uint32_t* off, const Win<R, W>& s)
+{
+#pragma unroll
+    for (int j = 0; j < (1 << W); j++) st[sb ^ off[j]] = s.a[j];
+}
+
+// apply gates [g0, g1) of a forward-only program (target seeding) to one state
+template <typename R, int W, bool SWZ>
+__device__ inline void win_apply_program(cpx<R>* st, const WBlock* blocks, int nb, const Inst* inst, const cpx<R>* coef,
+                                         uint32_t ntup, int tl, int cgw, bool on)
+{
+    for (int k = 0; k < nb; k++) {
+        const BlockRegs<W> bk = load_block<W>(blocks + k);
+        if (on)
+            for (uint32_t u = tl; u < ntup; u += cgw) {
+                const uint32_t base = deposit<W>(u, bk.pos);
+                const uint32_t sb = lin<R, SWZ>(base);
+                Win<R, W> P, D;
+                win_load<R, W, SWZ>(st, sb, bk.off, P);
+                for (int g = bk.g0; g < bk.g1; g++) {
+                    const int wcase = cptr(inst)[g].lo;
+                    cpx<R> cf[8];
+                    load_coef_c<R, 8>(coef + (size_t)g * kCoefPerInst, cf);
+#define MG_FWD0(WW, A, B) wg_forward<R, WW, A, B, 0>(cf, P, D)
+                    MG_WCASES(W, MG_FWD0)
+#undef MG_FWD0
+                }
+                win_store<R, W, SWZ>(st, sb, bk.off, P);
+            }
+        __syncthreads();
+    }
+}
+
+template <typename R, int T, int W, int MODE, bool LDS>
+__device__ void batch_item(const BatchInst<R>& in, const BatchItem& it, unsigned char* smem, cpx<R>* gslice,
+                           double* out)
+{
+    constexpr int NW = T / 64;
+    constexpr int NV = MODE == 2 ? 32 : 16;
+    constexpr int SHIFT = 6 - log2c<NV>();
+    constexpr int NSL = MODE == 2 ? 4 : 2;
+    constexpr bool SWZ = LDS;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const uint32_t S = 1u << in.L, ntup = S >> W;
+    const int cgw = in.cgw, ncc = T / cgw, grp = tid / cgw, tl = tid - grp * cgw;
+
+    double* stage = reinterpret_cast<double*>(smem);        // [2][kWinMaxGates][NW][32]
+    double* fst = stage + 2 * kWinMaxGates * NW * 32;       // [NW][2]
+    double* acc = reinterpret_cast<double*>(smem + in.acc_off);
+    cpx<R>* st = (LDS ? reinterpret_cast<cpx<R>*>(smem + in.state_off) : gslice) + (size_t)grp * NSL * S;
+    cpx<R>* lam = st;
+    cpx<R>* psi = st + S;
+    cpx<R>* dlam = st + 2 * S;
+    cpx<R>* dpsi = st + 3 * S;
+
+    for (int k = tid; k < 32 * in.P; k += T) acc[k] = 0.0;
+    double fre = 0.0, fim = 0.0;
+    __syncthreads();
+
+    for (int r = it.b0; r < it.b1; r += ncc) {
+        const uint32_t b = (uint32_t)(r + grp);
+        const bool on = (int)b < it.b1;
+        // ---- seed: lam = -conj(U e_b) -------------------------------------------------
+        if (in.target_kind == 0) {
+            if (on) {
+                const cpx<R>* row = in.seed + (size_t)b * S;
+                for (uint32_t i = tl; i < S; i += cgw) lam[lin<R, SWZ>(i)] = row[i];
+            }
+        } else {
+            if (on)
+                for (uint32_t i = tl; i < S; i += cgw) lam[lin<R, SWZ>(i)] = {R(i == b), R(0)};
+            __syncthreads();
+            win_apply_program<R, W, SWZ>(lam, in.tblocks, in.ntblocks, in.tinst, in.tcoef, ntup, tl, cgw, on);
+            if (on)
+                for (uint32_t i = tl; i < S; i += cgw) {
+                    cpx<R>& v = lam[lin<R, SWZ>(i)];
+                    v = {-v.re, v.im};
+                }
+        }
+        // ---- forward sweep -------------------------------------------------------------
+        if (on)
+            for (uint32_t i = tl; i < S; i += cgw) {
+                psi[lin<R, SWZ>(i)] = {R(i == b), R(0)};
+                if constexpr (MODE == 2) dpsi[lin<R, SWZ>(i)] = {R(0), R(0)};
+            }
+        __syncthreads();
+        for (int k = 0; k < in.nblocks; k++) {
+            const BlockRegs<W> bk = load_block<W>(in.blocks + k);
+            if (on)
+                for (uint32_t u = tl; u < ntup; u += cgw) {
+                    const uint32_t sb = lin<R, SWZ>(deposit<W>(u, bk.pos));
+                    Win<R, W> P, D;
+                    win_load<R, W, SWZ>(psi, sb, bk.off, P);
+                    if constexpr (MODE == 2) win_load<R, W, SWZ>(dpsi, sb, bk.off, D);
+                    for (int g = bk.g0; g < bk.g1; g++) {
+                        const int wcase = cptr(in.inst)[g].lo;
+                        cpx<R> cf[24];
+                        load_coef_c<R, 8>(in.coef + (size_t)g * kCoefPerInst, cf);
+                        if constexpr (MODE == 2) load_coef_c<R, 8>(in.coef + (size_t)g * kCoefPerInst + 16, cf + 16);
+#define MG_FWD(WW, A, B) wg_forward<R, WW, A, B, MODE == 2 ? 2 : 0>(cf, P, D)
+                        MG_WCASES(W, MG_FWD)
+#undef MG_FWD
+                    }
+                    win_store<R, W, SWZ>(psi, sb, bk.off, P);
+                    if constexpr (MODE == 2) win_store<R, W, SWZ>(dpsi, sb, bk.off, D);
+                }
+            __syncthreads();
+        }
+        // ---- f_b = lam . psi_G ------------------------------------------------------------
+        {
+            double sr = 0.0, si = 0.0;
+            if (on)
+                for (uint32_t i = tl; i < S; i += cgw) {
+                    const cpx<R> t = cmul(lam[lin<R, SWZ>(i)], psi[lin<R, SWZ>(i)]);
+                    sr += (double)t.re;
+                    si += (double)t.im;
+                }
+            sr = wave_sum(sr);
+            si = wave_sum(si);
+            if (lane == 0) {
+                fst[2 * wv] = sr;
+                fst[2 * wv + 1] = si;
+            }
+            __syncthreads();
+            if (tid == 0) {
+                double tr = 0.0, ti = 0.0;
+                for (int w = 0; w < NW; w++) {
+                    tr += fst[2 * w];
+                    ti += fst[2 * w + 1];
+                }
+                fre += tr;
+                fim += ti;
+            }
+        }
+        if constexpr (MODE >= 1) {
+            // ---- reverse sweep -------------------------------------------------------------
+            if constexpr (MODE == 2) {
+                if (on)
+                    for (uint32_t i = tl; i < S; i += cgw) dlam[lin<R, SWZ>(i)] = {R(0), R(0)};
+            }
+            __syncthreads();
+            for (int k = in.nblocks - 1; k >= 0; k--) {
+                const BlockRegs<W> bk = load_block<W>(in.blocks + k);
+                const int buf = k & 1;
+                double* stg = stage + (size_t)buf * kWinMaxGates * NW * 32;
+                for (uint32_t u = tl; u < ntup; u += cgw) {
+                    // inactive groups and idle lanes still take part in the wave butterflies
+                    const bool act = on;
+                    const uint32_t sb = lin<R, SWZ>(deposit<W>(act ? u : 0u, bk.pos));
+                    Win<R, W> Lm, P, DL, D;
+                    if (act) {
+                        win_load<R, W, SWZ>(lam, sb, bk.off, Lm);
+                        win_load<R, W, SWZ>(psi, sb, bk.off, P);
+                        if constexpr (MODE == 2) {
+                            win_load<R, W, SWZ>(dlam, sb, bk.off, DL);
+                            win_load<R, W, SWZ>(dpsi, sb, bk.off, D);
+                        }
+                    }
+                    for (int g = bk.g1 - 1; g >= bk.g0; g--) {
+                        const int wcase = cptr(in.inst)[g].lo;
+                        cpx<R> cf[24];
+                        load_coef_c<R, MODE == 2 ? 24 : 16>(in.coef + (size_t)g * kCoefPerInst, cf);
+                        cpx<R> ga[8], ha[8];
+#pragma unroll
+                        for (int e = 0; e < 8; e++) {
+                            ga[e] = {R(0), R(0)};
+                            ha[e] = {R(0), R(0)};
+                        }
+                        if (act) {
+#define MG_BWD(WW, A, B) wg_backward<R, WW, A, B, MODE>(cf, Lm, P, DL, D, ga, ha)
+                            MG_WCASES(W, MG_BWD)
+#undef MG_BWD
+                        }
+                        R v[NV];
+#pragma unroll
+                        for (int e = 0; e < 8; e++) {
+                            v[2 * e] = ga[e].re;
+                            v[2 * e + 1] = ga[e].im;
+                            if constexpr (MODE == 2) {
+                                v[16 + 2 * e] = ha[e].re;
+                                v[16 + 2 * e + 1] = ha[e].im;
+                            }
+                        }
+                        wave_halve<NV, 32, R>(v, lane);
+                        if ((lane & ((1 << SHIFT) - 1)) == 0)
+                            stg[((g - bk.g0) * NW + wv) * 32 + (lane >> SHIFT)] += (double)v[0];
+                    }
+                    if (act) {
+                        win_store<R, W, SWZ>(lam, sb, bk.off, Lm);
+                        win_store<R, W, SWZ>(psi, sb, bk.off, P);
+                        if constexpr (MODE == 2) {
+                            win_store<R, W, SWZ>(dlam, sb, bk.off, DL);
+                            win_store<R, W, SWZ>(dpsi, sb, bk.off, D);
+                        }
+                    }
+                }
+                __syncthreads();
+                // one wave per gate adds the staged sums in wave order, then clears its slots
+                for (int g = bk.g0; g < bk.g1; g++) {
+                    if (wv == (g - bk.g0) % NW && lane < NV) {
+                        double* sg = stg + (size_t)(g - bk.g0) * NW * 32;
+                        double s = 0.0;
+                        for (int w = 0; w < NW; w++) {
+                            s += sg[w * 32 + lane];
+                            sg[w * 32 + lane] = 0.0;
+                        }
+                        const int gparam = cptr(in.inst)[g].param, gflip = cptr(in.inst)[g].flip;
+                        const int ci = lane >> 1;
+                        int e = ci & 7;
+                        if (gflip && e < 4) e = 3 - e;
+                        acc[32 * gparam + 2 * ((ci & 8) | e) + (lane & 1)] += s;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+    __syncthreads();
+    if (tid == 0) {
+        out[0] = fre;
+        out[1] = fim;
+    }
+    for (int k = tid; k < 32 * in.P; k += T) out[2 + k] = acc[k];
+}
+
+template <typename R, int T, int W>
+__global__ __launch_bounds__(T) void batch_kernel(const BatchInst<R>* __restrict__ insts,
+                                                  const BatchItem* __restrict__ items, int nitems, int* counter,
+                                                  double* partial, cpx<R>* scratch, long long scratch_stride)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ int next;
+    cpx<R>* gslice = scratch ? scratch + (size_t)blockIdx.x * scratch_stride : nullptr;
+    {
+        double* stage = reinterpret_cast<double*>(smem);
+        for (int k = threadIdx.x; k < 2 * kWinMaxGates * (T / 64) * 32; k += T) stage[k] = 0.0;
+    }
+    for (;;) {
+        if (threadIdx.x == 0) next = atomicAdd(counter, 1);
+        __syncthreads();
+        const int k = __builtin_amdgcn_readfirstlane(next);
+        __syncthreads();
+        if (k >= nitems) break;
+        const BatchItem it = items[k];
+        const BatchInst<R> in = insts[it.inst];
+        double* out = partial + it.out;
+        const int path = in.mode * 2 + (in.lds ? 1 : 0);
+        switch (path) {
+            case 0: batch_item<R, T, W, 0, false>(in, it, smem, gslice, out); break;
+            case 1: batch_item<R, T, W, 0, true>(in, it, smem, gslice, out); break;
+            case 2: batch_item<R, T, W, 1, false>(in, it, smem, gslice, out); break;
+            case 3: batch_item<R, T, W, 1, true>(in, it, smem, gslice, out); break;
+            case 4: batch_item<R, T, W, 2, false>(in, it, smem, gslice, out); break;
+            default: batch_item<R, T, W, 2, true>(in, it, smem, gslice, out); break;
+        }
+    }
+}
+
+__global__ void batch_reset(int* counter) { *counter = 0; }
+
+// out[seg.dst + k] = sum over items j of partial[seg.src + j * seg.stride + k], in j order
+// (thread row ty sums j = ty, ty + 16, ...; row 0 adds the 16 row sums in order)
+__global__ __launch_bounds__(1024) void batch_reduce(const ReduceSeg* __restrict__ segs, const int2* __restrict__ blocks,
+                                                     const double* __restrict__ partial, double* __restrict__ out)
+{
+    __shared__ double rows[16][64];
+    const int2 bk = blocks[blockIdx.x];
+    const ReduceSeg sg = segs[bk.x];
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    const int k = bk.y + tx;
+    double s = 0.0;
+    if (k < sg.stride)
+        for (int j = ty; j < sg.n; j += 16) s += partial[sg.src + (long long)j * sg.stride + k];
+    rows[ty][tx] = s;
+    __syncthreads();
+    if (ty == 0 && k < sg.stride) {
+        double t = 0.0;
+        for (int r = 0; r < 16; r++) t += rows[r][tx];
+        out[sg.dst + k] = t;
+    }
+}
+
+template <typename R, int T, int W>
+static hipError_t launch_bt(const BatchInst<R>* insts, const BatchItem* items, int nitems, int* counter,
+                            double* partial, cpx<R>* scratch, long long scratch_stride, int grid, size_t lds,
+                            hipStream_t s)
+{
+    auto k = batch_kernel<R, T, W>;
+    if (lds > 65536) {
+        hipError_t e = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(batch_reset, dim3(1), dim3(1), 0, s, counter);
+    hipLaunchKernelGGL(k, dim3(grid), dim3(T), lds, s, insts, items, nitems, counter, partial, scratch, scratch_stride);
+    return hipGetLastError();
+}
+
+bool batch_variant_exists(int precision_fp32, int threads, int wbits)
+{
+    if (precision_fp32)
+        return (threads == 256 && wbits >= 2 && wbits <= 4) || (threads == 512 && (wbits == 2 || wbits == 3));
+    return (threads == 256 && (wbits == 2 || wbits == 3)) || (threads == 512 && wbits == 2);
+}
+
+template <>
+hipError_t launch_batch<float>(const BatchInst<float>* insts, const BatchItem* items, int nitems, int* counter,
+                               double* partial, cpx<float>* scratch, long long scratch_stride, int threads, int wbits,
+                               int grid, size_t lds, hipStream_t s)
+{
+#define MG_BV(TT, WW)                                                                                          \
+    if (threads == TT && wbits == WW)                                                                          \
+        return launch_bt<float, TT, WW>(insts, items, nitems, counter, partial, scratch, scratch_stride, grid, lds, s);
+    MG_BV(256, 4) MG_BV(256, 3) MG_BV(256, 2) MG_BV(512, 3) MG_BV(512, 2)
+    return hipErrorInvalidValue;
+}
+
+template <>
+hipError_t launch_batch<double>(const BatchInst<double>* insts, const BatchItem* items, int nitems, int* counter,
+                                double* partial, cpx<double>* scratch, long long scratch_stride, int threads,
+                                int wbits, int grid, size_t lds, hipStream_t s)
+{
+#define MG_BVD(TT, WW)                                                                                         \
+    if (threads == TT && wbits == WW)                                                                          \
+        return launch_bt<double, TT, WW>(insts, items, nitems, counter, partial, scratch, scratch_stride, grid, lds, s);
+    MG_BVD(256, 3) MG_BVD(256, 2) MG_BVD(512, 2)
+#undef MG_BVD
+#undef MG_BV
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_batch_reduce(const ReduceSeg* segs, const int2* blocks, int nblocks, const double* partial, double* out,
+                               hipStream_t s)
+{
+    if (nblocks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(batch_reduce, dim3(nblocks), dim3(1024), 0, s, segs, blocks, partial, out);
+    return hipGetLastError();
+}
+
+}  // namespace mg

@@ -8,6 +8,7 @@
 #include <map>
 #include <memory>
 
+#include "mg_batch.hpp"
 #include "mg_sweep.hpp"
 
 namespace mg {
@@ -60,6 +61,12 @@ struct Context {
     DevBuf inst, coef, partial, out, scratch, ckpt;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::vector<double> out_h;
+    // batched evaluation: packed descriptor upload, partials, outputs, scratch, counter
+    DevBuf b_up, b_partial, b_out, b_scratch, b_counter;
+    void* b_up_h = nullptr;   // pinned staging of the upload
+    size_t b_up_hn = 0;
+    void* b_out_h = nullptr;  // pinned staging of the outputs
+    size_t b_out_hn = 0;
 };
 
 static Context* ctx()
@@ -453,6 +460,339 @@ int evaluate(const Target& tc, const Program& p, const zd* gates, const zd* Z, i
         return run_sweep<float>(c, t, p, b, mode, b0, b1, out, stats);
     }
     return run_sweep<double>(c, t, p, b, mode, b0, b1, out, stats);
+}
+
+}  // namespace mg
+
+// ---- batched evaluation ------------------------------------------------------------------------
+namespace mg {
+
+BatchTuning& batch_tuning()
+{
+    static BatchTuning t;
+    return t;
+}
+
+BatchStats& last_batch_stats()
+{
+    static BatchStats s;
+    return s;
+}
+
+static int pinned_ensure(void** p, size_t* n, size_t bytes)
+{
+    if (*p && *n >= bytes) return 0;
+    if (*p) (void)hipHostFree(*p);
+    *p = nullptr;
+    *n = 0;
+    if (hipHostMalloc(p, bytes ? bytes : 256, hipHostMallocDefault) != hipSuccess) {
+        *p = nullptr;
+        return -1;
+    }
+    *n = bytes;
+    return 0;
+}
+
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+void batch_shape(int L, int precision, int* cgw, int* cpi)
+{
+    const BatchTuning& bt = batch_tuning();
+    const int pi = precision == kFP32 ? 1 : 0;
+    const int W = bt.wbits[pi], threads = bt.threads[pi];
+    const long S = 1L << L, ntup = std::max(1L, S >> W);
+    const long wmax = std::min<long>(threads, ntup);
+    long w = (L < 31 && bt.cgw[pi][L]) ? bt.cgw[pi][L] : wmax;
+    w = std::max(1L, std::min(wmax, w));
+    while (w & (w - 1)) w &= w - 1;  // a power of two
+    const int ncc = (int)(threads / w);
+    long c = (L < 31 && bt.cpi[pi][L]) ? bt.cpi[pi][L] : std::max<long>(ncc, S / 256);
+    c = (c + ncc - 1) / ncc * ncc;
+    *cgw = (int)w;
+    *cpi = (int)std::min<long>(c, S);
+}
+
+// window blocks of an instance list, reordered instances (lo = window-local pair
+// index) and coefficients, block offsets for LDS (swizzled) and global states
+struct Windowed {
+    std::vector<Inst> inst;
+    std::vector<cpx<double>> coef;
+    std::vector<WBlock> blk_lds, blk_glob;
+};
+
+template <typename R>
+static void make_windowed(const std::vector<Inst>& inst, const std::vector<cpx<double>>& coef, int L, int W,
+                          Windowed* w)
+{
+    std::vector<int> order;
+    std::vector<WBlock> blocks;
+    window_blocks(inst, L, W, kWinMaxGates, &order, &blocks);
+    const size_t G = inst.size();
+    w->inst.resize(G);
+    w->coef.resize(G * kCoefPerInst);
+    for (const WBlock& bk : blocks)
+        for (int k = bk.g0; k < bk.g1; k++) {
+            Inst in = inst[order[k]];
+            in.lo = window_case(bk, W, in.lo, in.hi);
+            in.hi = 0;
+            w->inst[k] = in;
+            std::copy(coef.begin() + (size_t)order[k] * kCoefPerInst, coef.begin() + (size_t)(order[k] + 1) * kCoefPerInst,
+                      w->coef.begin() + (size_t)k * kCoefPerInst);
+        }
+    w->blk_lds = blocks;
+    w->blk_glob = blocks;
+    for (size_t k = 0; k < blocks.size(); k++)
+        for (int j = 0; j < (1 << W); j++) {
+            uint32_t o = 0;
+            for (int i = 0; i < W; i++) o |= ((uint32_t)(j >> i) & 1u) << blocks[k].pos[i];
+            w->blk_glob[k].off[j] = o;
+            w->blk_lds[k].off[j] = win_swz<batch_bank_bits<R>()>(o);
+        }
+}
+
+// device copy of a target program in window form (per precision, built on first use)
+template <typename R>
+static int ensure_windowed_target(Target& t)
+{
+    const int pi = std::is_same<R, float>::value ? 1 : 0;
+    Target::WinCache& wc = t.wc[pi];
+    const int W = batch_tuning().wbits[pi];
+    if (wc.wbits == W) return 0;
+    Windowed w;
+    make_windowed<R>(t.tinst_h, t.tcoef_h, t.L, W, &w);
+    std::vector<cpx<R>> cR(w.coef.size());
+    for (size_t k = 0; k < cR.size(); k++) cR[k] = {(R)w.coef[k].re, (R)w.coef[k].im};
+    MG_HIP(wc.inst.ensure(w.inst.size() * sizeof(Inst)));
+    MG_HIP(wc.coef.ensure(cR.size() * sizeof(cpx<R>)));
+    MG_HIP(wc.blk_lds.ensure(w.blk_lds.size() * sizeof(WBlock)));
+    MG_HIP(wc.blk_glob.ensure(w.blk_glob.size() * sizeof(WBlock)));
+    MG_HIP(hipMemcpy(wc.inst.p, w.inst.data(), w.inst.size() * sizeof(Inst), hipMemcpyHostToDevice));
+    MG_HIP(hipMemcpy(wc.coef.p, cR.data(), cR.size() * sizeof(cpx<R>), hipMemcpyHostToDevice));
+    MG_HIP(hipMemcpy(wc.blk_lds.p, w.blk_lds.data(), w.blk_lds.size() * sizeof(WBlock), hipMemcpyHostToDevice));
+    MG_HIP(hipMemcpy(wc.blk_glob.p, w.blk_glob.data(), w.blk_glob.size() * sizeof(WBlock), hipMemcpyHostToDevice));
+    wc.nblocks = (int)w.blk_lds.size();
+    wc.wbits = W;
+    return 0;
+}
+
+// host staging of one upload: regions appended at 256-byte alignment
+struct Arena {
+    std::vector<unsigned char> h;
+    size_t add(const void* p, size_t n)
+    {
+        const size_t o = align256(h.size());
+        h.resize(o + n);
+        if (n) std::memcpy(h.data() + o, p, n);
+        return o;
+    }
+};
+
+template <typename R>
+static int run_batch(Context* c, const std::vector<int>& idx, BatchReq* reqs, BatchStats* st)
+{
+    hipStream_t s = c->s;
+    const int prec = std::is_same<R, float>::value ? kFP32 : kFP64;
+    const int pi = prec == kFP32 ? 1 : 0;
+    const int T = batch_tuning().threads[pi], W = batch_tuning().wbits[pi];
+    if (!batch_variant_exists(pi, T, W)) {
+        set_error("batched evaluation: no kernel for " + std::to_string(T) + " threads and " + std::to_string(W) +
+                  " window bits");
+        return -1;
+    }
+    const int n = (int)idx.size();
+    const size_t fixed = batch_lds_fixed_bytes(T);
+    const size_t lds_cap = c->lds_limit - 256;  // the kernel also holds 16 bytes of static LDS
+
+    std::vector<BatchInst<R>> insts(n);
+    std::vector<int> cpis(n);
+    size_t lds = fixed, scratch_stride = 0;
+    Arena ar;
+    // per instance: offsets of its gate instances, coefficients and blocks in the arena
+    std::vector<size_t> o_inst(n), o_coef(n), o_blk(n);
+    for (int i = 0; i < n; i++) {
+        const BatchReq& r = reqs[idx[i]];
+        Built built;
+        build_instances(*r.p, r.gates, r.Z, &built);
+        if (r.mode >= 1 && !built.invertible) {
+            set_error("batched evaluation: singular gate block (cannot uncompute)");
+            return -1;
+        }
+        if (prec == kFP32 && ensure_fp32_target(*r.t, s) < 0) return -1;
+        if (r.t->kind == 1 && ensure_windowed_target<R>(*r.t) < 0) return -1;
+        const int L = r.p->L;
+        const size_t S = (size_t)1 << L;
+        const int nsl = r.mode == 2 ? 4 : 2;
+        int cgw, cpi;
+        batch_shape(L, prec, &cgw, &cpi);
+        BatchInst<R>& in = insts[i];
+        in = BatchInst<R>{};
+        in.L = L;
+        in.G = r.p->G;
+        in.P = r.p->P;
+        in.mode = r.mode;
+        in.target_kind = r.t->kind;
+        in.Gt = r.t->kind == 1 ? r.t->Gt : 0;
+        in.acc_off = (int)fixed;
+        in.state_off = (int)align256(fixed + sizeof(double) * 32 * (size_t)in.P);
+        if ((size_t)in.state_off > lds_cap) {
+            set_error("batched evaluation: too many parameter gates for the LDS accumulator");
+            return -1;
+        }
+        in.cgw = cgw;
+        const size_t col_states = (size_t)(T / cgw) * nsl * S;
+        size_t need = in.state_off + col_states * sizeof(cpx<R>);
+        in.lds = need <= lds_cap ? 1 : 0;
+        if (!in.lds) {
+            need = in.state_off;
+            scratch_stride = std::max(scratch_stride, col_states);
+        }
+        cpis[i] = cpi;
+        lds = std::max(lds, need);
+
+        Windowed w;
+        make_windowed<R>(built.inst, built.coef, L, W, &w);
+        std::vector<cpx<R>> cR(w.coef.size());
+        for (size_t k = 0; k < cR.size(); k++) cR[k] = {(R)w.coef[k].re, (R)w.coef[k].im};
+        o_inst[i] = ar.add(w.inst.data(), w.inst.size() * sizeof(Inst));
+        o_coef[i] = ar.add(cR.data(), cR.size() * sizeof(cpx<R>));
+        const std::vector<WBlock>& bl = in.lds ? w.blk_lds : w.blk_glob;
+        o_blk[i] = ar.add(bl.data(), bl.size() * sizeof(WBlock));
+        in.nblocks = (int)bl.size();
+        if (r.t->kind == 0) {
+            in.seed = prec == kFP32 ? (const cpx<R>*)r.t->seed32.p : (const cpx<R>*)r.t->seed64.p;
+        } else {
+            const Target::WinCache& wc = r.t->wc[pi];
+            in.tinst = (const Inst*)wc.inst.p;
+            in.tcoef = (const cpx<R>*)wc.coef.p;
+            in.tblocks = (const WBlock*)(in.lds ? wc.blk_lds.p : wc.blk_glob.p);
+            in.ntblocks = wc.nblocks;
+        }
+    }
+
+    // work items and reduction segments
+    struct ItemCost {
+        BatchItem it;
+        double cost;
+    };
+    std::vector<ItemCost> items;
+    std::vector<ReduceSeg> segs(n);
+    std::vector<int2> blocks;
+    long long poff = 0, ooff = 0;
+    std::vector<long long> out_off(n);
+    for (int i = 0; i < n; i++) {
+        const BatchInst<R>& in = insts[i];
+        const long S = 1L << in.L;
+        const int stride = 2 + 32 * in.P;
+        const double per_col = (double)S * ((double)in.G * (in.mode == 2 ? 18 : in.mode == 1 ? 6 : 2) + 2.0 * in.Gt + 1.0);
+        segs[i].src = poff;
+        segs[i].dst = ooff;
+        segs[i].stride = stride;
+        int cnt = 0;
+        for (long b = 0; b < S; b += cpis[i]) {
+            const long e = std::min<long>(S, b + cpis[i]);
+            items.push_back({BatchItem{i, (int)b, (int)e, 0, poff}, per_col * (double)(e - b)});
+            poff += stride;
+            cnt++;
+        }
+        segs[i].n = cnt;
+        out_off[i] = ooff;
+        for (int k0 = 0; k0 < stride; k0 += 64) blocks.push_back(int2{i, k0});
+        ooff += stride;
+    }
+    std::stable_sort(items.begin(), items.end(), [](const ItemCost& a, const ItemCost& b) { return a.cost > b.cost; });
+    const int nitems = (int)items.size();
+    std::vector<BatchItem> items_h(nitems);
+    for (int k = 0; k < nitems; k++) items_h[k] = items[k].it;
+
+    // packed upload: per-instance arrays (above), items, segments, reduce blocks, descriptors
+    const size_t o_items = ar.add(items_h.data(), items_h.size() * sizeof(BatchItem));
+    const size_t o_segs = ar.add(segs.data(), segs.size() * sizeof(ReduceSeg));
+    const size_t o_rblk = ar.add(blocks.data(), blocks.size() * sizeof(int2));
+    const size_t o_insts = align256(ar.h.size());
+    const size_t up_bytes = align256(o_insts + n * sizeof(BatchInst<R>));
+    MG_HIP(c->b_up.ensure(up_bytes));
+    unsigned char* d = (unsigned char*)c->b_up.p;
+    for (int i = 0; i < n; i++) {
+        insts[i].inst = (const Inst*)(d + o_inst[i]);
+        insts[i].coef = (const cpx<R>*)(d + o_coef[i]);
+        insts[i].blocks = (const WBlock*)(d + o_blk[i]);
+    }
+    if (pinned_ensure(&c->b_up_h, &c->b_up_hn, up_bytes) < 0) { set_error("pinned staging allocation failed"); return -1; }
+    unsigned char* h = (unsigned char*)c->b_up_h;
+    std::memcpy(h, ar.h.data(), ar.h.size());
+    std::memcpy(h + o_insts, insts.data(), n * sizeof(BatchInst<R>));
+
+    const Knobs& kn = knobs();
+    int per_cu = batch_tuning().grid_per_cu;
+    if (per_cu <= 0) per_cu = (int)std::max<size_t>(1, std::min<size_t>(T == 256 ? 4 : 2, c->lds_limit / lds));
+    const int grid = (int)std::max<long>(1, std::min<long>(nitems, (long)c->ncu * per_cu));
+    MG_HIP(c->b_partial.ensure((size_t)poff * sizeof(double)));
+    MG_HIP(c->b_out.ensure((size_t)ooff * sizeof(double)));
+    MG_HIP(c->b_counter.ensure(sizeof(int)));
+    if (scratch_stride) MG_HIP(c->b_scratch.ensure((size_t)grid * scratch_stride * sizeof(cpx<R>)));
+    if (pinned_ensure(&c->b_out_h, &c->b_out_hn, (size_t)ooff * sizeof(double)) < 0) {
+        set_error("pinned staging allocation failed");
+        return -1;
+    }
+
+    MG_HIP(hipMemcpyAsync(c->b_up.p, c->b_up_h, up_bytes, hipMemcpyHostToDevice, s));
+    if (kn.time_kernels) MG_HIP(hipEventRecord(c->ev0, s));
+    MG_HIP(launch_batch<R>((const BatchInst<R>*)(d + o_insts), (const BatchItem*)(d + o_items), nitems, (int*)c->b_counter.p,
+                           (double*)c->b_partial.p, scratch_stride ? (cpx<R>*)c->b_scratch.p : nullptr,
+                           (long long)scratch_stride, T, W, grid, lds, s));
+    if (kn.time_kernels) MG_HIP(hipEventRecord(c->ev1, s));
+    MG_HIP(launch_batch_reduce((const ReduceSeg*)(d + o_segs), (const int2*)(d + o_rblk), (int)blocks.size(),
+                               (const double*)c->b_partial.p, (double*)c->b_out.p, s));
+    MG_HIP(hipMemcpyAsync(c->b_out_h, c->b_out.p, (size_t)ooff * sizeof(double), hipMemcpyDeviceToHost, s));
+    MG_HIP(hipStreamSynchronize(s));
+
+    const double* o = (const double*)c->b_out_h;
+    for (int i = 0; i < n; i++) {
+        const BatchReq& r = reqs[idx[i]];
+        const double* oi = o + out_off[i];
+        const int P = r.p->P;
+        EvalOut* out = r.out;
+        out->f = zd(oi[0], oi[1]);
+        out->grad.assign((size_t)8 * P, zd(0.0));
+        out->dhz.assign((size_t)8 * P, zd(0.0));
+        for (int q = 0; q < P; q++)
+            for (int e = 0; e < 8; e++) {
+                out->grad[8 * q + e] = zd(oi[2 + 32 * q + 2 * e], oi[2 + 32 * q + 2 * e + 1]);
+                out->dhz[8 * q + e] = zd(oi[2 + 32 * q + 16 + 2 * e], oi[2 + 32 * q + 16 + 2 * e + 1]);
+            }
+    }
+    if (st) {
+        st->launches++;
+        st->items += nitems;
+        st->grid = grid;
+        st->lds_bytes = std::max(st->lds_bytes, lds);
+        float ms = 0.f;
+        if (kn.time_kernels) MG_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        st->kernel_ms += ms;
+    }
+    return 0;
+}
+
+int evaluate_batch(BatchReq* reqs, int n, BatchStats* st)
+{
+    Context* c = ctx();
+    if (!c) { set_error("no HIP device available"); return -1; }
+    if (st) *st = BatchStats{};
+    std::vector<int> i32, i64;
+    for (int i = 0; i < n; i++) {
+        BatchReq& r = reqs[i];
+        std::string why;
+        if (!r.t || !r.p || !r.gates || !r.out) { set_error("batched evaluation: null request field"); return -1; }
+        if (!r.p->validate(&why)) { set_error("batched evaluation: program: " + why); return -1; }
+        if (r.t->L != r.p->L) { set_error("batched evaluation: target and program have different qubit counts"); return -1; }
+        if (r.p->L > 16) { set_error("batched evaluation: at most 16 qubits"); return -1; }
+        if (r.mode < 0 || r.mode > 2) { set_error("batched evaluation: invalid mode"); return -1; }
+        if (r.mode == 2 && !r.Z) { set_error("batched evaluation: Hessian-vector product needs a direction"); return -1; }
+        (r.precision == kFP32 ? i32 : i64).push_back(i);
+    }
+    if (!i32.empty() && run_batch<float>(c, i32, reqs, st) < 0) return -1;
+    if (!i64.empty() && run_batch<double>(c, i64, reqs, st) < 0) return -1;
+    return 0;
 }
 
 }  // namespace mg

@@ -49,6 +49,13 @@ struct Target {
     std::vector<Inst> tinst_h;
     std::vector<cpx<double>> tcoef_h;
     bool seed32_ready = false, tcoef32_ready = false;
+    // target program in gate-window form for the batched kernel, per precision (FP64, FP32)
+    struct WinCache {
+        DevBuf inst, coef, blk_lds, blk_glob;
+        int nblocks = 0;
+        int wbits = 0;  // window bits it was built for (0: not built)
+    };
+    WinCache wc[2];
 };
 
 // U row-major (S x S)
@@ -93,5 +100,43 @@ Knobs& knobs();
 
 hipStream_t stream();
 int device_cu_count();
+
+// ---- batched evaluation (one persistent launch per precision over many requests) ----
+struct BatchReq {
+    Target* t = nullptr;
+    const Program* p = nullptr;
+    const zd* gates = nullptr;  // [P*8]
+    const zd* Z = nullptr;      // [P*8] (mode 2)
+    int mode = 0;
+    int precision = 0;
+    EvalOut* out = nullptr;
+};
+
+struct BatchStats {
+    int launches = 0;
+    int items = 0;
+    int grid = 0;
+    size_t lds_bytes = 0;
+    float kernel_ms = 0.f;
+};
+
+// Per (precision, L) shape of the batched kernel: column-group width cgw (threads per
+// column, a power of two up to the 2^(L-W) window tuples) and columns per work item;
+// 0 = default.  Results depend on these (summation
+// order), never on which other requests share the launch.
+struct BatchTuning {
+    int threads[2] = {256, 256};  // per precision (FP64, FP32): threads per workgroup
+    int wbits[2] = {3, 4};        // per precision: gate-window bits
+    int grid_per_cu = 0;
+    int cgw[2][31] = {};
+    int cpi[2][31] = {};
+};
+BatchTuning& batch_tuning();
+void batch_shape(int L, int precision, int* cgw, int* cpi);
+// a batched kernel is compiled for (precision, threads, window bits)
+bool batch_variant_exists(int precision_fp32, int threads, int wbits);
+
+int evaluate_batch(BatchReq* reqs, int n, BatchStats* st);
+BatchStats& last_batch_stats();
 
 }  // namespace mg

@@ -1,0 +1,446 @@
+// Batched evaluation and the optimisation-job server (include/mg_gpu.h).
+//
+// mg_evaluate_batch merges requests of any targets, programs, sizes and modes into
+// one persistent launch per precision (mg_batch.hip) and projects the results like
+// mg_riemannian_gradient_hvp.
+//
+// mg_serve runs Riemannian trust-region optimisations (the reference's
+// riemannian_trust_region_optimize_* loop, src/trust_region.c:235-306, in its HVP
+// form) for a stream of jobs.  With batching on, every admitted job runs its trust
+// region in a host thread of its own; each f / gradient / HVP evaluation the job
+// needs is posted to the server, which launches one batched evaluation as soon as
+// every admitted job has posted one (continuous batching: jobs that arrive while a
+// launch is on the GPU are admitted before the next one, finished jobs leave at once).
+// Targets are created on the device at admission and freed at completion; admission
+// is limited by a job count and by the device memory the admitted jobs hold.
+// Without batching the jobs run one after another, each evaluation its own launch
+// (the plain single-job path).
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <cstring>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "../../include/mg_gpu.h"
+#include "../../include/trust_region.h"
+#include "mg_host.hpp"
+
+using mg::zd;
+
+using mg::riemannian_gradient;
+
+namespace {
+
+const zd* Zp(const struct matchgate* g) { return reinterpret_cast<const zd*>(g); }
+
+// Riemannian outputs of one request from the sweep outputs
+void project_request(const struct matchgate* V, int P, int mode, const struct matchgate* Z, const mg::EvalOut& o,
+                     numeric* f, double* grad_vec, double* hvp)
+{
+    if (f) *f = o.f;
+    if (mode >= 1 && grad_vec) riemannian_gradient(V, P, o.grad.data(), grad_vec);
+    if (mode == 2 && hvp)
+        for (int i = 0; i < P; i++) mg::weingarten_block(&V[i], &o.grad[8 * i], &Z[i], &o.dhz[8 * i], hvp + 8 * i);
+}
+
+// ---- the job server ------------------------------------------------------------------------
+
+struct Server;
+
+struct Posted {
+    mg::BatchReq req;
+    mg::EvalOut out;
+    int status = 0;
+    bool done = false;
+};
+
+struct JobRt {
+    Server* srv = nullptr;
+    mg_job* spec = nullptr;
+    std::unique_ptr<mg::Target> target;
+    size_t footprint = 0;
+    int precision = MG_FP64, switch_last = 0, niter = 0, iter = -1, nhvp = 0, failed = 0;
+    std::thread th;
+    bool finished = false;
+};
+
+struct Server {
+    std::mutex mu;
+    std::condition_variable cv_server, cv_jobs;
+    std::vector<Posted*> posted;
+    int running = 0;
+    bool batched = true;
+    long rounds = 0, requests = 0;
+    double kernel_ms = 0.0;
+};
+
+int job_precision(const JobRt* j)
+{
+    if (j->precision == MG_FP64) return MG_FP64;
+    return (j->iter >= j->niter - j->switch_last) ? MG_FP64 : MG_FP32;
+}
+
+// one evaluation for a job: posted to the server (batched) or launched directly
+int job_eval(JobRt* j, const zd* gates, const zd* Z, int mode, mg::EvalOut* out)
+{
+    const mg::Program& prog = j->spec->program->p;
+    if (!j->srv->batched) {
+        mg::EvalStats st;
+        const int rc = mg::evaluate(*j->target, prog, gates, Z, mode, job_precision(j), out, 0, -1, &st);
+        j->srv->rounds++;
+        j->srv->requests++;
+        j->srv->kernel_ms += st.kernel_ms;
+        return rc;
+    }
+    Posted p;
+    p.req.t = j->target.get();
+    p.req.p = &prog;
+    p.req.gates = gates;
+    p.req.Z = Z;
+    p.req.mode = mode;
+    p.req.precision = job_precision(j);
+    p.req.out = &p.out;
+    std::unique_lock<std::mutex> lk(j->srv->mu);
+    j->srv->posted.push_back(&p);
+    j->srv->cv_server.notify_one();
+    j->srv->cv_jobs.wait(lk, [&] { return p.done; });
+    if (p.status == 0) *out = std::move(p.out);
+    return p.status;
+}
+
+double job_f(const double* x, void* fdata, numeric* f_complex)
+{
+    JobRt* j = (JobRt*)fdata;
+    mg::EvalOut out;
+    if (job_eval(j, reinterpret_cast<const zd*>(x), nullptr, mg::kF, &out) < 0) {
+        j->failed = 1;
+        return NAN;
+    }
+    *f_complex = out.f;
+    return out.f.real();
+}
+
+double job_f_deriv(const double* x, void* fdata, double* grad)
+{
+    JobRt* j = (JobRt*)fdata;
+    j->iter++;
+    mg::EvalOut out;
+    if (job_eval(j, reinterpret_cast<const zd*>(x), nullptr, mg::kFGrad, &out) < 0) {
+        j->failed = 1;
+        return NAN;
+    }
+    riemannian_gradient(reinterpret_cast<const struct matchgate*>(x), j->spec->program->p.P, out.grad.data(), grad);
+    return out.f.real();
+}
+
+void job_hvp(const double* x, void* fdata, double* vec, double* hvp)
+{
+    JobRt* j = (JobRt*)fdata;
+    const int P = j->spec->program->p.P;
+    const struct matchgate* V = reinterpret_cast<const struct matchgate*>(x);
+    std::vector<struct matchgate> Z(P);
+    for (int i = 0; i < P; i++) real_to_tangent_matchgate(vec + 8 * i, &V[i], &Z[i]);
+    mg::EvalOut out;
+    if (job_eval(j, reinterpret_cast<const zd*>(x), Zp(Z.data()), mg::kFGradHvp, &out) < 0) {
+        j->failed = 1;
+        std::fill(hvp, hvp + 8 * P, NAN);
+        return;
+    }
+    j->nhvp++;
+    project_request(V, P, 2, Z.data(), out, nullptr, nullptr, hvp);
+}
+
+void job_retract(const double* x, const double* eta, void* rdata, double* xs)
+{
+    const int P = *(const int*)rdata;
+    const struct matchgate* V = reinterpret_cast<const struct matchgate*>(x);
+    struct matchgate* W = reinterpret_cast<struct matchgate*>(xs);
+    for (int i = 0; i < P; i++) retract_matchgate(&V[i], eta + 8 * i, &W[i]);
+}
+
+// the job's trust region (reference src/mg_brickwall_opt.c:99-122 with the HVP solver)
+void run_job(JobRt* j)
+{
+    mg_job* s = j->spec;
+    int P = s->program->p.P;
+    struct rtr_params prm;
+    set_rtr_default_params(16 * P, &prm);
+    riemannian_trust_region_optimize_hvp(job_f, job_f_deriv, job_hvp, j, job_retract, &P, 8 * P,
+                                         reinterpret_cast<const double*>(s->gates_start), 16 * P, &prm, s->niter,
+                                         s->f_iter, reinterpret_cast<double*>(s->gates_opt));
+    s->nhvp = j->nhvp;
+    s->status = j->failed ? -1 : 0;
+}
+
+// device memory a job holds while admitted: its target, its coefficients and its
+// share of the batched partial sums (two precisions)
+size_t job_footprint(const mg_job& s)
+{
+    const int L = s.nqubits;
+    const size_t S = (size_t)1 << L;
+    const mg::Program& p = s.program->p;
+    size_t b = 0;
+    if (s.U) b += S * S * (sizeof(double) * 2 + sizeof(float) * 2);
+    else b += (size_t)s.target_ngates * (mg::kCoefPerInst * 24 + sizeof(mg::Inst));
+    b += (size_t)p.G * (mg::kCoefPerInst * 16 + sizeof(mg::Inst));
+    int cgw, cpi;
+    mg::batch_shape(L, mg::kFP32, &cgw, &cpi);
+    b += 2 * (S / cpi + 1) * (2 + 32 * (size_t)p.P) * sizeof(double);
+    return b;
+}
+
+mg::Target* make_target(const mg_job& s)
+{
+    if (s.U) return mg::target_dense(reinterpret_cast<const zd*>(s.U), s.nqubits);
+    mg::Program tp;
+    tp.L = s.nqubits;
+    tp.G = s.target_ngates;
+    tp.P = s.target_ngates;
+    tp.param.resize(tp.G);
+    tp.wires.assign(s.target_wires, s.target_wires + 2 * tp.G);
+    for (int g = 0; g < tp.G; g++) tp.param[g] = g;
+    return mg::target_program(tp, Zp(s.target_gates));
+}
+
+bool valid_job(const mg_job& s)
+{
+    if (!s.program || !s.gates_start || !s.f_iter || !s.gates_opt || s.niter < 0) return false;
+    if (s.nqubits != s.program->p.L || s.nqubits > 16) return false;
+    if (!s.U && (s.target_ngates < 1 || !s.target_gates || !s.target_wires)) return false;
+    return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mg_evaluate_batch(mg_request* reqs, int n)
+{
+    if (n < 0 || (n > 0 && !reqs)) {
+        mg::set_error("mg_evaluate_batch: invalid argument");
+        return -1;
+    }
+    std::vector<mg::BatchReq> br(n);
+    std::vector<mg::EvalOut> outs(n);
+    std::vector<std::vector<struct matchgate>> Z(n);
+    for (int i = 0; i < n; i++) {
+        const mg_request& r = reqs[i];
+        if (!r.target || !r.program || !r.gates || r.what < MG_EVAL_F || r.what > MG_EVAL_F_GRAD_HVP ||
+            (r.what == MG_EVAL_F_GRAD_HVP && !r.v)) {
+            mg::set_error("mg_evaluate_batch: invalid request " + std::to_string(i));
+            return -1;
+        }
+        const int P = r.program->p.P;
+        if (r.what == MG_EVAL_F_GRAD_HVP) {
+            Z[i].resize(P);
+            for (int k = 0; k < P; k++) real_to_tangent_matchgate(r.v + 8 * k, &r.gates[k], &Z[i][k]);
+        }
+        br[i].t = r.target->t.get();
+        br[i].p = &r.program->p;
+        br[i].gates = Zp(r.gates);
+        br[i].Z = Z[i].empty() ? nullptr : Zp(Z[i].data());
+        br[i].mode = r.what;
+        br[i].precision = r.precision == MG_FP32 ? mg::kFP32 : mg::kFP64;
+        br[i].out = &outs[i];
+    }
+    mg::BatchStats st;
+    if (mg::evaluate_batch(br.data(), n, &st) < 0) return -1;
+    mg::last_batch_stats() = st;
+    for (int i = 0; i < n; i++) {
+        const mg_request& r = reqs[i];
+        project_request(r.gates, r.program->p.P, r.what, Z[i].empty() ? nullptr : Z[i].data(), outs[i], r.f,
+                        r.grad_vec, r.hvp);
+    }
+    return 0;
+}
+
+int mg_batch_set_kernel(int precision, int threads, int window_bits)
+{
+    const int pi = precision == MG_FP32 ? 1 : 0;
+    if (!mg::batch_variant_exists(pi, threads, window_bits)) {
+        mg::set_error("mg_batch_set_kernel: no kernel for this precision, thread count and window size");
+        return -1;
+    }
+    mg::batch_tuning().threads[pi] = threads;
+    mg::batch_tuning().wbits[pi] = window_bits;
+    return 0;
+}
+
+void mg_batch_get_kernel(int precision, int* threads, int* window_bits)
+{
+    const int pi = precision == MG_FP32 ? 1 : 0;
+    if (threads) *threads = mg::batch_tuning().threads[pi];
+    if (window_bits) *window_bits = mg::batch_tuning().wbits[pi];
+}
+
+void mg_batch_set_grid_per_cu(int n) { mg::batch_tuning().grid_per_cu = n > 0 ? n : 0; }
+
+void mg_batch_set_shape(int precision, int nqubits, int threads_per_column, int columns_per_item)
+{
+    if (nqubits < 0 || nqubits > 30) return;
+    const int pi = precision == MG_FP32 ? 1 : 0;
+    mg::batch_tuning().cgw[pi][nqubits] = threads_per_column;
+    mg::batch_tuning().cpi[pi][nqubits] = columns_per_item;
+}
+
+void mg_batch_get_shape(int precision, int nqubits, int* threads_per_column, int* columns_per_item)
+{
+    int cgw = 0, cpi = 0;
+    if (nqubits >= 2 && nqubits <= 30) mg::batch_shape(nqubits, precision == MG_FP32 ? mg::kFP32 : mg::kFP64, &cgw, &cpi);
+    if (threads_per_column) *threads_per_column = cgw;
+    if (columns_per_item) *columns_per_item = cpi;
+}
+
+float mg_batch_last_kernel_ms(void) { return mg::last_batch_stats().kernel_ms; }
+
+int mg_serve(mg_job* jobs, int njobs, const mg_serve_options* opt, mg_serve_stats* stats)
+{
+    if (njobs < 0 || (njobs > 0 && !jobs) || !opt) {
+        mg::set_error("mg_serve: invalid argument");
+        return -1;
+    }
+    for (int i = 0; i < njobs; i++)
+        if (!valid_job(jobs[i])) {
+            mg::set_error("mg_serve: invalid job " + std::to_string(i));
+            return -1;
+        }
+    using clock = std::chrono::steady_clock;
+    const auto t0 = clock::now();
+    auto now_s = [&] { return std::chrono::duration<double>(clock::now() - t0).count(); };
+    std::vector<int> order(njobs);
+    for (int i = 0; i < njobs; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return jobs[a].arrival < jobs[b].arrival; });
+    auto arrival = [&](int k) { return opt->realtime ? jobs[order[k]].arrival : 0.0; };
+
+    Server srv;
+    srv.batched = opt->batched != 0;
+    std::vector<std::unique_ptr<JobRt>> rt(njobs);
+    int next = 0, done = 0, peak_active = 0, rc = 0;
+    size_t held = 0, peak_held = 0;
+
+    auto start_job = [&](int i) -> bool {
+        auto j = std::make_unique<JobRt>();
+        j->srv = &srv;
+        j->spec = &jobs[i];
+        j->precision = opt->precision == MG_FP32 ? MG_FP32 : MG_FP64;
+        j->switch_last = std::max(0, opt->precision_switch);
+        j->niter = jobs[i].niter;
+        j->footprint = job_footprint(jobs[i]);
+        jobs[i].t_admit = now_s();
+        j->target.reset(make_target(jobs[i]));
+        if (!j->target) {
+            jobs[i].status = -1;
+            return false;
+        }
+        rt[i] = std::move(j);
+        return true;
+    };
+
+    if (!srv.batched) {
+        // one job at a time, one launch per evaluation
+        for (int k = 0; k < njobs; k++) {
+            const int i = order[k];
+            const double wait = arrival(k) - now_s();
+            if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(wait));
+            if (!start_job(i)) { rc = -1; continue; }
+            peak_active = 1;
+            peak_held = std::max(peak_held, rt[i]->footprint);
+            run_job(rt[i].get());
+            rt[i]->target.reset();
+            jobs[i].t_done = now_s();
+            if (jobs[i].status != 0) rc = -1;
+        }
+    } else {
+        std::unique_lock<std::mutex> lk(srv.mu);
+        while (done < njobs) {
+            // admit arrived jobs while slots and device memory allow
+            while (next < njobs && arrival(next) <= now_s()) {
+                const int i = order[next];
+                const size_t fp = job_footprint(jobs[i]);
+                const bool slot = opt->max_active <= 0 || srv.running < opt->max_active;
+                const bool mem = opt->hbm_budget <= 0 || srv.running == 0 || (double)(held + fp) <= opt->hbm_budget;
+                if (!slot || !mem) break;
+                lk.unlock();
+                const bool ok = start_job(i);
+                lk.lock();
+                next++;
+                if (!ok) {
+                    jobs[i].t_done = now_s();
+                    done++;
+                    rc = -1;
+                    continue;
+                }
+                held += rt[i]->footprint;
+                peak_held = std::max(peak_held, held);
+                srv.running++;
+                peak_active = std::max(peak_active, srv.running);
+                JobRt* j = rt[i].get();
+                j->th = std::thread([j, &srv] {
+                    run_job(j);
+                    std::lock_guard<std::mutex> g(srv.mu);
+                    j->finished = true;
+                    srv.running--;
+                    srv.cv_server.notify_one();
+                });
+            }
+            // retire finished jobs
+            for (int k = 0; k < next; k++) {
+                JobRt* j = rt[order[k]].get();
+                if (j && j->finished && j->th.joinable()) {
+                    j->th.join();
+                    j->target.reset();
+                    held -= j->footprint;
+                    j->spec->t_done = now_s();
+                    if (j->spec->status != 0) rc = -1;
+                    done++;
+                }
+            }
+            if (done >= njobs) break;
+            // launch when every admitted job waits on an evaluation
+            if (srv.running > 0 && (int)srv.posted.size() == srv.running) {
+                std::vector<Posted*> batch;
+                batch.swap(srv.posted);
+                lk.unlock();
+                std::vector<mg::BatchReq> reqs(batch.size());
+                for (size_t k = 0; k < batch.size(); k++) reqs[k] = batch[k]->req;
+                mg::BatchStats st;
+                const int r = mg::evaluate_batch(reqs.data(), (int)reqs.size(), &st);
+                lk.lock();
+                srv.rounds++;
+                srv.requests += (long)batch.size();
+                srv.kernel_ms += st.kernel_ms;
+                for (Posted* p : batch) {
+                    p->status = r;
+                    p->done = true;
+                }
+                srv.cv_jobs.notify_all();
+                continue;
+            }
+            // otherwise wait for a posting, a finished job or the next arrival
+            const bool jobs_pending = srv.running > 0 || next < njobs;
+            if (!jobs_pending) break;
+            if (next < njobs && srv.running == 0) {
+                const double wait = arrival(next) - now_s();
+                if (wait > 0) srv.cv_server.wait_for(lk, std::chrono::duration<double>(wait));
+            } else if (next < njobs && arrival(next) > now_s()) {
+                srv.cv_server.wait_for(lk, std::chrono::duration<double>(arrival(next) - now_s()));
+            } else {
+                srv.cv_server.wait(lk);
+            }
+        }
+    }
+    if (stats) {
+        stats->wall_s = now_s();
+        stats->rounds = srv.rounds;
+        stats->requests = srv.requests;
+        stats->kernel_ms = srv.kernel_ms;
+        stats->max_active = peak_active;
+        stats->peak_footprint = (double)peak_held;
+    }
+    return rc;
+}
+
+}  // extern "C"

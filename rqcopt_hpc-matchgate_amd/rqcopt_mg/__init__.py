@@ -54,6 +54,32 @@ class RTRParams(ctypes.Structure):
                 ("g_func", _P), ("g_data", _P), ("g_iter", _P)]
 
 
+class Request(ctypes.Structure):
+    """struct mg_request (include/mg_gpu.h)"""
+    _fields_ = [("target", _P), ("program", _P), ("gates", _P), ("v", _P), ("what", ctypes.c_int),
+                ("precision", ctypes.c_int), ("f", _P), ("grad_vec", _P), ("hvp", _P)]
+
+
+class Job(ctypes.Structure):
+    """struct mg_job (include/mg_gpu.h)"""
+    _fields_ = [("nqubits", ctypes.c_int), ("program", _P), ("gates_start", _P), ("U", _P),
+                ("target_ngates", ctypes.c_int), ("target_gates", _P), ("target_wires", _P),
+                ("niter", ctypes.c_int), ("arrival", _D), ("f_iter", _P), ("gates_opt", _P),
+                ("t_admit", _D), ("t_done", _D), ("nhvp", ctypes.c_int), ("status", ctypes.c_int)]
+
+
+class ServeOptions(ctypes.Structure):
+    """struct mg_serve_options (include/mg_gpu.h)"""
+    _fields_ = [("batched", ctypes.c_int), ("precision", ctypes.c_int), ("precision_switch", ctypes.c_int),
+                ("max_active", ctypes.c_int), ("hbm_budget", _D), ("realtime", ctypes.c_int)]
+
+
+class ServeStats(ctypes.Structure):
+    """struct mg_serve_stats (include/mg_gpu.h)"""
+    _fields_ = [("wall_s", _D), ("rounds", ctypes.c_long), ("requests", ctypes.c_long), ("kernel_ms", _D),
+                ("max_active", ctypes.c_int), ("peak_footprint", _D)]
+
+
 def rtr_default_params(n):
     """set_rtr_default_params (reference src/trust_region.h:69-78)"""
     return RTRParams(TruncatedCGParams(2 * n, 1e-8, 1e-6), 0.125, 0.01, 0.1, None, None, None)
@@ -93,6 +119,14 @@ _sig("mg_last_sweep_threads", [])
 _sig("mg_last_kernel_ms", [], ctypes.c_float)
 _sig("mg_last_sweep_info", [_P, _P, _P, _P], None)
 _sig("mg_target_cache_clear", [], None)
+_sig("mg_evaluate_batch", [ctypes.POINTER(Request), _I])
+_sig("mg_batch_set_kernel", [_I, _I, _I])
+_sig("mg_batch_get_kernel", [_I, _P, _P], None)
+_sig("mg_batch_set_grid_per_cu", [_I], None)
+_sig("mg_batch_set_shape", [_I, _I, _I, _I], None)
+_sig("mg_batch_get_shape", [_I, _I, _P, _P], None)
+_sig("mg_batch_last_kernel_ms", [], ctypes.c_float)
+_sig("mg_serve", [ctypes.POINTER(Job), _I, ctypes.POINTER(ServeOptions), ctypes.POINTER(ServeStats)])
 # reference entry points (include/matchgate_target.h, mg_brickwall_opt.h)
 _sig("matchgate_circuit_unitary_target", [LINEAR_FUNC, _P, _P, _I, _P, _I, _P])
 _sig("parallel_matchgate_circuit_unitary_target", [LINEAR_FUNC, _P, _P, _I, _P, _I, _P])
@@ -289,6 +323,109 @@ def optimize(target, program, gates, niter, hvp=True, precision=MG_FP64, precisi
     _check(lib.mg_optimize(target.h, program.h, _p(gates), ctypes.byref(params), niter, int(hvp), precision,
                            precision_switch, _p(f_iter), _p(f_citer), _p(out), _p(nh)), "mg_optimize")
     return f_iter, f_citer, out, int(nh[0])
+
+
+# ---------------- batched evaluation and serving ----------------
+
+def evaluate_batch(requests):
+    """Evaluate many requests in shared launches.
+
+    requests: list of dicts with keys target, program, gates, v (or None), what,
+    precision.  Returns a list of (f, grad_vec, hvp) like riemannian_gradient_hvp
+    (grad_vec None for what == EVAL_F, hvp None unless what == EVAL_F_GRAD_HVP).
+    """
+    n = len(requests)
+    arr = (Request * n)()
+    keep, outs = [], []
+    for k, r in enumerate(requests):
+        prog = r["program"]
+        gates = _c(r["gates"])
+        what = int(r.get("what", EVAL_F_GRAD_HVP))
+        v = r.get("v")
+        vv = np.ascontiguousarray(v, dtype=np.float64) if v is not None else None
+        f = np.zeros(1, np.complex128)
+        g = np.zeros(8 * prog.P)
+        hv = np.zeros(8 * prog.P)
+        keep += [gates, vv, f, g, hv]
+        arr[k] = Request(r["target"].h, prog.h, gates.ctypes.data, vv.ctypes.data if vv is not None else None, what,
+                         int(r.get("precision", MG_FP64)), f.ctypes.data, g.ctypes.data, hv.ctypes.data)
+        outs.append((f, g, hv, what))
+    _check(lib.mg_evaluate_batch(arr, n), "mg_evaluate_batch")
+    return [(f[0], g if what >= EVAL_F_GRAD else None, hv if what == EVAL_F_GRAD_HVP else None)
+            for f, g, hv, what in outs]
+
+
+BATCH_VARIANTS = {MG_FP32: [(256, 4), (256, 3), (256, 2), (512, 3), (512, 2)],
+                  MG_FP64: [(256, 3), (256, 2), (512, 2)]}
+
+
+def batch_set_kernel(precision, threads, window_bits):
+    _check(lib.mg_batch_set_kernel(int(precision), int(threads), int(window_bits)), "mg_batch_set_kernel")
+
+
+def batch_get_kernel(precision):
+    a, b = ctypes.c_int(), ctypes.c_int()
+    lib.mg_batch_get_kernel(int(precision), ctypes.byref(a), ctypes.byref(b))
+    return a.value, b.value
+
+
+def batch_set_grid_per_cu(n):
+    lib.mg_batch_set_grid_per_cu(int(n))
+
+
+def batch_set_shape(precision, L, threads_per_column=0, columns_per_item=0):
+    lib.mg_batch_set_shape(int(precision), int(L), int(threads_per_column), int(columns_per_item))
+
+
+def batch_get_shape(precision, L):
+    a, b = ctypes.c_int(), ctypes.c_int()
+    lib.mg_batch_get_shape(int(precision), int(L), ctypes.byref(a), ctypes.byref(b))
+    return a.value, b.value
+
+
+def batch_last_kernel_ms():
+    return float(lib.mg_batch_last_kernel_ms())
+
+
+def serve(jobs, batched=True, precision=MG_FP32, precision_switch=3, max_active=0, hbm_budget=0.0,
+          realtime=True):
+    """Run a stream of trust-region optimisation jobs (mg_serve).
+
+    jobs: list of dicts with keys program (Program), gates (P, 8), niter, arrival (s),
+    and either U (dense target) or target_gates + target_wires (circuit target).
+    Returns (results, stats): per job a dict with f_iter, gates, t_admit, t_done,
+    latency (t_done - arrival), nhvp, status; stats a dict of mg_serve_stats.
+    """
+    n = len(jobs)
+    arr = (Job * n)()
+    keep, outs = [], []
+    for k, j in enumerate(jobs):
+        prog = j["program"]
+        g0 = _c(j["gates"])
+        niter = int(j["niter"])
+        f_iter = np.zeros(niter + 1)
+        gout = np.zeros_like(g0)
+        U = _c(j["U"]) if j.get("U") is not None else None
+        tg = _c(j["target_gates"]) if U is None else None
+        tw = _i(j["target_wires"]) if U is None else None
+        keep += [prog, g0, U, tg, tw]
+        arr[k] = Job(prog.L, prog.h, g0.ctypes.data, U.ctypes.data if U is not None else None,
+                     tg.shape[0] if tg is not None else 0, tg.ctypes.data if tg is not None else None,
+                     tw.ctypes.data if tw is not None else None, niter, float(j.get("arrival", 0.0)),
+                     f_iter.ctypes.data, gout.ctypes.data, 0.0, 0.0, 0, 0)
+        outs.append((f_iter, gout))
+    opt = ServeOptions(int(bool(batched)), int(precision), int(precision_switch), int(max_active), float(hbm_budget),
+                       int(bool(realtime)))
+    st = ServeStats()
+    _check(lib.mg_serve(arr, n, ctypes.byref(opt), ctypes.byref(st)), "mg_serve")
+    res = []
+    for k, (f_iter, gout) in enumerate(outs):
+        a = arr[k]
+        arrival = float(jobs[k].get("arrival", 0.0)) if realtime else 0.0
+        res.append({"f_iter": f_iter, "gates": gout, "t_admit": a.t_admit, "t_done": a.t_done,
+                    "latency": a.t_done - arrival, "nhvp": a.nhvp, "status": a.status})
+    stats = {name: getattr(st, name) for name, _ in ServeStats._fields_}
+    return res, stats
 
 
 def set_checkpointing(mode):

@@ -79,3 +79,46 @@ def sweep_flops(L, ngates, ntarget_gates=0):
     matrix-free target gate; 8 real flops per complex MAC."""
     S = 1 << L
     return 8.0 * S * S * (18 * ngates + 2 * ntarget_gates + 1)
+
+
+def brickwall_job(rng, L, depth, niter, target_factor=3):
+    """One optimisation job: an open brickwall of `depth` layers with seeded Haar
+    gates, and a target that is a seeded random open brickwall of target_factor x depth
+    layers (a circuit target, applied matrix-free)."""
+    param, wires, G = open_brickwall(L, depth)
+    gates = haar_matchgates(rng, G)
+    _, twires, Gt = open_brickwall(L, target_factor * depth)
+    tgates = haar_matchgates(rng, Gt)
+    return {"L": L, "depth": depth, "param": param, "wires": wires, "G": G, "gates": gates,
+            "target_gates": tgates, "target_wires": twires, "Gt": Gt, "niter": int(niter)}
+
+
+def batched_instances(n=64, seed=0, Ls=(8, 10, 12, 14), depths=(3, 5, 7, 9, 11), niter=20):
+    """The batched workload: n independent instances with L uniform in Ls, odd depth
+    uniform in depths, targets random brickwalls of 3x the depth."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        L = int(rng.choice(Ls))
+        d = int(rng.choice(depths))
+        out.append(brickwall_job(rng, L, d, niter))
+    return out
+
+
+def served_stream(n, seed=1, rate=200.0, Ls=(8, 10, 12, 14, 16), depths=tuple(range(3, 16, 2)),
+                  budget=(5, 50)):
+    """The served stream: Poisson arrivals (mean `rate` jobs/s) of jobs with L uniform
+    in Ls, odd depth uniform in depths, iteration budget uniform in [budget[0],
+    budget[1]], targets random brickwalls of 3x the depth."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for _ in range(n):
+        t += rng.exponential(1.0 / rate)
+        L = int(rng.choice(Ls))
+        d = int(rng.choice(depths))
+        it = int(rng.integers(budget[0], budget[1] + 1))
+        job = brickwall_job(rng, L, d, it)
+        job["arrival"] = t
+        out.append(job)
+    return out

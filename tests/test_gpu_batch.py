@@ -1,0 +1,189 @@
+"""Batched evaluation and serving on the GPU.
+
+- mg_evaluate_batch: every request of a mixed batch (qubit counts 6-12, circuit and
+  dense targets, f / f+grad / f+grad+HVP, FP64 and FP32) against the oracle (FP64:
+  1e-11 relative; FP32 sweeps: 2e-5) and against the single-request path;
+- a request gives the same bits alone and in any mix (the batched kernel's summation
+  order depends only on the request's own size);
+- mg_serve: continuous batching of trust-region jobs reproduces each job run alone
+  bitwise, and the one-job-at-a-time path within 1e-9 (different kernels).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as orc
+from dense_ref import program_matrix
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def mg():
+    import rqcopt_mg
+
+    return rqcopt_mg
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def rel(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return np.max(np.abs(a - b)) / max(1e-300, np.max(np.abs(b)))
+
+
+def make_instance(mg, rng, L, depth, dense=False):
+    from rqcopt_mg.workloads import brickwall_job
+
+    job = brickwall_job(rng, L, depth, 5)
+    prog = mg.Program.create(L, job["G"], job["param"], job["wires"])
+    if dense:
+        W = program_matrix(job["target_gates"], np.arange(job["Gt"]), job["target_wires"], L)
+        tgt = mg.Target.dense(W)
+    else:
+        tgt = mg.Target.circuit(job["target_gates"], job["target_wires"], L)
+    v = rng.standard_normal(8 * job["G"])
+    return job, prog, tgt, v
+
+
+def oracle_riemannian(job, v, L):
+    """f, Riemannian gradient and HVP from the oracle (dense target, FP64)."""
+    W = program_matrix(job["target_gates"], np.arange(job["Gt"]), job["target_wires"], L)
+    V = job["gates"]
+    Z = orc.real_to_directions(V, v)
+    f, dg, dh = orc.target_gradient_hvp(W, V, Z, job["param"], job["wires"], L)
+    return f, orc.riemannian_gradient(V, dg), orc.riemannian_hvp(V, dg, Z, dh)
+
+
+MIX = [(6, 3, False), (8, 5, True), (8, 3, False), (10, 4, False), (6, 5, True), (12, 3, False)]
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_batch_matches_oracle_and_single_path(mg, prec):
+    rng = np.random.default_rng(11)
+    P = mg.MG_FP64 if prec == "fp64" else mg.MG_FP32
+    tol = 1e-11 if prec == "fp64" else 2e-5
+    inst = [make_instance(mg, rng, L, d, dense) for L, d, dense in MIX]
+    reqs = []
+    for k, (job, prog, tgt, v) in enumerate(inst):
+        what = [mg.EVAL_F_GRAD_HVP, mg.EVAL_F_GRAD, mg.EVAL_F][k % 3] if k < 3 else mg.EVAL_F_GRAD_HVP
+        reqs.append({"target": tgt, "program": prog, "gates": job["gates"], "v": v, "what": what, "precision": P})
+    say(f"case: batch of {len(reqs)} requests, {prec}, L {[m[0] for m in MIX]}")
+    out = mg.evaluate_batch(reqs)
+    for (job, prog, tgt, v), r, (f, g, hv) in zip(inst, reqs, out):
+        L = job["L"]
+        fs, gs, hs = mg.riemannian_gradient_hvp(tgt, prog, job["gates"], v, P)
+        if L <= 10:
+            fo, go, ho = oracle_riemannian(job, v, L)
+            assert abs(f - fo) <= tol * abs(fo), (L, f, fo)
+            if r["what"] >= mg.EVAL_F_GRAD:
+                assert rel(g, go) < tol, (L, rel(g, go))
+            if r["what"] == mg.EVAL_F_GRAD_HVP:
+                assert rel(hv, ho) < tol, (L, rel(hv, ho))
+        # the single-request path (different kernels): same values to rounding
+        tol_s = 1e-12 if prec == "fp64" else 2e-5
+        assert abs(f - fs) <= tol_s * abs(fs)
+        if r["what"] >= mg.EVAL_F_GRAD:
+            assert rel(g, gs) < tol_s
+        if r["what"] == mg.EVAL_F_GRAD_HVP:
+            assert rel(hv, hs) < tol_s
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_batch_alone_equals_mixed_bitwise(mg, prec):
+    rng = np.random.default_rng(12)
+    P = mg.MG_FP64 if prec == "fp64" else mg.MG_FP32
+    inst = [make_instance(mg, rng, L, d, dense) for L, d, dense in MIX]
+    reqs = [{"target": t, "program": p, "gates": j["gates"], "v": v, "what": mg.EVAL_F_GRAD_HVP, "precision": P}
+            for j, p, t, v in inst]
+    say(f"case: alone vs mixed, {prec}")
+    mixed = mg.evaluate_batch(reqs)
+    mixed_rev = mg.evaluate_batch(reqs[::-1])[::-1]
+    for k, r in enumerate(reqs):
+        alone = mg.evaluate_batch([r])[0]
+        for a, b, c in zip(alone, mixed[k], mixed_rev[k]):
+            assert np.array_equal(np.asarray(a), np.asarray(b)) and np.array_equal(np.asarray(a), np.asarray(c)), k
+
+
+def test_batch_global_states_and_shapes(mg):
+    """FP64 at L=12 keeps its states in global scratch; other column-group widths and
+    item sizes change only the summation order."""
+    rng = np.random.default_rng(13)
+    job, prog, tgt, v = make_instance(mg, rng, 12, 3)
+    jb, pb, tb, vb = make_instance(mg, rng, 8, 4)
+    say("case: batch, L=12 FP64 global states + L=8 with non-default shapes")
+    base = mg.evaluate_batch([
+        {"target": tgt, "program": prog, "gates": job["gates"], "v": v, "what": 2, "precision": mg.MG_FP64},
+        {"target": tb, "program": pb, "gates": jb["gates"], "v": vb, "what": 2, "precision": mg.MG_FP64}])
+    fs, gs, hs = mg.riemannian_gradient_hvp(tgt, prog, job["gates"], v, mg.MG_FP64)
+    assert abs(base[0][0] - fs) <= 1e-12 * abs(fs) and rel(base[0][1], gs) < 1e-12 and rel(base[0][2], hs) < 1e-12
+    try:
+        # (threads per column, columns per item): 16 and 8 threads per column put 16 and
+        # 32 columns of L=8 FP64 states into one workgroup, beyond the LDS (global scratch)
+        for cgw, cpi in [(16, 16), (8, 40), (32, 64), (4, 3)]:
+            mg.batch_set_shape(mg.MG_FP64, 8, cgw, cpi)
+            o = mg.evaluate_batch([{"target": tb, "program": pb, "gates": jb["gates"], "v": vb, "what": 2,
+                                    "precision": mg.MG_FP64}])[0]
+            assert abs(o[0] - base[1][0]) <= 1e-12 * abs(base[1][0])
+            assert rel(o[1], base[1][1]) < 1e-12 and rel(o[2], base[1][2]) < 1e-12
+    finally:
+        mg.batch_set_shape(mg.MG_FP64, 8, 0, 0)
+
+
+def _jobs(mg, seed, spec):
+    from rqcopt_mg.workloads import brickwall_job
+
+    rng = np.random.default_rng(seed)
+    out = []
+    for k, (L, d, it) in enumerate(spec):
+        j = brickwall_job(rng, L, d, it)
+        out.append({"program": mg.Program.create(L, j["G"], j["param"], j["wires"]), "gates": j["gates"],
+                    "niter": it, "target_gates": j["target_gates"], "target_wires": j["target_wires"],
+                    "arrival": 0.002 * k})
+    return out
+
+
+SERVE_SPEC = [(6, 3, 6), (8, 5, 4), (8, 3, 7), (10, 3, 3), (6, 5, 5)]
+
+
+def test_serve_batched_equals_alone_and_single_path(mg):
+    jobs = _jobs(mg, 21, SERVE_SPEC)
+    say("case: serve 5 jobs batched (FP32 sweeps, last 3 iterations FP64)")
+    res, st = mg.serve(jobs, batched=True, precision=mg.MG_FP32, precision_switch=3, realtime=False)
+    assert all(r["status"] == 0 for r in res)
+    assert st["max_active"] == len(jobs) and st["rounds"] < st["requests"]
+    say("case: the same jobs with real-time arrivals")
+    rt, _ = mg.serve(jobs, batched=True, precision=mg.MG_FP32, precision_switch=3, realtime=True)
+    for a, b in zip(rt, res):
+        assert np.array_equal(a["f_iter"], b["f_iter"]) and np.array_equal(a["gates"], b["gates"])
+    for k, j in enumerate(jobs):
+        say(f"case: serve job {k} alone")
+        alone, _ = mg.serve([dict(j, arrival=0.0)], batched=True, precision=mg.MG_FP32, precision_switch=3)
+        assert np.array_equal(alone[0]["f_iter"], res[k]["f_iter"]), k
+        assert np.array_equal(alone[0]["gates"], res[k]["gates"]), k
+    say("case: serve FP64 batched vs one job at a time")
+    b64, _ = mg.serve(jobs, batched=True, precision=mg.MG_FP64, realtime=False)
+    s64, st1 = mg.serve(jobs, batched=False, precision=mg.MG_FP64, realtime=False)
+    assert st1["rounds"] == st1["requests"]
+    for a, b in zip(b64, s64):
+        assert np.max(np.abs(a["f_iter"] - b["f_iter"])) < 1e-9
+        assert np.max(np.abs(a["gates"] - b["gates"])) < 1e-9
+    # the FP32-sweep trajectories end on FP64 iterations close to the FP64 ones
+    for a, b in zip(res, b64):
+        assert abs(a["f_iter"][-1] - b["f_iter"][-1]) < 1e-3 * abs(b["f_iter"][-1])
+
+
+def test_serve_admission_limits(mg):
+    jobs = _jobs(mg, 22, SERVE_SPEC[:4])
+    say("case: serve with one slot and with a tight memory budget")
+    free, _ = mg.serve(jobs, batched=True, precision=mg.MG_FP64, realtime=False)
+    one, st = mg.serve(jobs, batched=True, precision=mg.MG_FP64, realtime=False, max_active=1)
+    assert st["max_active"] == 1
+    tight, st2 = mg.serve(jobs, batched=True, precision=mg.MG_FP64, realtime=False, hbm_budget=1.0)
+    assert st2["max_active"] == 1
+    for a, b, c in zip(free, one, tight):
+        assert np.array_equal(a["f_iter"], b["f_iter"]) and np.array_equal(a["f_iter"], c["f_iter"])
