@@ -9,6 +9,9 @@ Hessian-vector product over all 4096 basis columns (one trust-region iteration
 unit).  The default precision is FP64 sweeps; --precision fp32 runs FP32 sweeps
 with FP64 accumulation.
 
+Also measured: the served stream of trust-region jobs (continuous batching, p50/p99
+latency, against the one-job-at-a-time FP64 path).
+
 Prints one JSON line.  `python bench.py --gpus 1 --steps K --warmup W`.
 """
 import argparse
@@ -83,6 +86,60 @@ def copy_calibration(mib):
     print(json.dumps({"copy_bytes": n, "copies": 5}))
 
 
+def served_bench(mg, njobs, seed=1):
+    """The served stream: Poisson arrivals (200 jobs/s, seed 1) of trust-region jobs with
+    L in {8, 10, 12}, odd depth 3..15, budget 5..50 iterations, circuit targets of 3x the
+    depth; FP32 sweeps with the last 3 iterations in FP64, continuous batching, targets
+    materialised at admission; against the same stream run one job at a time in FP64 with
+    one launch per evaluation."""
+    from rqcopt_mg.workloads import served_stream
+
+    stream = served_stream(njobs, seed=seed, rate=200.0, Ls=(8, 10, 12))
+    jobs = [{"program": mg.Program.create(j["L"], j["G"], j["param"], j["wires"]), "gates": j["gates"],
+             "niter": j["niter"], "target_gates": j["target_gates"], "target_wires": j["target_wires"],
+             "arrival": j["arrival"]} for j in stream]
+    # autotune the batched kernel on one HVP per job of the first 16 (their initial gates)
+    rng = np.random.default_rng(seed)
+    calib = []
+    for j in jobs[:16]:
+        t = mg.Target.circuit(j["target_gates"], j["target_wires"], j["program"].L).densify()
+        calib.append({"target": t, "program": j["program"], "gates": j["gates"],
+                      "v": rng.standard_normal(8 * j["program"].P), "what": mg.EVAL_F_GRAD_HVP})
+    tune = {"fp32": mg.batch_autotune(calib, mg.MG_FP32), "fp64": mg.batch_autotune(calib, mg.MG_FP64)}
+    log(f"autotuned (threads, window bits, workgroups/CU, ms): {tune}")
+    log(f"serving {njobs} jobs with continuous batching ...")
+    res, st = mg.serve(jobs, batched=True, precision=mg.MG_FP32, precision_switch=3, densify_qubits=12,
+                       realtime=True)
+    log(f"  {njobs / st['wall_s']:.2f} jobs/s, {st['rounds']} launches for {st['requests']} evaluations")
+    log("the same stream one job at a time (FP64, one launch per evaluation) ...")
+    base, stb = mg.serve(jobs, batched=False, precision=mg.MG_FP64, realtime=True)
+    lat = np.array([r["latency"] for r in res]) * 1e3
+    latb = np.array([r["latency"] for r in base]) * 1e3
+    # error of each job's final evaluation (FP64 sweeps in the last iterations) and of an FP32
+    # sweep at the same gates, against the FP64 single-request path (pinned to the oracle in tests)
+    err = {"grad_final": 0.0, "hvp_final": 0.0, "grad_fp32_sweep": 0.0, "hvp_fp32_sweep": 0.0, "f_vs_fp64_path": 0.0}
+    for j, r, b in zip(jobs, res, base):
+        t = mg.Target.circuit(j["target_gates"], j["target_wires"], j["program"].L)
+        v = rng.standard_normal(8 * j["program"].P)
+        f0, g0, h0 = mg.riemannian_gradient_hvp(t, j["program"], r["gates"], v, mg.MG_FP64)
+        req = {"target": t, "program": j["program"], "gates": r["gates"], "v": v, "what": mg.EVAL_F_GRAD_HVP}
+        o64, o32 = mg.evaluate_batch([dict(req, precision=mg.MG_FP64)])[0], mg.evaluate_batch([dict(req, precision=mg.MG_FP32)])[0]
+        gn, hn = max(np.max(np.abs(g0)), 1e-300), max(np.max(np.abs(h0)), 1e-300)
+        err["grad_final"] = max(err["grad_final"], float(np.max(np.abs(o64[1] - g0)) / gn))
+        err["hvp_final"] = max(err["hvp_final"], float(np.max(np.abs(o64[2] - h0)) / hn))
+        err["grad_fp32_sweep"] = max(err["grad_fp32_sweep"], float(np.max(np.abs(o32[1] - g0)) / gn))
+        err["hvp_fp32_sweep"] = max(err["hvp_fp32_sweep"], float(np.max(np.abs(o32[2] - h0)) / hn))
+        err["f_vs_fp64_path"] = max(err["f_vs_fp64_path"], float(abs(r["f_iter"][-1] - b["f_iter"][-1]) / abs(b["f_iter"][-1])))
+    jps, jpsb = njobs / st["wall_s"], njobs / stb["wall_s"]
+    return {"jobs": njobs, "jobs_per_s": jps, "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+            "launches": st["rounds"], "evaluations": st["requests"], "peak_admitted": st["max_active"],
+            "baseline_fp64_one_job": {"jobs_per_s": jpsb, "p50_ms": float(np.percentile(latb, 50)),
+                                      "p99_ms": float(np.percentile(latb, 99))},
+            "speedup_jobs_per_s": jps / jpsb, "rel_err_vs_fp64": err, "autotune": tune,
+            "workload": "Poisson 200 jobs/s seed 1, L in {8,10,12}, odd depth 3..15, 5..50 iterations, "
+                        "circuit targets of 3x depth; FP32 sweeps, last 3 iterations FP64"}
+
+
 def time_steps(mg, t, p, V, v, prec, steps, warmup):
     for _ in range(warmup):
         mg.riemannian_gradient_hvp(t, p, V, v, prec)
@@ -104,6 +161,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--rtr-iters", type=int, default=5, help="full trust-region iterations timed for the extra field")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-sweep variant")
+    ap.add_argument("--serve-jobs", type=int, default=32, help="jobs of the served stream (0 skips it)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 5 times (reference kernel for HBM byte counters)")
     args = ap.parse_args()
@@ -162,13 +220,16 @@ def main():
         extra["rtr"] = {"iters": args.rtr_iters, "iters_per_s": args.rtr_iters / rt, "hvp_sweeps": nh,
                         "f_first": float(f_iter[0]), "f_last": float(f_iter[-1])}
 
+    if args.serve_jobs > 0:
+        extra["served"] = served_bench(mg, args.serve_jobs)
+
     cpu = None
     if not args.no_cpu_baseline:
         log("timing the CPU baseline (oracle, bounded sample) ...")
         cpu = cpu_baseline(W, V, v, param, wires, L)
 
     out = {
-        "metric": "trust-region iters/s (f+grad+HVP) at L=12",
+        "metric": "trust-region iters/s (f+grad+HVP) at L=12; served jobs/s + p50/p99 latency",
         "value": value,
         "unit": "f+grad+HVP evaluations/s",
         "n_gpus": 1,

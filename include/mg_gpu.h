@@ -40,6 +40,10 @@ mg_target* mg_target_circuit(const struct matchgate gates[], int ngates, const i
 /* any linear map, materialised once by 2^n calls of ufunc (reentrant, called from several threads) */
 mg_target* mg_target_from_linear_func(linear_func ufunc, void* udata, int nqubits);
 void mg_target_free(mg_target* t);
+/* materialise a circuit or brickwall target once on the GPU as dense rows (2^n x 2^n FP64);
+ * later evaluations read the rows instead of applying the gates per column.  No-op for dense
+ * targets. */
+int mg_target_densify(mg_target* t);
 
 /* ---- gate programs ---- */
 /* ninst instances; instance g applies parameter gate param[g] to (wires[2g], wires[2g+1]) */
@@ -113,6 +117,13 @@ void mg_batch_set_shape(int precision, int nqubits, int threads_per_column, int 
 void mg_batch_get_shape(int precision, int nqubits, int* threads_per_column, int* columns_per_item);
 /* duration of the last mg_evaluate_batch kernels in ms (needs mg_set_kernel_timing(1)) */
 float mg_batch_last_kernel_ms(void);
+/* autotuner: times every compiled (threads, window bits) variant with 1 and 2 workgroups per
+ * CU on the given calibration requests (their precision field is ignored; `precision` is
+ * tuned), reps timed launches each, and keeps the fastest for that precision.  Outputs
+ * (nullable) the choice and its mean kernel time.  Run it before serving: results depend on
+ * the kernel variant, so it is fixed while requests are compared. */
+int mg_batch_autotune(mg_request* reqs, int n, int precision, int reps, int* threads, int* window_bits, int* per_cu,
+                      float* kernel_ms);
 
 /* ---- serving: a stream of trust-region optimisation jobs ----
  * Each job runs riemannian_trust_region_optimize_hvp with set_rtr_default_params(16 nparams)
@@ -144,6 +155,9 @@ typedef struct mg_serve_options {
     int max_active;       /* admitted jobs at most (0: no limit) */
     double hbm_budget;    /* device bytes the admitted jobs may hold (0: no limit) */
     int realtime;         /* 1: admit a job no earlier than its arrival time; 0: all arrive at 0 */
+    int densify_qubits;   /* circuit targets of at most this many qubits are materialised once on the
+                             GPU at admission (2^n x 2^n FP64, plus an FP32 copy) and read as dense
+                             rows afterwards; 0: always applied gate by gate per column */
 } mg_serve_options;
 
 typedef struct mg_serve_stats {

@@ -485,6 +485,15 @@ mg_target* mg_target_from_linear_func(linear_func ufunc, void* udata, int nqubit
 
 void mg_target_free(mg_target* t) { delete t; }
 
+int mg_target_densify(mg_target* t)
+{
+    if (!t || !t->t) {
+        mg::set_error("mg_target_densify: null target");
+        return -1;
+    }
+    return mg::target_densify(*t->t);
+}
+
 mg_program* mg_program_create(int nqubits, int nparams, int ninst, const int* param, const int* wires)
 {
     auto* h = new mg_program;

@@ -438,6 +438,47 @@ __global__ __launch_bounds__(T) void batch_kernel(const BatchInst<R>* __restrict
     }
 }
 
+// seed[b][a] = -conj((U e_b)[a]) for the columns b of a target program, FP64: one
+// column per group of cgw threads, the state in LDS (window form as batch_item)
+template <int T, int W>
+__global__ __launch_bounds__(T) void materialize_kernel(int L, const Inst* tinst, const cpx<double>* tcoef,
+                                                        const WBlock* tblocks, int ntblocks, int cgw,
+                                                        cpx<double>* seed)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const uint32_t S = 1u << L, ntup = S >> W;
+    const int ncc = T / cgw, grp = threadIdx.x / cgw, tl = threadIdx.x - grp * cgw;
+    cpx<double>* st = reinterpret_cast<cpx<double>*>(smem) + (size_t)grp * S;
+    for (uint32_t b0 = blockIdx.x * ncc; b0 < S; b0 += gridDim.x * ncc) {
+        const uint32_t b = b0 + grp;
+        const bool on = b < S;
+        if (on)
+            for (uint32_t i = tl; i < S; i += cgw) st[lin<double, true>(i)] = {double(i == b), 0.0};
+        __syncthreads();
+        win_apply_program<double, W, true>(st, tblocks, ntblocks, tinst, tcoef, ntup, tl, cgw, on);
+        if (on)
+            for (uint32_t i = tl; i < S; i += cgw) {
+                const cpx<double> v = st[lin<double, true>(i)];
+                seed[(size_t)b * S + i] = {-v.re, v.im};
+            }
+        __syncthreads();
+    }
+}
+
+hipError_t launch_materialize(int L, const Inst* tinst, const cpx<double>* tcoef, const WBlock* tblocks, int ntblocks,
+                              cpx<double>* seed, int grid, size_t lds, hipStream_t s)
+{
+    constexpr int T = 256, W = 3;
+    const int cgw = (int)std::min<long>(T, std::max<long>(1, (1L << L) >> W));
+    auto k = materialize_kernel<T, W>;
+    if (lds > 65536) {
+        hipError_t e = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(k, dim3(grid), dim3(T), lds, s, L, tinst, tcoef, tblocks, ntblocks, cgw, seed);
+    return hipGetLastError();
+}
+
 __global__ void batch_reset(int* counter) { *counter = 0; }
 
 // out[seg.dst + k] = sum over items j of partial[seg.src + j * seg.stride + k], in j order

@@ -61,6 +61,11 @@ hipError_t launch_batch(const BatchInst<R>* insts, const BatchItem* items, int n
                         cpx<R>* scratch, long long scratch_stride, int threads, int wbits, int grid, size_t lds,
                         hipStream_t s);
 
+// dense seed rows of a target program (FP64, window bits 3, 256 threads; blocks in LDS
+// swizzled form): lds = (256 / cgw) * 2^L * 16 bytes with cgw = min(256, 2^(L-3))
+hipError_t launch_materialize(int L, const Inst* tinst, const cpx<double>* tcoef, const WBlock* tblocks, int ntblocks,
+                              cpx<double>* seed, int grid, size_t lds, hipStream_t s);
+
 hipError_t launch_batch_reduce(const ReduceSeg* segs, const int2* blocks, int nblocks, const double* partial, double* out,
                                hipStream_t s);
 

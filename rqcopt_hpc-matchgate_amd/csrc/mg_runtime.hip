@@ -575,6 +575,36 @@ static int ensure_windowed_target(Target& t)
     return 0;
 }
 
+int target_densify(Target& t)
+{
+    if (t.kind != 1 || t.L < 3) return 0;
+    Context* c = ctx();
+    if (!c) { set_error("no HIP device"); return -1; }
+    const int L = t.L;
+    const size_t S = (size_t)1 << L;
+    // the FP64 window form (3 bits) of the target program, LDS offsets
+    Windowed w;
+    make_windowed<double>(t.tinst_h, t.tcoef_h, L, 3, &w);
+    DevBuf di, dc, db;
+    MG_HIP(di.ensure(w.inst.size() * sizeof(Inst)));
+    MG_HIP(dc.ensure(w.coef.size() * sizeof(cpx<double>)));
+    MG_HIP(db.ensure(w.blk_lds.size() * sizeof(WBlock)));
+    MG_HIP(hipMemcpy(di.p, w.inst.data(), w.inst.size() * sizeof(Inst), hipMemcpyHostToDevice));
+    MG_HIP(hipMemcpy(dc.p, w.coef.data(), w.coef.size() * sizeof(cpx<double>), hipMemcpyHostToDevice));
+    MG_HIP(hipMemcpy(db.p, w.blk_lds.data(), w.blk_lds.size() * sizeof(WBlock), hipMemcpyHostToDevice));
+    const long cgw = std::min<long>(256, std::max<long>(1, (long)(S >> 3)));
+    const size_t lds = (size_t)(256 / cgw) * S * sizeof(cpx<double>);
+    if (lds > c->lds_limit - 256) { set_error("target too large to materialise"); return -1; }
+    if (t.seed64.ensure(S * S * sizeof(cpx<double>)) != hipSuccess) { set_error("out of device memory for the target"); return -1; }
+    const int grid = (int)std::max<long>(1, std::min<long>((long)(S / (256 / cgw)), (long)c->ncu * 2));
+    MG_HIP(launch_materialize(L, (const Inst*)di.p, (const cpx<double>*)dc.p, (const WBlock*)db.p, (int)w.blk_lds.size(),
+                              (cpx<double>*)t.seed64.p, grid, lds, c->s));
+    MG_HIP(hipStreamSynchronize(c->s));
+    t.kind = 0;
+    t.seed32_ready = false;
+    return 0;
+}
+
 // host staging of one upload: regions appended at 256-byte alignment
 struct Arena {
     std::vector<unsigned char> h;
@@ -786,6 +816,10 @@ int evaluate_batch(BatchReq* reqs, int n, BatchStats* st)
         if (!r.p->validate(&why)) { set_error("batched evaluation: program: " + why); return -1; }
         if (r.t->L != r.p->L) { set_error("batched evaluation: target and program have different qubit counts"); return -1; }
         if (r.p->L > 16) { set_error("batched evaluation: at most 16 qubits"); return -1; }
+        if (r.p->L < batch_tuning().wbits[r.precision == kFP32 ? 1 : 0]) {
+            set_error("batched evaluation: fewer qubits than the gate-window bits");
+            return -1;
+        }
         if (r.mode < 0 || r.mode > 2) { set_error("batched evaluation: invalid mode"); return -1; }
         if (r.mode == 2 && !r.Z) { set_error("batched evaluation: Hessian-vector product needs a direction"); return -1; }
         (r.precision == kFP32 ? i32 : i64).push_back(i);

@@ -63,6 +63,9 @@ Target* target_dense(const zd* U, int L);
 // rows already in seed layout: seed[b*S + a] = -conj(U[a][b])
 Target* target_dense_seed(std::vector<cpx<double>>&& seed, int L);
 Target* target_program(const Program& prog, const zd* gates);
+// turn a target program into dense seed rows on the device (FP64, materialised once on
+// the GPU); later evaluations read the rows instead of applying the program per column
+int target_densify(Target& t);
 
 enum Mode { kF = 0, kFGrad = 1, kFGradHvp = 2 };
 enum Precision { kFP64 = 0, kFP32 = 1 };

@@ -177,13 +177,13 @@ void run_job(JobRt* j)
 
 // device memory a job holds while admitted: its target, its coefficients and its
 // share of the batched partial sums (two precisions)
-size_t job_footprint(const mg_job& s)
+size_t job_footprint(const mg_job& s, int densify_qubits)
 {
     const int L = s.nqubits;
     const size_t S = (size_t)1 << L;
     const mg::Program& p = s.program->p;
     size_t b = 0;
-    if (s.U) b += S * S * (sizeof(double) * 2 + sizeof(float) * 2);
+    if (s.U || L <= densify_qubits) b += S * S * (sizeof(double) * 2 + sizeof(float) * 2);
     else b += (size_t)s.target_ngates * (mg::kCoefPerInst * 24 + sizeof(mg::Inst));
     b += (size_t)p.G * (mg::kCoefPerInst * 16 + sizeof(mg::Inst));
     int cgw, cpi;
@@ -192,7 +192,7 @@ size_t job_footprint(const mg_job& s)
     return b;
 }
 
-mg::Target* make_target(const mg_job& s)
+mg::Target* make_target(const mg_job& s, int densify_qubits)
 {
     if (s.U) return mg::target_dense(reinterpret_cast<const zd*>(s.U), s.nqubits);
     mg::Program tp;
@@ -202,7 +202,9 @@ mg::Target* make_target(const mg_job& s)
     tp.param.resize(tp.G);
     tp.wires.assign(s.target_wires, s.target_wires + 2 * tp.G);
     for (int g = 0; g < tp.G; g++) tp.param[g] = g;
-    return mg::target_program(tp, Zp(s.target_gates));
+    std::unique_ptr<mg::Target> t(mg::target_program(tp, Zp(s.target_gates)));
+    if (t && s.nqubits <= densify_qubits && mg::target_densify(*t) < 0) return nullptr;
+    return t.release();
 }
 
 bool valid_job(const mg_job& s)
@@ -296,6 +298,75 @@ void mg_batch_get_shape(int precision, int nqubits, int* threads_per_column, int
 
 float mg_batch_last_kernel_ms(void) { return mg::last_batch_stats().kernel_ms; }
 
+int mg_batch_autotune(mg_request* reqs, int n, int precision, int reps, int* threads, int* window_bits, int* per_cu,
+                      float* kernel_ms)
+{
+    if (n <= 0 || !reqs || reps < 1) {
+        mg::set_error("mg_batch_autotune: invalid argument");
+        return -1;
+    }
+    const int pi = precision == MG_FP32 ? 1 : 0;
+    std::vector<mg::BatchReq> br(n);
+    std::vector<mg::EvalOut> outs(n);
+    std::vector<std::vector<struct matchgate>> Z(n);
+    for (int i = 0; i < n; i++) {
+        const mg_request& r = reqs[i];
+        if (!r.target || !r.program || !r.gates || (r.what == MG_EVAL_F_GRAD_HVP && !r.v)) {
+            mg::set_error("mg_batch_autotune: invalid request " + std::to_string(i));
+            return -1;
+        }
+        if (r.what == MG_EVAL_F_GRAD_HVP) {
+            Z[i].resize(r.program->p.P);
+            for (int k = 0; k < r.program->p.P; k++) real_to_tangent_matchgate(r.v + 8 * k, &r.gates[k], &Z[i][k]);
+        }
+        br[i] = mg::BatchReq{r.target->t.get(), &r.program->p, Zp(r.gates), Z[i].empty() ? nullptr : Zp(Z[i].data()),
+                             r.what, pi ? mg::kFP32 : mg::kFP64, &outs[i]};
+    }
+    mg::BatchTuning& bt = mg::batch_tuning();
+    const mg::BatchTuning saved = bt;
+    const bool timing = mg::knobs().time_kernels;
+    mg::knobs().time_kernels = true;
+    float best = 1e30f;
+    int bT = saved.threads[pi], bW = saved.wbits[pi], bP = saved.grid_per_cu;
+    for (int T : {256, 512})
+        for (int W = 2; W <= 4; W++) {
+            if (!mg::batch_variant_exists(pi, T, W)) continue;
+            for (int pc : {1, 2}) {
+                bt.threads[pi] = T;
+                bt.wbits[pi] = W;
+                bt.grid_per_cu = pc;
+                mg::BatchStats st;
+                if (mg::evaluate_batch(br.data(), n, &st) < 0) continue;  // warm-up (and shape check)
+                float ms = 0.f;
+                bool ok = true;
+                for (int k = 0; k < reps && ok; k++) {
+                    ok = mg::evaluate_batch(br.data(), n, &st) == 0;
+                    ms += st.kernel_ms;
+                }
+                if (ok && ms < best) {
+                    best = ms;
+                    bT = T;
+                    bW = W;
+                    bP = pc;
+                }
+            }
+        }
+    mg::knobs().time_kernels = timing;
+    bt = saved;
+    if (best >= 1e30f) {
+        mg::set_error("mg_batch_autotune: no variant ran");
+        return -1;
+    }
+    bt.threads[pi] = bT;
+    bt.wbits[pi] = bW;
+    bt.grid_per_cu = bP;
+    if (threads) *threads = bT;
+    if (window_bits) *window_bits = bW;
+    if (per_cu) *per_cu = bP;
+    if (kernel_ms) *kernel_ms = best / reps;
+    return 0;
+}
+
 int mg_serve(mg_job* jobs, int njobs, const mg_serve_options* opt, mg_serve_stats* stats)
 {
     if (njobs < 0 || (njobs > 0 && !jobs) || !opt) {
@@ -328,9 +399,9 @@ int mg_serve(mg_job* jobs, int njobs, const mg_serve_options* opt, mg_serve_stat
         j->precision = opt->precision == MG_FP32 ? MG_FP32 : MG_FP64;
         j->switch_last = std::max(0, opt->precision_switch);
         j->niter = jobs[i].niter;
-        j->footprint = job_footprint(jobs[i]);
+        j->footprint = job_footprint(jobs[i], opt->densify_qubits);
         jobs[i].t_admit = now_s();
-        j->target.reset(make_target(jobs[i]));
+        j->target.reset(make_target(jobs[i], opt->densify_qubits));
         if (!j->target) {
             jobs[i].status = -1;
             return false;
@@ -359,7 +430,7 @@ int mg_serve(mg_job* jobs, int njobs, const mg_serve_options* opt, mg_serve_stat
             // admit arrived jobs while slots and device memory allow
             while (next < njobs && arrival(next) <= now_s()) {
                 const int i = order[next];
-                const size_t fp = job_footprint(jobs[i]);
+                const size_t fp = job_footprint(jobs[i], opt->densify_qubits);
                 const bool slot = opt->max_active <= 0 || srv.running < opt->max_active;
                 const bool mem = opt->hbm_budget <= 0 || srv.running == 0 || (double)(held + fp) <= opt->hbm_budget;
                 if (!slot || !mem) break;

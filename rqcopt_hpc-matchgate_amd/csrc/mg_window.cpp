@@ -26,6 +26,7 @@ void window_blocks(const std::vector<Inst>& inst, int L, int W, int kmax, std::v
     const int G = (int)inst.size();
     order->clear();
     blocks->clear();
+    if (W > L || W > kWinMaxBits || W < 2) return;  // callers keep 2 <= W <= min(L, kWinMaxBits)
     // dependencies: the previous and next gate on each of a gate's two qubits
     std::vector<int> npred(G, 0);
     std::vector<int> succ(2 * G, -1);

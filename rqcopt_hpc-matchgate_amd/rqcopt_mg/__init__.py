@@ -71,7 +71,8 @@ class Job(ctypes.Structure):
 class ServeOptions(ctypes.Structure):
     """struct mg_serve_options (include/mg_gpu.h)"""
     _fields_ = [("batched", ctypes.c_int), ("precision", ctypes.c_int), ("precision_switch", ctypes.c_int),
-                ("max_active", ctypes.c_int), ("hbm_budget", _D), ("realtime", ctypes.c_int)]
+                ("max_active", ctypes.c_int), ("hbm_budget", _D), ("realtime", ctypes.c_int),
+                ("densify_qubits", ctypes.c_int)]
 
 
 class ServeStats(ctypes.Structure):
@@ -98,6 +99,7 @@ _sig("mg_target_brickwall", [_P, _I, _P, _I], _P)
 _sig("mg_target_circuit", [_P, _I, _P, _I], _P)
 _sig("mg_target_from_linear_func", [LINEAR_FUNC, _P, _I], _P)
 _sig("mg_target_free", [_P], None)
+_sig("mg_target_densify", [_P])
 _sig("mg_program_create", [_I, _I, _I, _P, _P], _P)
 _sig("mg_program_brickwall", [_I, _I, _P], _P)
 _sig("mg_program_free", [_P], None)
@@ -126,6 +128,7 @@ _sig("mg_batch_set_grid_per_cu", [_I], None)
 _sig("mg_batch_set_shape", [_I, _I, _I, _I], None)
 _sig("mg_batch_get_shape", [_I, _I, _P, _P], None)
 _sig("mg_batch_last_kernel_ms", [], ctypes.c_float)
+_sig("mg_batch_autotune", [ctypes.POINTER(Request), _I, _I, _I, _P, _P, _P, _P])
 _sig("mg_serve", [ctypes.POINTER(Job), _I, ctypes.POINTER(ServeOptions), ctypes.POINTER(ServeStats)])
 # reference entry points (include/matchgate_target.h, mg_brickwall_opt.h)
 _sig("matchgate_circuit_unitary_target", [LINEAR_FUNC, _P, _P, _I, _P, _I, _P])
@@ -222,6 +225,11 @@ class Target:
         ulist = _c(ulist)
         perms, arr = _perm_ptrs(uperms)
         return cls(lib.mg_target_brickwall(_p(ulist), perms.shape[0], ctypes.cast(arr, _P), L))
+
+    def densify(self):
+        """Materialise a circuit/brickwall target as dense rows on the GPU (mg_target_densify)."""
+        _check(lib.mg_target_densify(self.h), "mg_target_densify")
+        return self
 
     @classmethod
     def circuit(cls, gates, wires, L):
@@ -327,13 +335,7 @@ def optimize(target, program, gates, niter, hvp=True, precision=MG_FP64, precisi
 
 # ---------------- batched evaluation and serving ----------------
 
-def evaluate_batch(requests):
-    """Evaluate many requests in shared launches.
-
-    requests: list of dicts with keys target, program, gates, v (or None), what,
-    precision.  Returns a list of (f, grad_vec, hvp) like riemannian_gradient_hvp
-    (grad_vec None for what == EVAL_F, hvp None unless what == EVAL_F_GRAD_HVP).
-    """
+def _requests(requests):
     n = len(requests)
     arr = (Request * n)()
     keep, outs = [], []
@@ -350,7 +352,29 @@ def evaluate_batch(requests):
         arr[k] = Request(r["target"].h, prog.h, gates.ctypes.data, vv.ctypes.data if vv is not None else None, what,
                          int(r.get("precision", MG_FP64)), f.ctypes.data, g.ctypes.data, hv.ctypes.data)
         outs.append((f, g, hv, what))
-    _check(lib.mg_evaluate_batch(arr, n), "mg_evaluate_batch")
+    return arr, keep, outs
+
+
+def batch_autotune(requests, precision, reps=2):
+    """Pick the fastest batched-kernel variant for `precision` on calibration requests
+    (mg_batch_autotune).  Returns (threads, window_bits, workgroups_per_cu, kernel_ms)."""
+    arr, keep, _ = _requests(requests)
+    t, w, p = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    ms = ctypes.c_float()
+    _check(lib.mg_batch_autotune(arr, len(requests), int(precision), int(reps), ctypes.byref(t), ctypes.byref(w),
+                                 ctypes.byref(p), ctypes.byref(ms)), "mg_batch_autotune")
+    return t.value, w.value, p.value, ms.value
+
+
+def evaluate_batch(requests):
+    """Evaluate many requests in shared launches.
+
+    requests: list of dicts with keys target, program, gates, v (or None), what,
+    precision.  Returns a list of (f, grad_vec, hvp) like riemannian_gradient_hvp
+    (grad_vec None for what == EVAL_F, hvp None unless what == EVAL_F_GRAD_HVP).
+    """
+    arr, keep, outs = _requests(requests)
+    _check(lib.mg_evaluate_batch(arr, len(requests)), "mg_evaluate_batch")
     return [(f[0], g if what >= EVAL_F_GRAD else None, hv if what == EVAL_F_GRAD_HVP else None)
             for f, g, hv, what in outs]
 
@@ -388,7 +412,7 @@ def batch_last_kernel_ms():
 
 
 def serve(jobs, batched=True, precision=MG_FP32, precision_switch=3, max_active=0, hbm_budget=0.0,
-          realtime=True):
+          realtime=True, densify_qubits=0):
     """Run a stream of trust-region optimisation jobs (mg_serve).
 
     jobs: list of dicts with keys program (Program), gates (P, 8), niter, arrival (s),
@@ -415,7 +439,7 @@ def serve(jobs, batched=True, precision=MG_FP32, precision_switch=3, max_active=
                      f_iter.ctypes.data, gout.ctypes.data, 0.0, 0.0, 0, 0)
         outs.append((f_iter, gout))
     opt = ServeOptions(int(bool(batched)), int(precision), int(precision_switch), int(max_active), float(hbm_budget),
-                       int(bool(realtime)))
+                       int(bool(realtime)), int(densify_qubits))
     st = ServeStats()
     _check(lib.mg_serve(arr, n, ctypes.byref(opt), ctypes.byref(st)), "mg_serve")
     res = []

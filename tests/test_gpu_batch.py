@@ -134,6 +134,22 @@ def test_batch_global_states_and_shapes(mg):
         mg.batch_set_shape(mg.MG_FP64, 8, 0, 0)
 
 
+def test_densified_target_matches_program(mg):
+    rng = np.random.default_rng(14)
+    job, prog, tgt, v = make_instance(mg, rng, 10, 5)
+    say("case: densified circuit target, L=10, single path and batched")
+    ref = mg.riemannian_gradient_hvp(tgt, prog, job["gates"], v, mg.MG_FP64)
+    dense = mg.Target.circuit(job["target_gates"], job["target_wires"], 10).densify()
+    for out in (mg.riemannian_gradient_hvp(dense, prog, job["gates"], v, mg.MG_FP64),
+                mg.evaluate_batch([{"target": dense, "program": prog, "gates": job["gates"], "v": v, "what": 2,
+                                    "precision": mg.MG_FP64}])[0]):
+        assert abs(out[0] - ref[0]) <= 1e-12 * abs(ref[0])
+        assert rel(out[1], ref[1]) < 1e-12 and rel(out[2], ref[2]) < 1e-12
+    W = program_matrix(job["target_gates"], np.arange(job["Gt"]), job["target_wires"], 10)
+    fo, go, ho = oracle_riemannian(job, v, 10)
+    assert abs(ref[0] - fo) <= 1e-11 * abs(fo) and rel(ref[1], go) < 1e-11 and rel(ref[2], ho) < 1e-11
+
+
 def _jobs(mg, seed, spec):
     from rqcopt_mg.workloads import brickwall_job
 
@@ -175,6 +191,10 @@ def test_serve_batched_equals_alone_and_single_path(mg):
     # the FP32-sweep trajectories end on FP64 iterations close to the FP64 ones
     for a, b in zip(res, b64):
         assert abs(a["f_iter"][-1] - b["f_iter"][-1]) < 1e-3 * abs(b["f_iter"][-1])
+    say("case: serve FP64 batched with targets materialised at admission")
+    d64, _ = mg.serve(jobs, batched=True, precision=mg.MG_FP64, realtime=False, densify_qubits=10)
+    for a, b in zip(d64, b64):
+        assert np.max(np.abs(a["f_iter"] - b["f_iter"])) < 1e-9 and np.max(np.abs(a["gates"] - b["gates"])) < 1e-9
 
 
 def test_serve_admission_limits(mg):
