@@ -194,7 +194,7 @@ def main():
     achieved = flops / (kernel_ms * 1e-3) / 1e12
     roofline = {"bound": "mfma", "achieved": achieved, "peak": PEAK_TFLOPS[dtype], "unit": "TFLOP/s",
                 "frac": achieved / PEAK_TFLOPS[dtype], "traffic": None,
-                "kernel": "sweep_reg_kernel" if info["reg_threads"] else "sweep_kernel",
+                "kernel": info["kernel"],
                 "kernel_ms": kernel_ms, "flops_per_launch": flops,
                 "note": "VALU FP64/FP32 FMA bound; MI355X FP64 vector peak equals its FP64 matrix peak"}
 

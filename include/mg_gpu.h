@@ -186,6 +186,12 @@ void mg_set_sweep_threads(int threads);
 void mg_set_grid_per_cu(int n);
 /* threads per workgroup of the last register-resident sweep, 0 if the last sweep used the LDS/global kernel */
 int mg_last_sweep_threads(void);
+/* name of the last sweep's kernel: "sweep_win_kernel" (register-resident, gate windows),
+ * "sweep_reg_kernel" (register-resident, one re-layout per gate) or "sweep_kernel" (LDS/global states) */
+const char* mg_last_sweep_kernel(void);
+/* register-resident gate-window kernel (FP64 at 12 qubits, dense targets, unitary gates):
+ * -1 automatic (used where it applies), 0 never */
+void mg_set_window_kernel(int mode);
 /* record HIP events around every sweep kernel on the library's stream */
 void mg_set_kernel_timing(int on);
 /* duration of the last sweep kernel in ms (needs mg_set_kernel_timing(1)) */

@@ -10,6 +10,7 @@
 
 #include "mg_batch.hpp"
 #include "mg_sweep.hpp"
+#include "mg_sweep_win.hpp"
 
 namespace mg {
 
@@ -59,6 +60,7 @@ struct Context {
     int ncu = 0;
     size_t lds_limit = 0;
     DevBuf inst, coef, partial, out, scratch, ckpt;
+    DevBuf w_inst, w_coef, w_blk;  // gate-window form of the program (register window kernel)
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::vector<double> out_h;
     // batched evaluation: packed descriptor upload, partials, outputs, scratch, counter
@@ -295,6 +297,46 @@ static int ensure_fp32_target(Target& t, hipStream_t s)
     return 0;
 }
 
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// window blocks of an instance list, reordered instances (lo = window-local pair
+// index) and coefficients, block offsets for LDS (swizzled) and global states
+struct Windowed {
+    std::vector<Inst> inst;
+    std::vector<cpx<double>> coef;
+    std::vector<WBlock> blk_lds, blk_glob;
+};
+
+template <typename R>
+static void make_windowed(const std::vector<Inst>& inst, const std::vector<cpx<double>>& coef, int L, int W,
+                          Windowed* w, int kmax = kWinMaxGates)
+{
+    std::vector<int> order;
+    std::vector<WBlock> blocks;
+    window_blocks(inst, L, W, kmax, &order, &blocks);
+    const size_t G = inst.size();
+    w->inst.resize(G);
+    w->coef.resize(G * kCoefPerInst);
+    for (const WBlock& bk : blocks)
+        for (int k = bk.g0; k < bk.g1; k++) {
+            Inst in = inst[order[k]];
+            in.lo = window_case(bk, W, in.lo, in.hi);
+            in.hi = 0;
+            w->inst[k] = in;
+            std::copy(coef.begin() + (size_t)order[k] * kCoefPerInst, coef.begin() + (size_t)(order[k] + 1) * kCoefPerInst,
+                      w->coef.begin() + (size_t)k * kCoefPerInst);
+        }
+    w->blk_lds = blocks;
+    w->blk_glob = blocks;
+    for (size_t k = 0; k < blocks.size(); k++)
+        for (int j = 0; j < (1 << W); j++) {
+            uint32_t o = 0;
+            for (int i = 0; i < W; i++) o |= ((uint32_t)(j >> i) & 1u) << blocks[k].pos[i];
+            w->blk_glob[k].off[j] = o;
+            w->blk_lds[k].off[j] = win_swz<batch_bank_bits<R>()>(o);
+        }
+}
+
 // ---- evaluation -------------------------------------------------------------------------------
 template <typename R>
 static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, int mode, long b0, long b1, EvalOut* out,
@@ -353,6 +395,15 @@ static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, in
         }
         if (reg_T && head + S * sizeof(cpx<R>) > c->lds_limit) reg_T = 0;
     }
+    // register-resident gate-window kernel (FP64, 512 threads x 8 amplitudes, dense targets)
+    size_t win_acc = 0, win_buf = 0, win_lds = 0;
+    bool use_win = false;
+    if (std::is_same<R, double>::value && !ckpt && b.defect <= 1e-12 && t.kind == 0 && win_sweep_fits(L) &&
+        kn.force_win != 0 && kn.force_reg != 0 && kn.force_lds != 0) {
+        win_lds = win_sweep_lds_bytes(L, p.P, &win_acc, &win_buf);
+        use_win = win_lds <= c->lds_limit;
+    }
+    if (use_win) reg_T = 0;
     const int nw_reg = reg_T / 64;
     const size_t head_reg = reg_T ? sweep_lds_head_bytes(p.P, nw_reg) : 0;
     int reg_nst = 0;
@@ -361,15 +412,16 @@ static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, in
         if (reg_nst == 3) reg_nst = 2;  // instantiated: 1, 2, 4 states per round
         if (reg_nst < 1) reg_T = 0;
     }
-    const size_t lds = reg_T ? head_reg + (size_t)reg_nst * S * sizeof(cpx<R>) : (lds_states ? head + state_bytes : head);
+    const size_t lds = use_win ? win_lds
+                     : reg_T ? head_reg + (size_t)reg_nst * S * sizeof(cpx<R>) : (lds_states ? head + state_bytes : head);
 
     const long ncols = b1 - b0;
     int per_cu = kn.grid_per_cu > 0 ? kn.grid_per_cu : std::max<int>(1, std::min<size_t>(4, (160 * 1024) / std::max<size_t>(lds, 1)));
-    if (reg_T && kn.grid_per_cu <= 0) per_cu = std::max<int>(1, std::min<size_t>(2, (160 * 1024) / lds));
+    if ((reg_T || use_win) && kn.grid_per_cu <= 0) per_cu = std::max<int>(1, std::min<size_t>(2, (160 * 1024) / lds));
     long grid = std::min<long>(ncols, (long)c->ncu * per_cu);
     // bound the global scratch and checkpoint footprint (2 GiB each)
     const size_t lim = (size_t)2 << 30;
-    if (!lds_states && !reg_T) grid = std::min<long>(grid, std::max<long>(1, lim / state_bytes));
+    if (!lds_states && !reg_T && !use_win) grid = std::min<long>(grid, std::max<long>(1, lim / state_bytes));
     const size_t ck_per = ckpt ? (size_t)2 * (p.G + 1) * S * sizeof(cpx<R>) : 0;
     if (ckpt) grid = std::min<long>(grid, std::max<long>(1, lim / ck_per));
     grid = std::max<long>(grid, 1);
@@ -377,7 +429,7 @@ static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, in
     const int stride = 2 + 32 * p.P;
     MG_HIP(c->partial.ensure((size_t)grid * stride * sizeof(double)));
     MG_HIP(c->out.ensure((size_t)stride * sizeof(double)));
-    if (!lds_states && !reg_T) MG_HIP(c->scratch.ensure((size_t)grid * state_bytes));
+    if (!lds_states && !reg_T && !use_win) MG_HIP(c->scratch.ensure((size_t)grid * state_bytes));
     if (ckpt) MG_HIP(c->ckpt.ensure((size_t)grid * ck_per));
 
     SweepArgs<R> a{};
@@ -402,8 +454,39 @@ static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, in
     a.nstate = reg_T ? reg_nst : nstate;
     a.lds_state_offset = (int)(reg_T ? head_reg : head);
 
+    WinSweepArgs wa{};
+    if (use_win) {
+        Windowed w;
+        make_windowed<double>(b.inst, b.coef, L, 3, &w, kWinRegGates);
+        MG_HIP(c->w_inst.ensure(w.inst.size() * sizeof(Inst)));
+        MG_HIP(c->w_coef.ensure(w.coef.size() * sizeof(cpx<double>)));
+        MG_HIP(c->w_blk.ensure(2 * w.blk_lds.size() * sizeof(WBlock)));
+        MG_HIP(hipMemcpyAsync(c->w_inst.p, w.inst.data(), w.inst.size() * sizeof(Inst), hipMemcpyHostToDevice, s));
+        MG_HIP(hipMemcpyAsync(c->w_coef.p, w.coef.data(), w.coef.size() * sizeof(cpx<double>), hipMemcpyHostToDevice, s));
+        MG_HIP(hipMemcpyAsync(c->w_blk.p, w.blk_lds.data(), w.blk_lds.size() * sizeof(WBlock), hipMemcpyHostToDevice, s));
+        MG_HIP(hipMemcpyAsync((WBlock*)c->w_blk.p + w.blk_lds.size(), w.blk_glob.data(), w.blk_glob.size() * sizeof(WBlock),
+                              hipMemcpyHostToDevice, s));
+        wa.L = L;
+        wa.G = p.G;
+        wa.P = p.P;
+        wa.nblocks = (int)w.blk_lds.size();
+        wa.inst = (const Inst*)c->w_inst.p;
+        wa.coef = (const cpx<double>*)c->w_coef.p;
+        wa.blk_lds = (const WBlock*)c->w_blk.p;
+        wa.blk_glob = (const WBlock*)c->w_blk.p + w.blk_lds.size();
+        wa.seed = (const cpx<double>*)t.seed64.p;
+        wa.b0 = b0;
+        wa.b1 = b1;
+        wa.partial = (double*)c->partial.p;
+        wa.acc_off = (int)win_acc;
+        wa.buf_off = (int)win_buf;
+        // the copies above read pageable host vectors: complete them before w goes away
+        MG_HIP(hipStreamSynchronize(s));
+    }
     if (kn.time_kernels) MG_HIP(hipEventRecord(c->ev0, s));
-    if (reg_T)
+    if (use_win)
+        MG_HIP(launch_sweep_win(wa, mode, (int)grid, lds, s));
+    else if (reg_T)
         MG_HIP(launch_sweep_reg<R>(a, mode, reg_T, reg_qpt, (int)grid, lds, s));
     else
         MG_HIP(launch_sweep<R>(a, mode, lds_states, ckpt, lds_T, (int)grid, lds, s));
@@ -423,10 +506,11 @@ static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, in
             out->dhz[8 * q + e] = zd(o[2 + 32 * q + 16 + 2 * e], o[2 + 32 * q + 16 + 2 * e + 1]);
         }
     if (st) {
-        st->lds_states = lds_states && !reg_T;
+        st->lds_states = lds_states && !reg_T && !use_win;
         st->ckpt = ckpt;
-        st->reg_threads = reg_T;
-        st->threads = reg_T ? reg_T : lds_T;
+        st->reg_threads = use_win ? 512 : reg_T;
+        st->threads = use_win ? 512 : reg_T ? reg_T : lds_T;
+        st->kernel = use_win ? "sweep_win_kernel" : reg_T ? "sweep_reg_kernel" : "sweep_kernel";
         st->grid = (int)grid;
         st->lds_bytes = lds;
         st->kernel_ms = 0.f;
@@ -493,8 +577,6 @@ static int pinned_ensure(void** p, size_t* n, size_t bytes)
     return 0;
 }
 
-static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
-
 void batch_shape(int L, int precision, int* cgw, int* cpi)
 {
     const BatchTuning& bt = batch_tuning();
@@ -510,44 +592,6 @@ void batch_shape(int L, int precision, int* cgw, int* cpi)
     c = (c + ncc - 1) / ncc * ncc;
     *cgw = (int)w;
     *cpi = (int)std::min<long>(c, S);
-}
-
-// window blocks of an instance list, reordered instances (lo = window-local pair
-// index) and coefficients, block offsets for LDS (swizzled) and global states
-struct Windowed {
-    std::vector<Inst> inst;
-    std::vector<cpx<double>> coef;
-    std::vector<WBlock> blk_lds, blk_glob;
-};
-
-template <typename R>
-static void make_windowed(const std::vector<Inst>& inst, const std::vector<cpx<double>>& coef, int L, int W,
-                          Windowed* w)
-{
-    std::vector<int> order;
-    std::vector<WBlock> blocks;
-    window_blocks(inst, L, W, kWinMaxGates, &order, &blocks);
-    const size_t G = inst.size();
-    w->inst.resize(G);
-    w->coef.resize(G * kCoefPerInst);
-    for (const WBlock& bk : blocks)
-        for (int k = bk.g0; k < bk.g1; k++) {
-            Inst in = inst[order[k]];
-            in.lo = window_case(bk, W, in.lo, in.hi);
-            in.hi = 0;
-            w->inst[k] = in;
-            std::copy(coef.begin() + (size_t)order[k] * kCoefPerInst, coef.begin() + (size_t)(order[k] + 1) * kCoefPerInst,
-                      w->coef.begin() + (size_t)k * kCoefPerInst);
-        }
-    w->blk_lds = blocks;
-    w->blk_glob = blocks;
-    for (size_t k = 0; k < blocks.size(); k++)
-        for (int j = 0; j < (1 << W); j++) {
-            uint32_t o = 0;
-            for (int i = 0; i < W; i++) o |= ((uint32_t)(j >> i) & 1u) << blocks[k].pos[i];
-            w->blk_glob[k].off[j] = o;
-            w->blk_lds[k].off[j] = win_swz<batch_bank_bits<R>()>(o);
-        }
 }
 
 // device copy of a target program in window form (per precision, built on first use)

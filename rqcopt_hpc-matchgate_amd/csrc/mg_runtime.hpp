@@ -83,6 +83,7 @@ struct EvalStats {
     int grid = 0;
     size_t lds_bytes = 0;
     float kernel_ms = 0.f;  // sweep kernel duration (HIP events on the launch stream)
+    const char* kernel = "";  // name of the sweep kernel
 };
 
 // gates, Z: P*8 complex each (Z may be null when mode < 2).  Column range [b0, b1).
@@ -97,6 +98,7 @@ struct Knobs {
     int force_reg = -1;       // -1 auto, 0 never, 1 whenever a register-resident configuration exists
     int reg_threads = 0;      // 0 auto, else 512 or 1024 threads for the register-resident kernel
     int lds_threads = 0;      // 0 auto, else 256, 512 or 1024 threads for the LDS/global kernel
+    int force_win = -1;       // -1 auto (register gate-window kernel where it applies), 0 never
     bool time_kernels = false;
 };
 Knobs& knobs();

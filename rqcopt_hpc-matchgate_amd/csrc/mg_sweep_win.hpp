@@ -1,0 +1,28 @@
+// Launch interface of the register-resident gate-window sweep (mg_sweep_win.hip).
+#pragma once
+#include "mg_device.hpp"
+#include "mg_window.hpp"
+
+namespace mg {
+
+constexpr int kWinRegGates = 4;  // gates per window block at most (staging slots)
+
+struct WinSweepArgs {
+    int L, G, P, nblocks;
+    const Inst* inst;          // [G] in block order; lo = window-local pair index
+    const cpx<double>* coef;   // [G][kCoefPerInst] in block order
+    const WBlock* blk_lds;     // [nblocks] offsets swizzled for the LDS re-layout buffer
+    const WBlock* blk_glob;    // [nblocks] plain offsets (seed rows, basis vectors)
+    const cpx<double>* seed;   // [S][S]: row b = -conj(U e_b)
+    long b0, b1;               // column range
+    double* partial;           // [gridDim.x][2 + 32 P]
+    int acc_off, buf_off;      // byte offsets in dynamic LDS
+};
+
+// dynamic LDS of the kernel (and the accumulator and buffer offsets)
+size_t win_sweep_lds_bytes(int L, int P, size_t* acc_off, size_t* buf_off);
+// 512 threads x 8 amplitudes cover the column
+bool win_sweep_fits(int L);
+hipError_t launch_sweep_win(const WinSweepArgs& a, int mode, int grid, size_t lds, hipStream_t s);
+
+}  // namespace mg

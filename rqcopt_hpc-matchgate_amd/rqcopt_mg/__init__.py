@@ -118,6 +118,8 @@ _sig("mg_set_register_kernel", [_I, _I], None)
 _sig("mg_set_grid_per_cu", [_I], None)
 _sig("mg_set_sweep_threads", [_I], None)
 _sig("mg_last_sweep_threads", [])
+_sig("mg_last_sweep_kernel", [], ctypes.c_char_p)
+_sig("mg_set_window_kernel", [_I], None)
 _sig("mg_last_kernel_ms", [], ctypes.c_float)
 _sig("mg_last_sweep_info", [_P, _P, _P, _P], None)
 _sig("mg_target_cache_clear", [], None)
@@ -464,6 +466,14 @@ def set_register_kernel(mode, threads=0):
     lib.mg_set_register_kernel(int(mode), int(threads))
 
 
+def set_window_kernel(mode):
+    lib.mg_set_window_kernel(int(mode))
+
+
+def last_sweep_kernel():
+    return lib.mg_last_sweep_kernel().decode()
+
+
 def set_sweep_threads(n):
     lib.mg_set_sweep_threads(int(n))
 
@@ -485,7 +495,7 @@ def last_sweep_info():
     d = ctypes.c_long()
     lib.mg_last_sweep_info(ctypes.byref(a), ctypes.byref(b), ctypes.byref(c), ctypes.byref(d))
     return {"lds_states": bool(a.value), "ckpt": bool(b.value), "grid": c.value, "lds_bytes": d.value,
-            "reg_threads": int(lib.mg_last_sweep_threads())}
+            "reg_threads": int(lib.mg_last_sweep_threads()), "kernel": last_sweep_kernel()}
 
 
 # ---------------- reference-API helpers ----------------
