@@ -189,8 +189,8 @@ int mg_last_sweep_threads(void);
 /* name of the last sweep's kernel: "sweep_win_kernel" (register-resident, gate windows),
  * "sweep_reg_kernel" (register-resident, one re-layout per gate) or "sweep_kernel" (LDS/global states) */
 const char* mg_last_sweep_kernel(void);
-/* register-resident gate-window kernel (FP64 at 12 qubits, dense targets, unitary gates):
- * -1 automatic (used where it applies), 0 never */
+/* register-resident gate-window kernel (12 qubits, dense targets, unitary gates): -1 automatic
+ * (used for FP64 sweeps where it applies), 0 never, 1 also for FP32 sweeps */
 void mg_set_window_kernel(int mode);
 /* record HIP events around every sweep kernel on the library's stream */
 void mg_set_kernel_timing(int on);

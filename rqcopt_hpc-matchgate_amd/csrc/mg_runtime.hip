@@ -398,9 +398,10 @@ static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, in
     // register-resident gate-window kernel (FP64, 512 threads x 8 amplitudes, dense targets)
     size_t win_acc = 0, win_buf = 0, win_lds = 0;
     bool use_win = false;
-    if (std::is_same<R, double>::value && !ckpt && b.defect <= 1e-12 && t.kind == 0 && win_sweep_fits(L) &&
-        kn.force_win != 0 && kn.force_reg != 0 && kn.force_lds != 0) {
-        win_lds = win_sweep_lds_bytes(L, p.P, &win_acc, &win_buf);
+    constexpr int fp32 = std::is_same<R, float>::value ? 1 : 0;
+    if (!ckpt && b.defect <= 1e-12 && t.kind == 0 && win_sweep_fits(L) && kn.force_win != 0 && kn.force_reg != 0 &&
+        kn.force_lds != 0 && (!fp32 || kn.force_win == 1)) {
+        win_lds = win_sweep_lds_bytes(L, p.P, fp32, &win_acc, &win_buf);
         use_win = win_lds <= c->lds_limit;
     }
     if (use_win) reg_T = 0;
@@ -457,12 +458,14 @@ static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, in
     WinSweepArgs wa{};
     if (use_win) {
         Windowed w;
-        make_windowed<double>(b.inst, b.coef, L, 3, &w, kWinRegGates);
+        make_windowed<R>(b.inst, b.coef, L, 3, &w, kWinRegGates);
+        std::vector<cpx<R>> wc(w.coef.size());
+        for (size_t k = 0; k < wc.size(); k++) wc[k] = {(R)w.coef[k].re, (R)w.coef[k].im};
         MG_HIP(c->w_inst.ensure(w.inst.size() * sizeof(Inst)));
-        MG_HIP(c->w_coef.ensure(w.coef.size() * sizeof(cpx<double>)));
+        MG_HIP(c->w_coef.ensure(wc.size() * sizeof(cpx<R>)));
         MG_HIP(c->w_blk.ensure(2 * w.blk_lds.size() * sizeof(WBlock)));
         MG_HIP(hipMemcpyAsync(c->w_inst.p, w.inst.data(), w.inst.size() * sizeof(Inst), hipMemcpyHostToDevice, s));
-        MG_HIP(hipMemcpyAsync(c->w_coef.p, w.coef.data(), w.coef.size() * sizeof(cpx<double>), hipMemcpyHostToDevice, s));
+        MG_HIP(hipMemcpyAsync(c->w_coef.p, wc.data(), wc.size() * sizeof(cpx<R>), hipMemcpyHostToDevice, s));
         MG_HIP(hipMemcpyAsync(c->w_blk.p, w.blk_lds.data(), w.blk_lds.size() * sizeof(WBlock), hipMemcpyHostToDevice, s));
         MG_HIP(hipMemcpyAsync((WBlock*)c->w_blk.p + w.blk_lds.size(), w.blk_glob.data(), w.blk_glob.size() * sizeof(WBlock),
                               hipMemcpyHostToDevice, s));
@@ -471,10 +474,10 @@ static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, in
         wa.P = p.P;
         wa.nblocks = (int)w.blk_lds.size();
         wa.inst = (const Inst*)c->w_inst.p;
-        wa.coef = (const cpx<double>*)c->w_coef.p;
+        wa.coef = c->w_coef.p;
         wa.blk_lds = (const WBlock*)c->w_blk.p;
         wa.blk_glob = (const WBlock*)c->w_blk.p + w.blk_lds.size();
-        wa.seed = (const cpx<double>*)t.seed64.p;
+        wa.seed = fp32 ? t.seed32.p : t.seed64.p;
         wa.b0 = b0;
         wa.b1 = b1;
         wa.partial = (double*)c->partial.p;
@@ -485,7 +488,7 @@ static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, in
     }
     if (kn.time_kernels) MG_HIP(hipEventRecord(c->ev0, s));
     if (use_win)
-        MG_HIP(launch_sweep_win(wa, mode, (int)grid, lds, s));
+        MG_HIP(launch_sweep_win(wa, mode, fp32, (int)grid, lds, s));
     else if (reg_T)
         MG_HIP(launch_sweep_reg<R>(a, mode, reg_T, reg_qpt, (int)grid, lds, s));
     else

@@ -1,12 +1,12 @@
-// Register-resident column sweep with gate windows: FP64 at L=12, the states of
-// which (4 x 64 KB) do not fit the LDS.
+// Register-resident column sweep with gate windows at L=12 (FP64, whose states,
+// 4 x 64 KB, do not fit the LDS, and FP32).
 //
 // The program's gates are regrouped into blocks acting inside 3-qubit windows
 // (mg_window.cpp).  Each of the 512 threads keeps, for every live state, the 8
 // amplitudes of one window tuple in VGPRs (512 x 8 = 2^12); a block's gates are
 // applied there, and only between blocks are the states re-laid out through LDS
 // (write in the block's tuple order, barrier, read in the next block's order),
-// two states per round.  Compared with a re-layout per gate this halves the LDS
+// two FP64 or four FP32 states per round.  Compared with a re-layout per gate this halves the LDS
 // traffic and the barriers of the brickwall sweeps.
 //
 // Reverse sweep per gate, in three passes over the tuple's quadruples so that one
@@ -29,116 +29,119 @@ namespace {
 constexpr int kT = 512, kW = 3, kNW = kT / 64;
 
 // (A) uncompute psi (and dpsi), gradient outer products
-template <int LA, int LB, int MODE>
-__device__ inline void pass_a(const cpx<double>* V, const cpx<double>* Z, const Win<double, kW>& Lm, Win<double, kW>& P,
-                              Win<double, kW>& D, cpx<double>* ga)
+template <typename R, int LA, int LB, int MODE>
+__device__ inline void pass_a(const cpx<R>* V, const cpx<R>* Z, const Win<R, kW>& Lm, Win<R, kW>& P, Win<R, kW>& D,
+                              cpx<R>* ga)
 {
 #pragma unroll
     for (int r = 0; r < WQuads<kW, LA, LB>::N; r++) {
-        const Quad<double> p = gapplyH(V, wq_get<double, kW, LA, LB>(P, r));
-        wq_put<double, kW, LA, LB>(P, r, p);
+        const Quad<R> p = gapplyH(V, wq_get<R, kW, LA, LB>(P, r));
+        wq_put<R, kW, LA, LB>(P, r, p);
         if constexpr (MODE == 2)
-            wq_put<double, kW, LA, LB>(D, r, gapplyH(V, qsub(wq_get<double, kW, LA, LB>(D, r), gapply(Z, p))));
-        outer_acc(ga, wq_get<double, kW, LA, LB>(Lm, r), p);
+            wq_put<R, kW, LA, LB>(D, r, gapplyH(V, qsub(wq_get<R, kW, LA, LB>(D, r), gapply(Z, p))));
+        outer_acc(ga, wq_get<R, kW, LA, LB>(Lm, r), p);
         __builtin_amdgcn_sched_barrier(0);  // one quadruple's temporaries at a time
     }
 }
 
 // (B) D^2 f[Z] outer products
-template <int LA, int LB>
-__device__ inline void pass_b(const Win<double, kW>& Lm, const Win<double, kW>& P, const Win<double, kW>& DL,
-                              const Win<double, kW>& D, cpx<double>* ha)
+template <typename R, int LA, int LB>
+__device__ inline void pass_b(const Win<R, kW>& Lm, const Win<R, kW>& P, const Win<R, kW>& DL, const Win<R, kW>& D,
+                              cpx<R>* ha)
 {
 #pragma unroll
     for (int r = 0; r < WQuads<kW, LA, LB>::N; r++) {
-        outer_acc(ha, wq_get<double, kW, LA, LB>(DL, r), wq_get<double, kW, LA, LB>(P, r));
-        outer_acc(ha, wq_get<double, kW, LA, LB>(Lm, r), wq_get<double, kW, LA, LB>(D, r));
+        outer_acc(ha, wq_get<R, kW, LA, LB>(DL, r), wq_get<R, kW, LA, LB>(P, r));
+        outer_acc(ha, wq_get<R, kW, LA, LB>(Lm, r), wq_get<R, kW, LA, LB>(D, r));
     }
 }
 
 // (C) lam <- V^T lam, dlam <- V^T dlam + Z^T lam
-template <int LA, int LB, int MODE>
-__device__ inline void pass_c(const cpx<double>* V, const cpx<double>* Z, Win<double, kW>& Lm, Win<double, kW>& DL)
+template <typename R, int LA, int LB, int MODE>
+__device__ inline void pass_c(const cpx<R>* V, const cpx<R>* Z, Win<R, kW>& Lm, Win<R, kW>& DL)
 {
 #pragma unroll
     for (int r = 0; r < WQuads<kW, LA, LB>::N; r++) {
-        const Quad<double> l = wq_get<double, kW, LA, LB>(Lm, r);
+        const Quad<R> l = wq_get<R, kW, LA, LB>(Lm, r);
         if constexpr (MODE == 2)
-            wq_put<double, kW, LA, LB>(DL, r, gapplyT_add(V, wq_get<double, kW, LA, LB>(DL, r), gapplyT(Z, l)));
-        wq_put<double, kW, LA, LB>(Lm, r, gapplyT(V, l));
+            wq_put<R, kW, LA, LB>(DL, r, gapplyT_add(V, wq_get<R, kW, LA, LB>(DL, r), gapplyT(Z, l)));
+        wq_put<R, kW, LA, LB>(Lm, r, gapplyT(V, l));
         __builtin_amdgcn_sched_barrier(0);
     }
 }
 
-// reduce 8 complex per-thread sums over the wave and stage them (slot base `at`)
-__device__ inline void stage8(const cpx<double>* s, double* slot, int lane)
+// reduce 8 complex per-thread sums over the wave and stage them in FP64
+template <typename R>
+__device__ inline void stage8(const cpx<R>* s, double* slot, int lane)
 {
-    double v[16];
+    R v[16];
 #pragma unroll
     for (int e = 0; e < 8; e++) {
         v[2 * e] = s[e].re;
         v[2 * e + 1] = s[e].im;
     }
-    wave_halve<16, 32, double>(v, lane);
-    if ((lane & 3) == 0) slot[lane >> 2] = v[0];
+    wave_halve<16, 32, R>(v, lane);
+    if ((lane & 3) == 0) slot[lane >> 2] = (double)v[0];
 }
 
 }  // namespace
 
-template <int MODE>
+template <typename R, int MODE>
 __global__ __launch_bounds__(kT) void sweep_win_kernel(WinSweepArgs a)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr int NV = MODE == 2 ? 32 : 16;
+    constexpr bool ONE_ROUND = sizeof(R) == 4;  // all four FP32 states fit the re-layout buffer
     const uint32_t S = 1u << a.L;
     const int tid0 = threadIdx.x, lane = tid0 & 63, wv = tid0 >> 6;
 
     double* stage = reinterpret_cast<double*>(smem);  // [kWinRegGates][kNW][32]
     double* fst = stage + kWinRegGates * kNW * 32;    // [kNW][2]
     double* acc = reinterpret_cast<double*>(smem + a.acc_off);
-    cpx<double>* buf = reinterpret_cast<cpx<double>*>(smem + a.buf_off);  // [2][S] re-layout buffer
+    cpx<R>* buf = reinterpret_cast<cpx<R>*>(smem + a.buf_off);  // [2 or 4][S] re-layout buffer
 
     for (int k = tid0; k < 32 * a.P; k += kT) acc[k] = 0.0;
     double fre = 0.0, fim = 0.0;
     __syncthreads();
 
     const int nb = a.nblocks;
+    const cpx<R>* coef = reinterpret_cast<const cpx<R>*>(a.coef);
     for (long b = a.b0 + blockIdx.x; b < a.b1; b += gridDim.x) {
         // an opaque copy of the thread index: keeps per-thread addresses from being
         // hoisted out of the column loop (and pinned in VGPRs for the whole kernel)
         int tid = tid0;
         asm volatile("" : "+v"(tid));
-        Win<double, kW> Lm, P, DL, D;
+        Win<R, kW> Lm, P, DL, D;
         // ---- forward: psi, dpsi ------------------------------------------------------------
         {
             const BlockRegs<kW> b0 = load_block<kW>(a.blk_glob);
             const uint32_t base = deposit<kW>((uint32_t)tid, b0.pos);
 #pragma unroll
             for (int j = 0; j < (1 << kW); j++) {
-                P.a[j] = {(base | b0.off[j]) == (uint32_t)b ? 1.0 : 0.0, 0.0};
-                D.a[j] = {0.0, 0.0};
+                P.a[j] = {(base | b0.off[j]) == (uint32_t)b ? R(1) : R(0), R(0)};
+                D.a[j] = {R(0), R(0)};
             }
         }
         for (int k = 0; k < nb; k++) {
             const BlockRegs<kW> bk = load_block<kW>(a.blk_lds + k);
             for (int g = bk.g0; g < bk.g1; g++) {
                 const int wcase = cptr(a.inst)[g].lo;
-                cpx<double> cf[24];
-                load_coef_c<double, 8>(a.coef + (size_t)g * kCoefPerInst, cf);
-                if constexpr (MODE == 2) load_coef_c<double, 8>(a.coef + (size_t)g * kCoefPerInst + 16, cf + 16);
-#define MG_FWD(WW, A, B) wg_forward<double, WW, A, B, MODE == 2 ? 2 : 0>(cf, P, D)
+                cpx<R> cf[24];
+                load_coef_c<R, 8>(coef + (size_t)g * kCoefPerInst, cf);
+                if constexpr (MODE == 2) load_coef_c<R, 8>(coef + (size_t)g * kCoefPerInst + 16, cf + 16);
+#define MG_FWD(WW, A, B) wg_forward<R, WW, A, B, MODE == 2 ? 2 : 0>(cf, P, D)
                 MG_WCASES(kW, MG_FWD)
 #undef MG_FWD
             }
             if (k + 1 < nb) {
                 const BlockRegs<kW> nx = load_block<kW>(a.blk_lds + k + 1);
-                const uint32_t sb = lin<double, true>(deposit<kW>((uint32_t)tid, bk.pos));
-                const uint32_t sn = lin<double, true>(deposit<kW>((uint32_t)tid, nx.pos));
-                win_store<double, kW, true>(buf, sb, bk.off, P);
-                if constexpr (MODE == 2) win_store<double, kW, true>(buf + S, sb, bk.off, D);
+                const uint32_t sb = lin<R, true>(deposit<kW>((uint32_t)tid, bk.pos));
+                const uint32_t sn = lin<R, true>(deposit<kW>((uint32_t)tid, nx.pos));
+                win_store<R, kW, true>(buf, sb, bk.off, P);
+                if constexpr (MODE == 2) win_store<R, kW, true>(buf + S, sb, bk.off, D);
                 __syncthreads();
-                win_load<double, kW, true>(buf, sn, nx.off, P);
-                if constexpr (MODE == 2) win_load<double, kW, true>(buf + S, sn, nx.off, D);
+                win_load<R, kW, true>(buf, sn, nx.off, P);
+                if constexpr (MODE == 2) win_load<R, kW, true>(buf + S, sn, nx.off, D);
                 __syncthreads();
             }
         }
@@ -147,7 +150,7 @@ __global__ __launch_bounds__(kT) void sweep_win_kernel(WinSweepArgs a)
         {
             const BlockRegs<kW> bl = load_block<kW>(a.blk_glob + nb - 1);
             const uint32_t base = deposit<kW>((uint32_t)tid, bl.pos);
-            const cpx<double>* row = a.seed + (size_t)b * S;
+            const cpx<R>* row = reinterpret_cast<const cpx<R>*>(a.seed) + (size_t)b * S;
 #pragma unroll
             for (int j = 0; j < (1 << kW); j++) Lm.a[j] = row[base | bl.off[j]];
         }
@@ -156,9 +159,9 @@ __global__ __launch_bounds__(kT) void sweep_win_kernel(WinSweepArgs a)
             double sr = 0.0, si = 0.0;
 #pragma unroll
             for (int j = 0; j < (1 << kW); j++) {
-                const cpx<double> t = cmul(Lm.a[j], P.a[j]);
-                sr += t.re;
-                si += t.im;
+                const cpx<R> t = cmul(Lm.a[j], P.a[j]);
+                sr += (double)t.re;
+                si += (double)t.im;
             }
             sr = wave_sum(sr);
             si = wave_sum(si);
@@ -179,34 +182,34 @@ __global__ __launch_bounds__(kT) void sweep_win_kernel(WinSweepArgs a)
         }
         if constexpr (MODE >= 1) {
 #pragma unroll
-            for (int j = 0; j < (1 << kW); j++) DL.a[j] = {0.0, 0.0};
+            for (int j = 0; j < (1 << kW); j++) DL.a[j] = {R(0), R(0)};
             for (int k = nb - 1; k >= 0; k--) {
                 const BlockRegs<kW> bk = load_block<kW>(a.blk_lds + k);
                 for (int g = bk.g1 - 1; g >= bk.g0; g--) {
                     const int wcase = cptr(a.inst)[g].lo;
                     double* slot = stage + ((g - bk.g0) * kNW + wv) * 32;
-                    cpx<double> V[8], Z[8];
-                    load_coef_c<double, 8>(a.coef + (size_t)g * kCoefPerInst, V);
-                    if constexpr (MODE == 2) load_coef_c<double, 8>(a.coef + (size_t)g * kCoefPerInst + 16, Z);
+                    cpx<R> V[8], Z[8];
+                    load_coef_c<R, 8>(coef + (size_t)g * kCoefPerInst, V);
+                    if constexpr (MODE == 2) load_coef_c<R, 8>(coef + (size_t)g * kCoefPerInst + 16, Z);
                     {
-                        cpx<double> ga[8];
+                        cpx<R> ga[8];
 #pragma unroll
-                        for (int e = 0; e < 8; e++) ga[e] = {0.0, 0.0};
-#define MG_PA(WW, A, B) pass_a<A, B, MODE>(V, Z, Lm, P, D, ga)
+                        for (int e = 0; e < 8; e++) ga[e] = {R(0), R(0)};
+#define MG_PA(WW, A, B) pass_a<R, A, B, MODE>(V, Z, Lm, P, D, ga)
                         MG_WCASES(kW, MG_PA)
 #undef MG_PA
                         stage8(ga, slot, lane);
                     }
                     if constexpr (MODE == 2) {
-                        cpx<double> ha[8];
+                        cpx<R> ha[8];
 #pragma unroll
-                        for (int e = 0; e < 8; e++) ha[e] = {0.0, 0.0};
-#define MG_PB(WW, A, B) pass_b<A, B>(Lm, P, DL, D, ha)
+                        for (int e = 0; e < 8; e++) ha[e] = {R(0), R(0)};
+#define MG_PB(WW, A, B) pass_b<R, A, B>(Lm, P, DL, D, ha)
                         MG_WCASES(kW, MG_PB)
 #undef MG_PB
                         stage8(ha, slot + 16, lane);
                     }
-#define MG_PC(WW, A, B) pass_c<A, B, MODE>(V, Z, Lm, DL)
+#define MG_PC(WW, A, B) pass_c<R, A, B, MODE>(V, Z, Lm, DL)
                     MG_WCASES(kW, MG_PC)
 #undef MG_PC
                 }
@@ -215,11 +218,15 @@ __global__ __launch_bounds__(kT) void sweep_win_kernel(WinSweepArgs a)
                 const bool move = k > 0;
                 BlockRegs<kW> pv = bk;
                 if (move) pv = load_block<kW>(a.blk_lds + k - 1);
-                const uint32_t sb = lin<double, true>(deposit<kW>((uint32_t)tid, bk.pos));
-                const uint32_t sn = lin<double, true>(deposit<kW>((uint32_t)tid, pv.pos));
+                const uint32_t sb = lin<R, true>(deposit<kW>((uint32_t)tid, bk.pos));
+                const uint32_t sn = lin<R, true>(deposit<kW>((uint32_t)tid, pv.pos));
                 if (move) {
-                    win_store<double, kW, true>(buf, sb, bk.off, Lm);
-                    win_store<double, kW, true>(buf + S, sb, bk.off, P);
+                    win_store<R, kW, true>(buf, sb, bk.off, Lm);
+                    win_store<R, kW, true>(buf + S, sb, bk.off, P);
+                    if constexpr (MODE == 2 && ONE_ROUND) {
+                        win_store<R, kW, true>(buf + 2 * S, sb, bk.off, DL);
+                        win_store<R, kW, true>(buf + 3 * S, sb, bk.off, D);
+                    }
                 }
                 __syncthreads();
                 for (int g = bk.g0; g < bk.g1; g++) {
@@ -235,15 +242,19 @@ __global__ __launch_bounds__(kT) void sweep_win_kernel(WinSweepArgs a)
                     }
                 }
                 if (move) {
-                    win_load<double, kW, true>(buf, sn, pv.off, Lm);
-                    win_load<double, kW, true>(buf + S, sn, pv.off, P);
+                    win_load<R, kW, true>(buf, sn, pv.off, Lm);
+                    win_load<R, kW, true>(buf + S, sn, pv.off, P);
+                    if constexpr (MODE == 2 && ONE_ROUND) {
+                        win_load<R, kW, true>(buf + 2 * S, sn, pv.off, DL);
+                        win_load<R, kW, true>(buf + 3 * S, sn, pv.off, D);
+                    }
                     __syncthreads();
-                    if constexpr (MODE == 2) {
-                        win_store<double, kW, true>(buf, sb, bk.off, DL);
-                        win_store<double, kW, true>(buf + S, sb, bk.off, D);
+                    if constexpr (MODE == 2 && !ONE_ROUND) {
+                        win_store<R, kW, true>(buf, sb, bk.off, DL);
+                        win_store<R, kW, true>(buf + S, sb, bk.off, D);
                         __syncthreads();
-                        win_load<double, kW, true>(buf, sn, pv.off, DL);
-                        win_load<double, kW, true>(buf + S, sn, pv.off, D);
+                        win_load<R, kW, true>(buf, sn, pv.off, DL);
+                        win_load<R, kW, true>(buf + S, sn, pv.off, D);
                         __syncthreads();
                     }
                 } else {
@@ -262,30 +273,37 @@ __global__ __launch_bounds__(kT) void sweep_win_kernel(WinSweepArgs a)
     for (int k = tid0; k < 32 * a.P; k += kT) out[2 + k] = acc[k];
 }
 
-size_t win_sweep_lds_bytes(int L, int P, size_t* acc_off, size_t* buf_off)
+size_t win_sweep_lds_bytes(int L, int P, int fp32, size_t* acc_off, size_t* buf_off)
 {
     const size_t fixed = sizeof(double) * ((size_t)kWinRegGates * kNW * 32 + 2 * kNW);
     const size_t a0 = (fixed + 255) & ~(size_t)255;
     const size_t b0 = (a0 + sizeof(double) * 32 * (size_t)P + 255) & ~(size_t)255;
     if (acc_off) *acc_off = a0;
     if (buf_off) *buf_off = b0;
-    return b0 + 2 * ((size_t)1 << L) * sizeof(cpx<double>);
+    // two FP64 or four FP32 states per re-layout round
+    return b0 + (fp32 ? 4 * sizeof(cpx<float>) : 2 * sizeof(cpx<double>)) * ((size_t)1 << L);
 }
 
 bool win_sweep_fits(int L) { return ((size_t)1 << L) == (size_t)kT << kW; }
 
-hipError_t launch_sweep_win(const WinSweepArgs& a, int mode, int grid, size_t lds, hipStream_t s)
+template <typename R>
+static hipError_t launch_win_r(const WinSweepArgs& a, int mode, int grid, size_t lds, hipStream_t s)
 {
-    const void* k = mode == 2 ? (const void*)sweep_win_kernel<2>
-                  : mode == 1 ? (const void*)sweep_win_kernel<1> : (const void*)sweep_win_kernel<0>;
+    const void* k = mode == 2 ? (const void*)sweep_win_kernel<R, 2>
+                  : mode == 1 ? (const void*)sweep_win_kernel<R, 1> : (const void*)sweep_win_kernel<R, 0>;
     if (lds > 65536) {
         hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
-    if (mode == 2) hipLaunchKernelGGL(sweep_win_kernel<2>, dim3(grid), dim3(kT), lds, s, a);
-    else if (mode == 1) hipLaunchKernelGGL(sweep_win_kernel<1>, dim3(grid), dim3(kT), lds, s, a);
-    else hipLaunchKernelGGL(sweep_win_kernel<0>, dim3(grid), dim3(kT), lds, s, a);
+    if (mode == 2) hipLaunchKernelGGL((sweep_win_kernel<R, 2>), dim3(grid), dim3(kT), lds, s, a);
+    else if (mode == 1) hipLaunchKernelGGL((sweep_win_kernel<R, 1>), dim3(grid), dim3(kT), lds, s, a);
+    else hipLaunchKernelGGL((sweep_win_kernel<R, 0>), dim3(grid), dim3(kT), lds, s, a);
     return hipGetLastError();
+}
+
+hipError_t launch_sweep_win(const WinSweepArgs& a, int mode, int fp32, int grid, size_t lds, hipStream_t s)
+{
+    return fp32 ? launch_win_r<float>(a, mode, grid, lds, s) : launch_win_r<double>(a, mode, grid, lds, s);
 }
 
 }  // namespace mg

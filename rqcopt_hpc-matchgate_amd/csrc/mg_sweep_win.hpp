@@ -10,19 +10,19 @@ constexpr int kWinRegGates = 4;  // gates per window block at most (staging slot
 struct WinSweepArgs {
     int L, G, P, nblocks;
     const Inst* inst;          // [G] in block order; lo = window-local pair index
-    const cpx<double>* coef;   // [G][kCoefPerInst] in block order
+    const void* coef;          // cpx<R> [G][kCoefPerInst] in block order
     const WBlock* blk_lds;     // [nblocks] offsets swizzled for the LDS re-layout buffer
     const WBlock* blk_glob;    // [nblocks] plain offsets (seed rows, basis vectors)
-    const cpx<double>* seed;   // [S][S]: row b = -conj(U e_b)
+    const void* seed;          // cpx<R> [S][S]: row b = -conj(U e_b)
     long b0, b1;               // column range
     double* partial;           // [gridDim.x][2 + 32 P]
     int acc_off, buf_off;      // byte offsets in dynamic LDS
 };
 
 // dynamic LDS of the kernel (and the accumulator and buffer offsets)
-size_t win_sweep_lds_bytes(int L, int P, size_t* acc_off, size_t* buf_off);
+size_t win_sweep_lds_bytes(int L, int P, int fp32, size_t* acc_off, size_t* buf_off);
 // 512 threads x 8 amplitudes cover the column
 bool win_sweep_fits(int L);
-hipError_t launch_sweep_win(const WinSweepArgs& a, int mode, int grid, size_t lds, hipStream_t s);
+hipError_t launch_sweep_win(const WinSweepArgs& a, int mode, int fp32, int grid, size_t lds, hipStream_t s);
 
 }  // namespace mg

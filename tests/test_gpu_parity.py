@@ -213,20 +213,26 @@ def test_register_kernel_matches_oracle(mg, L, layout, depth, threads, prec):
 
 
 @pytest.mark.parametrize("layout,depth", [("tied", 3), ("open", 5), ("open", 9)])
-def test_window_register_kernel_matches_oracle(mg, layout, depth):
-    """The register-resident gate-window kernel (FP64, L=12): tied brickwalls with random
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_window_register_kernel_matches_oracle(mg, layout, depth, prec):
+    """The register-resident gate-window kernel (L=12): tied brickwalls with random
     permutations (windows of scattered qubits) and open brickwalls, all three modes."""
     L = 12
-    say(f"case: register window kernel L={L} {layout} depth={depth}")
+    say(f"case: register window kernel L={L} {layout} depth={depth} {prec}")
     V, Z, param, wires, P, U = build_case(L, layout, depth, seed=depth * 5 + 1)
     t = mg.Target.dense(U)
     p = mg.Program.create(L, P, param, wires)
-    r = mg.evaluate_columns(t, p, V, Z, mg.EVAL_F_GRAD_HVP, mg.MG_FP64, 7, 71)
-    assert mg.last_sweep_info()["kernel"] == "sweep_win_kernel"
-    r1 = mg.evaluate_columns(t, p, V, None, mg.EVAL_F_GRAD, mg.MG_FP64, 7, 71)
-    r0 = mg.evaluate_columns(t, p, V, None, mg.EVAL_F, mg.MG_FP64, 7, 71)
+    precision = mg.MG_FP64 if prec == "fp64" else mg.MG_FP32
+    mg.set_window_kernel(1)
+    try:
+        r = mg.evaluate_columns(t, p, V, Z, mg.EVAL_F_GRAD_HVP, precision, 7, 71)
+        assert mg.last_sweep_info()["kernel"] == "sweep_win_kernel"
+        r1 = mg.evaluate_columns(t, p, V, None, mg.EVAL_F_GRAD, precision, 7, 71)
+        r0 = mg.evaluate_columns(t, p, V, None, mg.EVAL_F, precision, 7, 71)
+    finally:
+        mg.set_window_kernel(-1)
     fo, dgo, dho = orc.target_gradient_hvp(U, V, Z, param, wires, L, 7, 71)
-    tol = 1e-11
+    tol = 1e-11 if prec == "fp64" else 2e-5
     assert abs(r[0] - fo) / abs(fo) < tol and rel(r[1], dgo) < tol and rel(r[2], dho) < tol
     assert abs(r1[0] - fo) / abs(fo) < tol and rel(r1[1], dgo) < tol and abs(r0[0] - fo) / abs(fo) < tol
 
