@@ -72,6 +72,16 @@ def cpu_baseline(W, V, v, param, wires, L, budget_s=12.0):
     }
 
 
+def measured_traffic(kernel, dtype):
+    """HBM bytes per launch of this kernel on this workload, from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE measurement (calibrated against a device copy), or None."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{'fp64' if dtype == 'f64' else 'fp32'}_L12_hubbard_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        return json.load(fh)["traffic_bytes"]
+
+
 def copy_calibration(mib):
     """hipMemcpyDtoD of a known size: its FETCH_SIZE / WRITE_SIZE calibrate the sweep kernel's counters."""
     import ctypes
@@ -111,8 +121,8 @@ def served_bench(mg, njobs, seed=1):
     res, st = mg.serve(jobs, batched=True, precision=mg.MG_FP32, precision_switch=3, densify_qubits=12,
                        realtime=True)
     log(f"  {njobs / st['wall_s']:.2f} jobs/s, {st['rounds']} launches for {st['requests']} evaluations")
-    log("the same stream one job at a time (FP64, one launch per evaluation) ...")
-    base, stb = mg.serve(jobs, batched=False, precision=mg.MG_FP64, realtime=True)
+    log("the same stream one job at a time (FP64, one launch per evaluation, targets materialised) ...")
+    base, stb = mg.serve(jobs, batched=False, precision=mg.MG_FP64, realtime=True, densify_qubits=12)
     lat = np.array([r["latency"] for r in res]) * 1e3
     latb = np.array([r["latency"] for r in base]) * 1e3
     # error of each job's final evaluation (FP64 sweeps in the last iterations) and of an FP32
@@ -193,10 +203,11 @@ def main():
     flops = sweep_flops(L, G)
     achieved = flops / (kernel_ms * 1e-3) / 1e12
     roofline = {"bound": "mfma", "achieved": achieved, "peak": PEAK_TFLOPS[dtype], "unit": "TFLOP/s",
-                "frac": achieved / PEAK_TFLOPS[dtype], "traffic": None,
+                "frac": achieved / PEAK_TFLOPS[dtype], "traffic": measured_traffic(info["kernel"], dtype),
                 "kernel": info["kernel"],
                 "kernel_ms": kernel_ms, "flops_per_launch": flops,
-                "note": "VALU FP64/FP32 FMA bound; MI355X FP64 vector peak equals its FP64 matrix peak"}
+                "note": "VALU FP64/FP32 FMA bound (MI355X FP64 vector peak equals its FP64 matrix peak); traffic = "
+                        "HBM bytes per launch from rocprofv3 FETCH_SIZE/WRITE_SIZE, profiles/"}
 
     extra = {}
     if prec == mg.MG_FP64 and not args.no_fp32:
