@@ -102,6 +102,8 @@ typedef struct mg_request {
     numeric* f;                    /* outputs, each nullable */
     double* grad_vec;              /* Riemannian gradient [8 nparams] (what >= MG_EVAL_F_GRAD) */
     double* hvp;                   /* Riemannian Hessian-vector product [8 nparams] */
+    long b0, b1;                   /* basis columns [b0, b1) (partial sums, as mg_evaluate_columns);
+                                      b1 <= 0: all 2^n columns */
 } mg_request;
 int mg_evaluate_batch(mg_request* reqs, int n);
 /* kernel variant of the batched launch per precision: threads per workgroup and gate-window

@@ -58,6 +58,7 @@ _sig("orc_target", [_P, _P, _I, _P, _P, _I, _I, _P])
 _sig("orc_target_gradient", [_P, _P, _I, _I, _P, _P, _I, _I, _P, _P])
 _sig("orc_target_gradient_hessian", [_P, _P, _I, _I, _P, _P, _I, _I, _I, _P, _P, _P])
 _sig("orc_target_gradient_hvp", [_P, _P, _P, _I, _I, _P, _P, _I, _L, _L, _I, _P, _P, _P])
+_sig("orc_target_gradient_hvp_program", [_P, _I, _P, _P, _P, _I, _I, _P, _P, _I, _L, _L, _I, _P, _P, _P])
 for _n in ("orc_multiply",):
     _sig(_n, [_P, _P, _P])
 _sig("orc_adjoint", [_P, _P])
@@ -224,6 +225,20 @@ def target_gradient_hessian(U, gates, param, wires, L, ti=False, nthreads=0):
     _lib.orc_target_gradient_hessian(_p(U), _p(gates), P, len(param), _p(param), _p(wires), L, int(ti), nthreads,
                                      _p(f), _p(dg), _p(hess))
     return f[0], dg.reshape(P, 8), hess
+
+
+def target_gradient_hvp_program(tgates, twires, gates, Z, param, wires, L, b0=0, b1=None, nthreads=0):
+    """As target_gradient_hvp with the target a gate program applied to e_b per column."""
+    tgates, twires = _c(tgates), _i(twires)
+    gates, Z, param, wires = _c(gates), _c(Z), _i(param), _i(wires)
+    P = gates.shape[0]
+    b1 = (1 << L) if b1 is None else b1
+    f = np.zeros(1, np.complex128)
+    dg = np.empty(8 * P, np.complex128)
+    dhz = np.empty(8 * P, np.complex128)
+    _lib.orc_target_gradient_hvp_program(_p(tgates), tgates.shape[0], _p(twires), _p(gates), _p(Z), P, len(param),
+                                         _p(param), _p(wires), L, b0, b1, nthreads, _p(f), _p(dg), _p(dhz))
+    return f[0], dg.reshape(P, 8), dhz.reshape(P, 8)
 
 
 def target_gradient_hvp(U, gates, Z, param, wires, L, b0=0, b1=None, nthreads=0):

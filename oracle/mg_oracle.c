@@ -468,9 +468,46 @@ void orc_target_gradient_hessian(const ocplx* U, const ocplx* gates, int P, int 
  *            lam_g = V_g^T lam_{g+1},  dlam_g = V_g^T dlam_{g+1} + Z_g^T lam_{g+1}
  * The outer products and transposed applications are the reference's backward
  * kernel (src/gate.c:659-772). */
+/* seed s = -conj(T e_b) for a target program T (gate k on (twires[2k], twires[2k+1])), the
+ * per-column ufunc of the matrix-free drivers (examples/hubbard1d/hubbard1d_opt.c:30-39) */
+static void seed_program(const ocplx* tgates, int Gt, const int* twires, int L, long b, ocplx* s, ocplx* tmp)
+{
+    const long S = 1L << L;
+    memset(tmp, 0, S * sizeof(ocplx));
+    tmp[b] = 1;
+    ocplx* in = tmp;
+    ocplx* out = s;
+    for (int g = 0; g < Gt; g++) {
+        orc_apply_gate(tgates + 8 * g, twires[2 * g], twires[2 * g + 1], L, in, out);
+        ocplx* t = in; in = out; out = t;
+    }
+    for (long a = 0; a < S; a++) s[a] = -conj(in[a]);
+}
+
+static void target_gradient_hvp_core(const ocplx* U, const ocplx* tgates, int Gt, const int* twires,
+                                     const ocplx* gates, const ocplx* Z, int P, int G, const int* param,
+                                     const int* wires, int L, long b0, long b1, int nthreads,
+                                     ocplx* f, ocplx* dgrad, ocplx* dhz);
+
 void orc_target_gradient_hvp(const ocplx* U, const ocplx* gates, const ocplx* Z, int P, int G, const int* param,
                              const int* wires, int L, long b0, long b1, int nthreads,
                              ocplx* f, ocplx* dgrad, ocplx* dhz)
+{
+    target_gradient_hvp_core(U, NULL, 0, NULL, gates, Z, P, G, param, wires, L, b0, b1, nthreads, f, dgrad, dhz);
+}
+
+void orc_target_gradient_hvp_program(const ocplx* tgates, int Gt, const int* twires, const ocplx* gates, const ocplx* Z,
+                                     int P, int G, const int* param, const int* wires, int L, long b0, long b1,
+                                     int nthreads, ocplx* f, ocplx* dgrad, ocplx* dhz)
+{
+    target_gradient_hvp_core(NULL, tgates, Gt, twires, gates, Z, P, G, param, wires, L, b0, b1, nthreads, f, dgrad,
+                             dhz);
+}
+
+static void target_gradient_hvp_core(const ocplx* U, const ocplx* tgates, int Gt, const int* twires,
+                                     const ocplx* gates, const ocplx* Z, int P, int G, const int* param,
+                                     const int* wires, int L, long b0, long b1, int nthreads,
+                                     ocplx* f, ocplx* dgrad, ocplx* dhz)
 {
     const long S = 1L << L;
     const int m = 8 * P;
@@ -490,7 +527,8 @@ void orc_target_gradient_hvp(const ocplx* U, const ocplx* gates, const ocplx* Z,
         ocplx* my = part + (size_t)thread_id() * stride;
 #pragma omp for schedule(static)
         for (long b = b0; b < b1; b++) {
-            seed_column(U, S, b, s);
+            if (U) seed_column(U, S, b, s);
+            else seed_program(tgates, Gt, twires, L, b, s, t0);
             memset(cp, 0, S * sizeof(ocplx));
             memset(cd, 0, S * sizeof(ocplx));
             cp[b] = 1;

@@ -62,6 +62,11 @@ void orc_target_gradient_hessian(const ocplx* U, const ocplx* gates, int P, int 
 void orc_target_gradient_hvp(const ocplx* U, const ocplx* gates, const ocplx* Z, int P, int G, const int* param,
                              const int* wires, int L, long b0, long b1, int nthreads,
                              ocplx* f, ocplx* dgrad, ocplx* dhz);
+/* the same with the target given as a gate program (gate k on (twires[2k], twires[2k+1])),
+ * applied to e_b per column instead of a dense U */
+void orc_target_gradient_hvp_program(const ocplx* tgates, int Gt, const int* twires, const ocplx* gates, const ocplx* Z,
+                                     int P, int G, const int* param, const int* wires, int L, long b0, long b1,
+                                     int nthreads, ocplx* f, ocplx* dgrad, ocplx* dhz);
 
 /* ---------------- matchgate algebra (reference src/matchgate.c) ---------------- */
 void orc_multiply(const ocplx* a, const ocplx* b, ocplx* c);

@@ -758,15 +758,17 @@ static int run_batch(Context* c, const std::vector<int>& idx, BatchReq* reqs, Ba
     std::vector<long long> out_off(n);
     for (int i = 0; i < n; i++) {
         const BatchInst<R>& in = insts[i];
+        const BatchReq& rq = reqs[idx[i]];
         const long S = 1L << in.L;
+        const long c0 = rq.b1 > 0 ? rq.b0 : 0, c1 = rq.b1 > 0 ? rq.b1 : S;
         const int stride = 2 + 32 * in.P;
         const double per_col = (double)S * ((double)in.G * (in.mode == 2 ? 18 : in.mode == 1 ? 6 : 2) + 2.0 * in.Gt + 1.0);
         segs[i].src = poff;
         segs[i].dst = ooff;
         segs[i].stride = stride;
         int cnt = 0;
-        for (long b = 0; b < S; b += cpis[i]) {
-            const long e = std::min<long>(S, b + cpis[i]);
+        for (long b = c0; b < c1; b += cpis[i]) {
+            const long e = std::min<long>(c1, b + cpis[i]);
             items.push_back({BatchItem{i, (int)b, (int)e, 0, poff}, per_col * (double)(e - b)});
             poff += stride;
             cnt++;
@@ -869,6 +871,10 @@ int evaluate_batch(BatchReq* reqs, int n, BatchStats* st)
         }
         if (r.mode < 0 || r.mode > 2) { set_error("batched evaluation: invalid mode"); return -1; }
         if (r.mode == 2 && !r.Z) { set_error("batched evaluation: Hessian-vector product needs a direction"); return -1; }
+        if (r.b1 > 0 && (r.b0 < 0 || r.b0 >= r.b1 || r.b1 > (1L << r.p->L))) {
+            set_error("batched evaluation: invalid column range");
+            return -1;
+        }
         (r.precision == kFP32 ? i32 : i64).push_back(i);
     }
     if (!i32.empty() && run_batch<float>(c, i32, reqs, st) < 0) return -1;

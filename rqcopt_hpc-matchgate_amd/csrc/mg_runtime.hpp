@@ -115,6 +115,7 @@ struct BatchReq {
     int mode = 0;
     int precision = 0;
     EvalOut* out = nullptr;
+    long b0 = 0, b1 = 0;        // basis columns [b0, b1); b1 <= 0: all 2^L
 };
 
 struct BatchStats {

@@ -247,6 +247,8 @@ int mg_evaluate_batch(mg_request* reqs, int n)
         br[i].mode = r.what;
         br[i].precision = r.precision == MG_FP32 ? mg::kFP32 : mg::kFP64;
         br[i].out = &outs[i];
+        br[i].b0 = r.b0;
+        br[i].b1 = r.b1;
     }
     mg::BatchStats st;
     if (mg::evaluate_batch(br.data(), n, &st) < 0) return -1;
@@ -320,7 +322,7 @@ int mg_batch_autotune(mg_request* reqs, int n, int precision, int reps, int* thr
             for (int k = 0; k < r.program->p.P; k++) real_to_tangent_matchgate(r.v + 8 * k, &r.gates[k], &Z[i][k]);
         }
         br[i] = mg::BatchReq{r.target->t.get(), &r.program->p, Zp(r.gates), Z[i].empty() ? nullptr : Zp(Z[i].data()),
-                             r.what, pi ? mg::kFP32 : mg::kFP64, &outs[i]};
+                             r.what, pi ? mg::kFP32 : mg::kFP64, &outs[i], r.b0, r.b1};
     }
     mg::BatchTuning& bt = mg::batch_tuning();
     const mg::BatchTuning saved = bt;

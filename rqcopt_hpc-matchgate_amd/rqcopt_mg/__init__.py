@@ -57,7 +57,8 @@ class RTRParams(ctypes.Structure):
 class Request(ctypes.Structure):
     """struct mg_request (include/mg_gpu.h)"""
     _fields_ = [("target", _P), ("program", _P), ("gates", _P), ("v", _P), ("what", ctypes.c_int),
-                ("precision", ctypes.c_int), ("f", _P), ("grad_vec", _P), ("hvp", _P)]
+                ("precision", ctypes.c_int), ("f", _P), ("grad_vec", _P), ("hvp", _P), ("b0", ctypes.c_long),
+                ("b1", ctypes.c_long)]
 
 
 class Job(ctypes.Structure):
@@ -352,7 +353,8 @@ def _requests(requests):
         hv = np.zeros(8 * prog.P)
         keep += [gates, vv, f, g, hv]
         arr[k] = Request(r["target"].h, prog.h, gates.ctypes.data, vv.ctypes.data if vv is not None else None, what,
-                         int(r.get("precision", MG_FP64)), f.ctypes.data, g.ctypes.data, hv.ctypes.data)
+                         int(r.get("precision", MG_FP64)), f.ctypes.data, g.ctypes.data, hv.ctypes.data,
+                         int(r.get("b0", 0)), int(r.get("b1", 0)))
         outs.append((f, g, hv, what))
     return arr, keep, outs
 
@@ -372,7 +374,8 @@ def evaluate_batch(requests):
     """Evaluate many requests in shared launches.
 
     requests: list of dicts with keys target, program, gates, v (or None), what,
-    precision.  Returns a list of (f, grad_vec, hvp) like riemannian_gradient_hvp
+    precision and optionally b0, b1 (a column range: partial sums).  Returns a list of
+    (f, grad_vec, hvp) like riemannian_gradient_hvp
     (grad_vec None for what == EVAL_F, hvp None unless what == EVAL_F_GRAD_HVP).
     """
     arr, keep, outs = _requests(requests)

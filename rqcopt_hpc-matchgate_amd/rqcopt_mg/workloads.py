@@ -122,3 +122,74 @@ def served_stream(n, seed=1, rate=200.0, Ls=(8, 10, 12, 14, 16), depths=tuple(ra
         job["arrival"] = t
         out.append(job)
     return out
+
+
+def matchgate_from_matrix(M):
+    """The 8 matchgate entries of a parity-conserving 4x4 matrix in the basis |q_i q_j> =
+    00, 01, 10, 11: center block on {01, 10} (entries 0-3), corner block on {00, 11} (4-7)."""
+    return np.array([M[1, 1], M[1, 2], M[2, 1], M[2, 2], M[0, 0], M[0, 3], M[3, 0], M[3, 3]], np.complex128)
+
+
+def xxz_bond_matrix(tau, hi=0.0, hj=0.0):
+    """exp(-i tau h) for the bond term h = (X X + Y Y)/2 + 2 Z Z + hi Z_i + hj Z_j
+    (4x4, basis 00, 01, 10, 11 of the bond's qubits i < j)."""
+    h = np.zeros((4, 4))
+    h[1, 2] = h[2, 1] = 1.0                       # (XX + YY)/2 flips 01 <-> 10
+    h += 2.0 * np.diag([1.0, -1.0, -1.0, 1.0])    # 2 ZZ
+    h += hi * np.diag([1.0, 1.0, -1.0, -1.0]) + hj * np.diag([1.0, -1.0, 1.0, -1.0])
+    w, v = np.linalg.eigh(h)
+    return (v * np.exp(-1j * tau * w)) @ v.conj().T
+
+
+def xxz_fields(L, seed=0):
+    """Random fields h_j ~ U(-1, 1) of the L=16 workload, seeded."""
+    return np.random.default_rng(seed).uniform(-1.0, 1.0, L)
+
+
+def xxz_trotter_target(L, steps=50, t=1.0, seed=0):
+    """Second-order Trotter circuit of H = sum_j [(X_j X_{j+1} + Y_j Y_{j+1})/2 + 2 Z_j Z_{j+1}]
+    + sum_j h_j Z_j (open chain, h_j ~ U(-1, 1) seeded) for time t in `steps` steps, as
+    parity-conserving nearest-neighbour matchgates.  H = H_even + H_odd by bond parity;
+    each site's field is split equally between its bonds (a boundary site gives all of it
+    to its one bond).  The steps' half layers are merged:
+    E(dt/2) [O(dt) E(dt)]^(steps-1) O(dt) E(dt/2), i.e. 2 steps + 1 layers.
+    Returns (gates (Gt, 8), wires (2 Gt,), fields)."""
+    h = xxz_fields(L, seed)
+    dt = t / steps
+    nb = np.array([2.0] * L)
+    nb[0] = nb[-1] = 1.0                            # bonds per site
+    layers = [(0, dt / 2)] + [(1, dt), (0, dt)] * (steps - 1) + [(1, dt), (0, dt / 2)]
+    gates, wires = [], []
+    for start, tau in layers:
+        for a in range(start, L - 1, 2):
+            M = xxz_bond_matrix(tau, h[a] / nb[a], h[a + 1] / nb[a + 1])
+            gates.append(matchgate_from_matrix(M))
+            wires += [a, a + 1]
+    return np.stack(gates), np.asarray(wires, np.int32), h
+
+
+def xxz_hamiltonian(L, h):
+    """Dense H of xxz_trotter_target (qubit 0 = most significant bit), for small L."""
+    S = 1 << L
+    idx = np.arange(S)
+    bits = [(idx >> (L - 1 - q)) & 1 for q in range(L)]
+    z = [1.0 - 2.0 * b for b in bits]
+    H = np.zeros((S, S), np.complex128)
+    diag = sum(2.0 * z[j] * z[j + 1] for j in range(L - 1)) + sum(h[j] * z[j] for j in range(L))
+    H[idx, idx] = diag
+    for j in range(L - 1):
+        flip = bits[j] != bits[j + 1]
+        s2 = idx ^ (1 << (L - 1 - j)) ^ (1 << (L - 2 - j))
+        H[s2[flip], idx[flip]] += 1.0
+    return H
+
+
+def large_instance(L=16, depth=11, steps=50, t=1.0, seed=0):
+    """The large single instance: an open brickwall of `depth` layers with seeded Haar
+    gates, and the XXZ Trotter circuit as its matrix-free target."""
+    rng = np.random.default_rng(seed)
+    param, wires, G = open_brickwall(L, depth)
+    gates = haar_matchgates(rng, G)
+    tg, tw, h = xxz_trotter_target(L, steps, t, seed)
+    return {"L": L, "param": param, "wires": wires, "G": G, "gates": gates, "target_gates": tg,
+            "target_wires": tw, "Gt": len(tg), "fields": h}

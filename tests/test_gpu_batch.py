@@ -207,3 +207,28 @@ def test_serve_admission_limits(mg):
     assert st2["max_active"] == 1
     for a, b, c in zip(free, one, tight):
         assert np.array_equal(a["f_iter"], b["f_iter"]) and np.array_equal(a["f_iter"], c["f_iter"])
+
+
+@pytest.mark.parametrize("L", [14, 16])
+def test_large_instances_column_subset_match_oracle(mg, L):
+    """L=14 (a batched-workload size) and the L=16 XXZ-Trotter instance (758 target gates
+    applied per column, states in global scratch): a column subset of the batched kernel
+    against the oracle's program-target evaluation, FP64 (1e-11) and FP32 sweeps (2e-5)."""
+    from rqcopt_mg.workloads import brickwall_job, large_instance
+
+    rng = np.random.default_rng(30 + L)
+    job = large_instance() if L == 16 else brickwall_job(rng, L, 5, 1)
+    prog = mg.Program.create(L, job["G"], job["param"], job["wires"])
+    tgt = mg.Target.circuit(job["target_gates"], job["target_wires"], L)
+    v = rng.standard_normal(8 * job["G"])
+    b0, b1 = 1000, 1008
+    say(f"case: L={L} column subset [{b0}, {b1}), {job['G']} gates, {job['Gt']} target gates")
+    V = job["gates"]
+    Z = orc.real_to_directions(V, v)
+    fo, dg, dh = orc.target_gradient_hvp_program(job["target_gates"], job["target_wires"], V, Z, job["param"],
+                                                 job["wires"], L, b0, b1, nthreads=8)
+    go, ho = orc.riemannian_gradient(V, dg), orc.riemannian_hvp(V, dg, Z, dh)
+    for prec, tol in ((mg.MG_FP64, 1e-11), (mg.MG_FP32, 2e-5)):
+        f, g, h = mg.evaluate_batch([{"target": tgt, "program": prog, "gates": V, "v": v, "what": mg.EVAL_F_GRAD_HVP,
+                                      "precision": prec, "b0": b0, "b1": b1}])[0]
+        assert abs(f - fo) <= tol * abs(fo) and rel(g, go) < tol and rel(h, ho) < tol, (prec, rel(g, go), rel(h, ho))
