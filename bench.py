@@ -150,6 +150,41 @@ def served_bench(mg, njobs, seed=1):
                         "circuit targets of 3x depth; FP32 sweeps, last 3 iterations FP64"}
 
 
+def large_bench(mg):
+    """The large single instance: L=16, open brickwall depth 11 (83 gates), target = 50
+    second-order Trotter steps of the XXZ chain with random fields (758 matrix-free gates);
+    one f + gradient + HVP over all 65536 columns with FP32 sweeps and with FP64 sweeps
+    (batched kernel, states in global scratch, gate windows)."""
+    from rqcopt_mg.workloads import large_instance, sweep_flops
+
+    w = large_instance()
+    prog = mg.Program.create(w["L"], w["G"], w["param"], w["wires"])
+    tgt = mg.Target.circuit(w["target_gates"], w["target_wires"], w["L"])
+    v = np.random.default_rng(0).standard_normal(8 * w["G"])
+    fl = sweep_flops(w["L"], w["G"], w["Gt"])
+    out = {"workload": "L=16 XXZ Trotter-2 target (50 steps, t=1, 758 gates), open brickwall depth 11 (83 gates), "
+                       "65536 columns", "flops": fl}
+    res = {}
+    for name, prec in (("fp32", mg.MG_FP32), ("fp64", mg.MG_FP64)):
+        log(f"L=16 f+grad+HVP, {name} sweeps ...")
+        req = {"target": tgt, "program": prog, "gates": w["gates"], "v": v, "what": mg.EVAL_F_GRAD_HVP, "precision": prec}
+        t0 = time.perf_counter()
+        res[name] = mg.evaluate_batch([req])[0]
+        dt = time.perf_counter() - t0
+        km = mg.batch_last_kernel_ms()
+        out[name] = {"s": dt, "kernel_ms": km, "tflops": fl / (km * 1e-3) / 1e12,
+                     "kernel": "batch_kernel<%s, %d, %d>" % ((("float",) if prec == mg.MG_FP32 else ("double",)) +
+                                                              mg.batch_get_kernel(prec))}
+        log(f"  {dt:.2f} s, kernel {km:.1f} ms, {out[name]['tflops']:.2f} TFLOP/s")
+    f32, g32, h32 = res["fp32"]
+    f64, g64, h64 = res["fp64"]
+    out["rel_err_fp32_vs_fp64"] = {"f": float(abs(f32 - f64) / abs(f64)),
+                                   "grad": float(np.max(np.abs(g32 - g64)) / np.max(np.abs(g64))),
+                                   "hvp": float(np.max(np.abs(h32 - h64)) / np.max(np.abs(h64)))}
+    out["f"] = [f64.real, f64.imag]
+    return out
+
+
 def time_steps(mg, t, p, V, v, prec, steps, warmup):
     for _ in range(warmup):
         mg.riemannian_gradient_hvp(t, p, V, v, prec)
@@ -172,6 +207,7 @@ def main():
     ap.add_argument("--rtr-iters", type=int, default=5, help="full trust-region iterations timed for the extra field")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-sweep variant")
     ap.add_argument("--serve-jobs", type=int, default=32, help="jobs of the served stream (0 skips it)")
+    ap.add_argument("--large", action="store_true", help="also run the L=16 instance (about 20 s of GPU time)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 5 times (reference kernel for HBM byte counters)")
     args = ap.parse_args()
@@ -233,6 +269,9 @@ def main():
 
     if args.serve_jobs > 0:
         extra["served"] = served_bench(mg, args.serve_jobs)
+    if args.large:
+        mg.set_kernel_timing(True)
+        extra["large_L16"] = large_bench(mg)
 
     cpu = None
     if not args.no_cpu_baseline:
