@@ -78,3 +78,35 @@ def test_oracle_program_target_equals_dense_target():
                                         job["wires"], 6, 3, 40)
     for x, y in zip(d, p):
         assert np.max(np.abs(np.asarray(x) - np.asarray(y))) < 1e-12 * max(1.0, np.max(np.abs(np.asarray(x))))
+
+
+def test_fermi_hubbard_target_matches_jordan_wigner_expm():
+    """The L=12 bench target's generator (sector-block eigendecomposition, bit arithmetic)
+    against exp(-iHt) of H built independently from Jordan-Wigner fermion operators
+    (c_q = Z_0 ... Z_{q-1} a_q, qubit 0 most significant; spin-up of site j on qubit 2j,
+    spin-down on 2j+1), for 3 sites."""
+    from rqcopt_mg.workloads import fermi_hubbard_1d_unitary
+
+    nsites, J, Uc, t = 3, 1.0, 4.0, 0.25
+    L = 2 * nsites
+    Zm, Im = np.diag([1.0, -1.0]), np.eye(2)
+    am = np.array([[0.0, 1.0], [0.0, 0.0]])  # |1> -> |0>
+
+    def op(q):
+        mats = [Zm] * q + [am] + [Im] * (L - q - 1)
+        out = mats[0]
+        for m in mats[1:]:
+            out = np.kron(out, m)
+        return out
+
+    c = [op(q) for q in range(L)]
+    n = [ci.T @ ci for ci in c]
+    H = np.zeros((1 << L, 1 << L))
+    for j in range(nsites - 1):
+        for s in range(2):
+            a, b = 2 * j + s, 2 * j + 2 + s
+            H += -J * (c[a].T @ c[b] + c[b].T @ c[a])
+    for j in range(nsites):
+        H += Uc * n[2 * j] @ n[2 * j + 1]
+    W = fermi_hubbard_1d_unitary(nsites, J=J, U=Uc, t=t)
+    assert np.max(np.abs(W - sl.expm(-1j * t * H))) < 1e-12
