@@ -111,6 +111,10 @@ int mg_evaluate_batch(mg_request* reqs, int n);
  * Compiled: FP32 (256, 2..4), (512, 2..3); FP64 (256, 2..3), (512, 2).  Returns -1 for others. */
 int mg_batch_set_kernel(int precision, int threads, int window_bits);
 void mg_batch_get_kernel(int precision, int* threads, int* window_bits);
+/* requests with dense targets of min_qubits..max_qubits qubits (default 12..12) fill the GPU on
+ * their own: mg_evaluate_batch runs them on the single-request kernels (register gate-window,
+ * LDS sweeps), one launch each, and the rest in the persistent batch; (0, 0) batches everything */
+void mg_batch_set_single_range(int min_qubits, int max_qubits);
 /* workgroups per CU of the batched launch (0 automatic) */
 void mg_batch_set_grid_per_cu(int n);
 /* batch shape for nqubits in the given precision: threads per column (64 .. threads, a power
@@ -169,6 +173,8 @@ typedef struct mg_serve_stats {
     double kernel_ms;      /* sweep kernel time (with mg_set_kernel_timing(1)) */
     int max_active;        /* peak admitted jobs */
     double peak_footprint; /* peak device bytes held by admitted jobs */
+    double eval_s;         /* wall time inside evaluations (host preparation, launches, waits) */
+    double prep_s;         /* host time preparing batched launches */
 } mg_serve_stats;
 
 int mg_serve(mg_job* jobs, int njobs, const mg_serve_options* opt, mg_serve_stats* stats);

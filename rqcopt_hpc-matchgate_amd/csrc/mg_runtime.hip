@@ -2,6 +2,7 @@
 #include "mg_runtime.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -667,6 +668,7 @@ struct Arena {
 template <typename R>
 static int run_batch(Context* c, const std::vector<int>& idx, BatchReq* reqs, BatchStats* st)
 {
+    const auto t_start = std::chrono::steady_clock::now();
     hipStream_t s = c->s;
     const int prec = std::is_same<R, float>::value ? kFP32 : kFP64;
     const int pi = prec == kFP32 ? 1 : 0;
@@ -814,6 +816,7 @@ static int run_batch(Context* c, const std::vector<int>& idx, BatchReq* reqs, Ba
         return -1;
     }
 
+    const double prep_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
     MG_HIP(hipMemcpyAsync(c->b_up.p, c->b_up_h, up_bytes, hipMemcpyHostToDevice, s));
     if (kn.time_kernels) MG_HIP(hipEventRecord(c->ev0, s));
     MG_HIP(launch_batch<R>((const BatchInst<R>*)(d + o_insts), (const BatchItem*)(d + o_items), nitems, (int*)c->b_counter.p,
@@ -845,6 +848,7 @@ static int run_batch(Context* c, const std::vector<int>& idx, BatchReq* reqs, Ba
         st->items += nitems;
         st->grid = grid;
         st->lds_bytes = std::max(st->lds_bytes, lds);
+        st->prep_ms += prep_ms;
         float ms = 0.f;
         if (kn.time_kernels) MG_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
         st->kernel_ms += ms;
@@ -857,7 +861,7 @@ int evaluate_batch(BatchReq* reqs, int n, BatchStats* st)
     Context* c = ctx();
     if (!c) { set_error("no HIP device available"); return -1; }
     if (st) *st = BatchStats{};
-    std::vector<int> i32, i64;
+    std::vector<int> i32, i64, single;
     for (int i = 0; i < n; i++) {
         BatchReq& r = reqs[i];
         std::string why;
@@ -875,10 +879,26 @@ int evaluate_batch(BatchReq* reqs, int n, BatchStats* st)
             set_error("batched evaluation: invalid column range");
             return -1;
         }
+        const BatchTuning& bt = batch_tuning();
+        if (r.t->kind == 0 && r.p->L >= bt.single_min && r.p->L <= bt.single_max) {
+            single.push_back(i);
+            continue;
+        }
         (r.precision == kFP32 ? i32 : i64).push_back(i);
     }
     if (!i32.empty() && run_batch<float>(c, i32, reqs, st) < 0) return -1;
     if (!i64.empty() && run_batch<double>(c, i64, reqs, st) < 0) return -1;
+    for (int i : single) {
+        BatchReq& r = reqs[i];
+        EvalStats es;
+        if (evaluate(*r.t, *r.p, r.gates, r.Z, r.mode, r.precision, r.out, r.b1 > 0 ? r.b0 : 0, r.b1 > 0 ? r.b1 : -1,
+                     &es) < 0)
+            return -1;
+        if (st) {
+            st->launches++;
+            st->kernel_ms += es.kernel_ms;
+        }
+    }
     return 0;
 }
 

@@ -124,6 +124,7 @@ struct BatchStats {
     int grid = 0;
     size_t lds_bytes = 0;
     float kernel_ms = 0.f;
+    double prep_ms = 0.0;  // host time preparing the launches (instances, window blocks, upload staging)
 };
 
 // Per (precision, L) shape of the batched kernel: column-group width cgw (threads per
@@ -136,6 +137,10 @@ struct BatchTuning {
     int grid_per_cu = 0;
     int cgw[2][31] = {};
     int cpi[2][31] = {};
+    // requests with dense targets and single_min <= L <= single_max fill the GPU on their own:
+    // they run on the single-request kernels (register gate-window / LDS sweeps) in their own
+    // launches instead of inside the persistent batch (0 / 0: never)
+    int single_min = 12, single_max = 12;
 };
 BatchTuning& batch_tuning();
 void batch_shape(int L, int precision, int* cgw, int* cpi);

@@ -74,7 +74,7 @@ struct Server {
     int running = 0;
     bool batched = true;
     long rounds = 0, requests = 0;
-    double kernel_ms = 0.0;
+    double kernel_ms = 0.0, eval_s = 0.0, prep_ms = 0.0;
 };
 
 int job_precision(const JobRt* j)
@@ -89,7 +89,9 @@ int job_eval(JobRt* j, const zd* gates, const zd* Z, int mode, mg::EvalOut* out)
     const mg::Program& prog = j->spec->program->p;
     if (!j->srv->batched) {
         mg::EvalStats st;
+        const auto t0 = std::chrono::steady_clock::now();
         const int rc = mg::evaluate(*j->target, prog, gates, Z, mode, job_precision(j), out, 0, -1, &st);
+        j->srv->eval_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         j->srv->rounds++;
         j->srv->requests++;
         j->srv->kernel_ms += st.kernel_ms;
@@ -271,6 +273,12 @@ int mg_batch_set_kernel(int precision, int threads, int window_bits)
     mg::batch_tuning().threads[pi] = threads;
     mg::batch_tuning().wbits[pi] = window_bits;
     return 0;
+}
+
+void mg_batch_set_single_range(int min_qubits, int max_qubits)
+{
+    mg::batch_tuning().single_min = min_qubits;
+    mg::batch_tuning().single_max = max_qubits;
 }
 
 void mg_batch_get_kernel(int precision, int* threads, int* window_bits)
@@ -480,11 +488,15 @@ int mg_serve(mg_job* jobs, int njobs, const mg_serve_options* opt, mg_serve_stat
                 std::vector<mg::BatchReq> reqs(batch.size());
                 for (size_t k = 0; k < batch.size(); k++) reqs[k] = batch[k]->req;
                 mg::BatchStats st;
+                const auto t0 = clock::now();
                 const int r = mg::evaluate_batch(reqs.data(), (int)reqs.size(), &st);
+                const double dt = std::chrono::duration<double>(clock::now() - t0).count();
                 lk.lock();
                 srv.rounds++;
                 srv.requests += (long)batch.size();
                 srv.kernel_ms += st.kernel_ms;
+                srv.eval_s += dt;
+                srv.prep_ms += st.prep_ms;
                 for (Posted* p : batch) {
                     p->status = r;
                     p->done = true;
@@ -512,6 +524,8 @@ int mg_serve(mg_job* jobs, int njobs, const mg_serve_options* opt, mg_serve_stat
         stats->kernel_ms = srv.kernel_ms;
         stats->max_active = peak_active;
         stats->peak_footprint = (double)peak_held;
+        stats->eval_s = srv.eval_s;
+        stats->prep_s = srv.prep_ms * 1e-3;
     }
     return rc;
 }

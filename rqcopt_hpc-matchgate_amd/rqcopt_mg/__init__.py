@@ -79,7 +79,7 @@ class ServeOptions(ctypes.Structure):
 class ServeStats(ctypes.Structure):
     """struct mg_serve_stats (include/mg_gpu.h)"""
     _fields_ = [("wall_s", _D), ("rounds", ctypes.c_long), ("requests", ctypes.c_long), ("kernel_ms", _D),
-                ("max_active", ctypes.c_int), ("peak_footprint", _D)]
+                ("max_active", ctypes.c_int), ("peak_footprint", _D), ("eval_s", _D), ("prep_s", _D)]
 
 
 def rtr_default_params(n):
@@ -127,6 +127,7 @@ _sig("mg_target_cache_clear", [], None)
 _sig("mg_evaluate_batch", [ctypes.POINTER(Request), _I])
 _sig("mg_batch_set_kernel", [_I, _I, _I])
 _sig("mg_batch_get_kernel", [_I, _P, _P], None)
+_sig("mg_batch_set_single_range", [_I, _I], None)
 _sig("mg_batch_set_grid_per_cu", [_I], None)
 _sig("mg_batch_set_shape", [_I, _I, _I, _I], None)
 _sig("mg_batch_get_shape", [_I, _I, _P, _P], None)
@@ -390,6 +391,10 @@ BATCH_VARIANTS = {MG_FP32: [(256, 4), (256, 3), (256, 2), (512, 3), (512, 2)],
 
 def batch_set_kernel(precision, threads, window_bits):
     _check(lib.mg_batch_set_kernel(int(precision), int(threads), int(window_bits)), "mg_batch_set_kernel")
+
+
+def batch_set_single_range(min_qubits, max_qubits):
+    lib.mg_batch_set_single_range(int(min_qubits), int(max_qubits))
 
 
 def batch_get_kernel(precision):

@@ -232,3 +232,33 @@ def test_large_instances_column_subset_match_oracle(mg, L):
         f, g, h = mg.evaluate_batch([{"target": tgt, "program": prog, "gates": V, "v": v, "what": mg.EVAL_F_GRAD_HVP,
                                       "precision": prec, "b0": b0, "b1": b1}])[0]
         assert abs(f - fo) <= tol * abs(fo) and rel(g, go) < tol and rel(h, ho) < tol, (prec, rel(g, go), rel(h, ho))
+
+
+def test_batch_routes_dense_L12_to_single_request_kernels(mg):
+    """Dense-target L=12 requests run on the single-request kernels (FP64: register gate
+    windows) inside mg_evaluate_batch: bitwise the single path's result; with the routing off
+    they run in the persistent batch kernel, equal to rounding."""
+    rng = np.random.default_rng(15)
+    from rqcopt_mg.workloads import brickwall_job
+
+    j12 = brickwall_job(rng, 12, 5, 1)
+    W = program_matrix(j12["target_gates"][:20], np.arange(20), j12["target_wires"][:40], 12)
+    t12 = mg.Target.dense(W)
+    p12 = mg.Program.create(12, j12["G"], j12["param"], j12["wires"])
+    job, prog, tgt, v = make_instance(mg, rng, 8, 3)
+    v12 = rng.standard_normal(8 * j12["G"])
+    say("case: batch with a dense L=12 request routed to the single-request kernels")
+    for prec, tol in ((mg.MG_FP64, 1e-12), (mg.MG_FP32, 1e-5)):
+        reqs = [{"target": t12, "program": p12, "gates": j12["gates"], "v": v12, "what": 2, "precision": prec},
+                {"target": tgt, "program": prog, "gates": job["gates"], "v": v, "what": 2, "precision": prec}]
+        out = mg.evaluate_batch(reqs)
+        single = mg.riemannian_gradient_hvp(t12, p12, j12["gates"], v12, prec)
+        for a, b in zip(out[0], single):
+            assert np.array_equal(np.asarray(a), np.asarray(b))
+        mg.batch_set_single_range(0, 0)
+        try:
+            o2 = mg.evaluate_batch(reqs)
+        finally:
+            mg.batch_set_single_range(12, 12)
+        assert abs(o2[0][0] - single[0]) <= tol * abs(single[0])
+        assert rel(o2[0][1], single[1]) < tol and rel(o2[0][2], single[2]) < tol
