@@ -46,11 +46,16 @@ struct Bits<float> {
 #define MG_SGPR_COEF 1
 #endif
 
+// With the permlane swaps (offsets 32 and 16) no lane select is needed: the swap leaves
+// (a_lo, b_lo) in one register and (a_hi, b_hi) in the other (halves of 32 lanes, or
+// alternating rows of 16), so their sum is a + partner's a where bit O is clear and
+// b + partner's b where it is set: the same operands as the select form, hence the same bits.
 template <int O>
 __device__ inline float halve_pair(float a, float b, int lane)
 {
     const bool up = (lane & O) != 0;
     if constexpr (MG_PERMLANE_BUTTERFLY && (O == 32 || O == 16)) {
+        (void)up;
         const uint32_t ua = __builtin_bit_cast(uint32_t, a), ub = __builtin_bit_cast(uint32_t, b);
         uint32_t na, nb;
         if constexpr (O == 32) {
@@ -62,7 +67,7 @@ __device__ inline float halve_pair(float a, float b, int lane)
             na = r[0];
             nb = r[1];
         }
-        return up ? b + __builtin_bit_cast(float, na) : a + __builtin_bit_cast(float, nb);
+        return __builtin_bit_cast(float, na) + __builtin_bit_cast(float, nb);
     } else {
         const float send = up ? a : b;
         const float keep = up ? b : a;
@@ -87,9 +92,10 @@ __device__ inline double halve_pair(double a, double b, int lane)
             auto r1 = __builtin_amdgcn_permlane16_swap(ah, bh, false, false);
             nal = r0[0]; nbl = r0[1]; nah = r1[0]; nbh = r1[1];
         }
+        (void)up;
         const double na = __builtin_bit_cast(double, (uint64_t)nal | ((uint64_t)nah << 32));
         const double nb = __builtin_bit_cast(double, (uint64_t)nbl | ((uint64_t)nbh << 32));
-        return up ? b + na : a + nb;
+        return na + nb;
     } else {
         const double send = up ? a : b;
         const double keep = up ? b : a;
