@@ -12,7 +12,7 @@
 namespace mg {
 
 template <typename R>
-struct cpx {
+struct alignas(2 * sizeof(R)) cpx {  // full-size alignment: one 64-bit (FP32) or 128-bit (FP64) access
     R re, im;
 };
 
