@@ -139,7 +139,9 @@ __device__ inline void load_coef(const cpx<R>* src, cpx<R>* dst)
 #pragma unroll
     for (int k = 0; k < N; k++) {
         // measured on MI355X: scalar coefficients relieve the FP64 kernels' VGPR pressure but
-        // slow the FP32 kernels, which have VGPRs to spare
+        // slow the FP32 kernels, which have VGPRs to spare.  (Scalar loads through the constant
+        // address space were slower still in the LDS-resident kernels: their waits share the
+        // lgkm counter with the LDS traffic.)
         if constexpr (MG_SGPR_COEF && sizeof(R) == 8) dst[k] = {uniform(src[k].re), uniform(src[k].im)};
         else dst[k] = src[k];
     }
