@@ -400,7 +400,7 @@ static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, in
     size_t win_acc = 0, win_buf = 0, win_lds = 0;
     bool use_win = false;
     constexpr int fp32 = std::is_same<R, float>::value ? 1 : 0;
-    if (!ckpt && b.defect <= 1e-12 && t.kind == 0 && win_sweep_fits(L) && kn.force_win != 0 && kn.force_reg != 0 &&
+    if (!ckpt && b.defect <= 1e-12 && t.kind == 0 && win_sweep_fits(L, fp32) && kn.force_win != 0 && kn.force_reg != 0 &&
         kn.force_lds != 0 && (!fp32 || kn.force_win == 1)) {
         win_lds = win_sweep_lds_bytes(L, p.P, fp32, &win_acc, &win_buf);
         use_win = win_lds <= c->lds_limit;
@@ -459,7 +459,7 @@ static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, in
     WinSweepArgs wa{};
     if (use_win) {
         Windowed w;
-        make_windowed<R>(b.inst, b.coef, L, 3, &w, kWinRegGates);
+        make_windowed<R>(b.inst, b.coef, L, win_sweep_window_bits(fp32), &w, kWinRegGates);
         std::vector<cpx<R>> wc(w.coef.size());
         for (size_t k = 0; k < wc.size(); k++) wc[k] = {(R)w.coef[k].re, (R)w.coef[k].im};
         MG_HIP(c->w_inst.ensure(w.inst.size() * sizeof(Inst)));

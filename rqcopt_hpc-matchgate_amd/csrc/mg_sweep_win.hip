@@ -1,9 +1,9 @@
 // Register-resident column sweep with gate windows at L=12 (FP64, whose states,
 // 4 x 64 KB, do not fit the LDS, and FP32).
 //
-// The program's gates are regrouped into blocks acting inside 3-qubit windows
-// (mg_window.cpp).  Each of the 512 threads keeps, for every live state, the 8
-// amplitudes of one window tuple in VGPRs (512 x 8 = 2^12); a block's gates are
+// The program's gates are regrouped into blocks acting inside W-qubit windows
+// (mg_window.cpp).  Each thread keeps, for every live state, the 2^W amplitudes of one
+// window tuple in VGPRs (FP64: 512 threads x 8, FP32: 256 x 16 = 2^12); a block's gates are
 // applied there, and only between blocks are the states re-laid out through LDS
 // (write in the block's tuple order, barrier, read in the next block's order),
 // two FP64 or four FP32 states per round.  Compared with a re-layout per gate this halves the LDS
@@ -26,10 +26,21 @@ namespace mg {
 
 namespace {
 
-constexpr int kT = 512, kW = 3, kNW = kT / 64;
+// FP64: 512 threads x 3-qubit windows (8 amplitudes per state); FP32: 256 threads x 4-qubit
+// windows (16 amplitudes per state, one relayout per ~3.6 brickwall gates)
+template <typename R>
+struct WinShape;
+template <>
+struct WinShape<double> {
+    static constexpr int T = 512, W = 3, NST = 2;
+};
+template <>
+struct WinShape<float> {
+    static constexpr int T = 256, W = 4, NST = 4;
+};
 
 // (A) uncompute psi (and dpsi), gradient outer products
-template <typename R, int LA, int LB, int MODE>
+template <typename R, int kW, int LA, int LB, int MODE>
 __device__ inline void pass_a(const cpx<R>* V, const cpx<R>* Z, const Win<R, kW>& Lm, Win<R, kW>& P, Win<R, kW>& D,
                               cpx<R>* ga)
 {
@@ -45,7 +56,7 @@ __device__ inline void pass_a(const cpx<R>* V, const cpx<R>* Z, const Win<R, kW>
 }
 
 // (B) D^2 f[Z] outer products
-template <typename R, int LA, int LB>
+template <typename R, int kW, int LA, int LB>
 __device__ inline void pass_b(const Win<R, kW>& Lm, const Win<R, kW>& P, const Win<R, kW>& DL, const Win<R, kW>& D,
                               cpx<R>* ha)
 {
@@ -57,7 +68,7 @@ __device__ inline void pass_b(const Win<R, kW>& Lm, const Win<R, kW>& P, const W
 }
 
 // (C) lam <- V^T lam, dlam <- V^T dlam + Z^T lam
-template <typename R, int LA, int LB, int MODE>
+template <typename R, int kW, int LA, int LB, int MODE>
 __device__ inline void pass_c(const cpx<R>* V, const cpx<R>* Z, Win<R, kW>& Lm, Win<R, kW>& DL)
 {
 #pragma unroll
@@ -87,8 +98,9 @@ __device__ inline void stage8(const cpx<R>* s, double* slot, int lane)
 }  // namespace
 
 template <typename R, int MODE>
-__global__ __launch_bounds__(kT) void sweep_win_kernel(WinSweepArgs a)
+__global__ __launch_bounds__(WinShape<R>::T) void sweep_win_kernel(WinSweepArgs a)
 {
+    constexpr int kT = WinShape<R>::T, kW = WinShape<R>::W, kNW = kT / 64;
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr int NV = MODE == 2 ? 32 : 16;
     constexpr bool ONE_ROUND = sizeof(R) == 4;  // all four FP32 states fit the re-layout buffer
@@ -195,7 +207,7 @@ __global__ __launch_bounds__(kT) void sweep_win_kernel(WinSweepArgs a)
                         cpx<R> ga[8];
 #pragma unroll
                         for (int e = 0; e < 8; e++) ga[e] = {R(0), R(0)};
-#define MG_PA(WW, A, B) pass_a<R, A, B, MODE>(V, Z, Lm, P, D, ga)
+#define MG_PA(WW, A, B) pass_a<R, WW, A, B, MODE>(V, Z, Lm, P, D, ga)
                         MG_WCASES(kW, MG_PA)
 #undef MG_PA
                         stage8(ga, slot, lane);
@@ -204,12 +216,12 @@ __global__ __launch_bounds__(kT) void sweep_win_kernel(WinSweepArgs a)
                         cpx<R> ha[8];
 #pragma unroll
                         for (int e = 0; e < 8; e++) ha[e] = {R(0), R(0)};
-#define MG_PB(WW, A, B) pass_b<R, A, B>(Lm, P, DL, D, ha)
+#define MG_PB(WW, A, B) pass_b<R, WW, A, B>(Lm, P, DL, D, ha)
                         MG_WCASES(kW, MG_PB)
 #undef MG_PB
                         stage8(ha, slot + 16, lane);
                     }
-#define MG_PC(WW, A, B) pass_c<R, A, B, MODE>(V, Z, Lm, DL)
+#define MG_PC(WW, A, B) pass_c<R, WW, A, B, MODE>(V, Z, Lm, DL)
                     MG_WCASES(kW, MG_PC)
 #undef MG_PC
                 }
@@ -275,7 +287,8 @@ __global__ __launch_bounds__(kT) void sweep_win_kernel(WinSweepArgs a)
 
 size_t win_sweep_lds_bytes(int L, int P, int fp32, size_t* acc_off, size_t* buf_off)
 {
-    const size_t fixed = sizeof(double) * ((size_t)kWinRegGates * kNW * 32 + 2 * kNW);
+    const int nw = (fp32 ? WinShape<float>::T : WinShape<double>::T) / 64;
+    const size_t fixed = sizeof(double) * ((size_t)kWinRegGates * nw * 32 + 2 * nw);
     const size_t a0 = (fixed + 255) & ~(size_t)255;
     const size_t b0 = (a0 + sizeof(double) * 32 * (size_t)P + 255) & ~(size_t)255;
     if (acc_off) *acc_off = a0;
@@ -284,7 +297,13 @@ size_t win_sweep_lds_bytes(int L, int P, int fp32, size_t* acc_off, size_t* buf_
     return b0 + (fp32 ? 4 * sizeof(cpx<float>) : 2 * sizeof(cpx<double>)) * ((size_t)1 << L);
 }
 
-bool win_sweep_fits(int L) { return ((size_t)1 << L) == (size_t)kT << kW; }
+bool win_sweep_fits(int L, int fp32)
+{
+    return fp32 ? ((size_t)1 << L) == ((size_t)WinShape<float>::T << WinShape<float>::W)
+                : ((size_t)1 << L) == ((size_t)WinShape<double>::T << WinShape<double>::W);
+}
+
+int win_sweep_window_bits(int fp32) { return fp32 ? WinShape<float>::W : WinShape<double>::W; }
 
 template <typename R>
 static hipError_t launch_win_r(const WinSweepArgs& a, int mode, int grid, size_t lds, hipStream_t s)
@@ -295,9 +314,10 @@ static hipError_t launch_win_r(const WinSweepArgs& a, int mode, int grid, size_t
         hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
-    if (mode == 2) hipLaunchKernelGGL((sweep_win_kernel<R, 2>), dim3(grid), dim3(kT), lds, s, a);
-    else if (mode == 1) hipLaunchKernelGGL((sweep_win_kernel<R, 1>), dim3(grid), dim3(kT), lds, s, a);
-    else hipLaunchKernelGGL((sweep_win_kernel<R, 0>), dim3(grid), dim3(kT), lds, s, a);
+    constexpr int T = WinShape<R>::T;
+    if (mode == 2) hipLaunchKernelGGL((sweep_win_kernel<R, 2>), dim3(grid), dim3(T), lds, s, a);
+    else if (mode == 1) hipLaunchKernelGGL((sweep_win_kernel<R, 1>), dim3(grid), dim3(T), lds, s, a);
+    else hipLaunchKernelGGL((sweep_win_kernel<R, 0>), dim3(grid), dim3(T), lds, s, a);
     return hipGetLastError();
 }
 

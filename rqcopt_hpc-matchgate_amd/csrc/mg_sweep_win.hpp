@@ -5,7 +5,7 @@
 
 namespace mg {
 
-constexpr int kWinRegGates = 4;  // gates per window block at most (staging slots)
+constexpr int kWinRegGates = 6;  // gates per window block at most (staging slots)
 
 struct WinSweepArgs {
     int L, G, P, nblocks;
@@ -21,8 +21,9 @@ struct WinSweepArgs {
 
 // dynamic LDS of the kernel (and the accumulator and buffer offsets)
 size_t win_sweep_lds_bytes(int L, int P, int fp32, size_t* acc_off, size_t* buf_off);
-// 512 threads x 8 amplitudes cover the column
-bool win_sweep_fits(int L);
+// the kernel's threads x window amplitudes cover the column (L=12 in both precisions)
+bool win_sweep_fits(int L, int fp32);
+int win_sweep_window_bits(int fp32);
 hipError_t launch_sweep_win(const WinSweepArgs& a, int mode, int fp32, int grid, size_t lds, hipStream_t s);
 
 }  // namespace mg
