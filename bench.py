@@ -186,6 +186,21 @@ def large_bench(mg):
                      "kernel": "batch_kernel<%s, %d, %d>" % ((("float",) if prec == mg.MG_FP32 else ("double",)) +
                                                               mg.batch_get_kernel(prec))}
         log(f"  {dt:.2f} s, kernel {km:.1f} ms, {out[name]['tflops']:.2f} TFLOP/s")
+    log("materialising the target in HBM (FP64 rows, then an FP32 copy) ...")
+    t0 = time.perf_counter()
+    tgt.densify()
+    out["densify_s"] = time.perf_counter() - t0
+    fl_dense = sweep_flops(w["L"], w["G"], 0)
+    for name, prec in (("fp32", mg.MG_FP32), ("fp64", mg.MG_FP64)):
+        req = {"target": tgt, "program": prog, "gates": w["gates"], "v": v, "what": mg.EVAL_F_GRAD_HVP, "precision": prec}
+        t0 = time.perf_counter()
+        r = mg.evaluate_batch([req])[0]
+        dt = time.perf_counter() - t0
+        km = mg.batch_last_kernel_ms()
+        out[name + "_dense_target"] = {"s": dt, "kernel_ms": km, "tflops": fl_dense / (km * 1e-3) / 1e12,
+                                       "rel_err_vs_in_kernel_target": float(np.max(np.abs(r[1] - res[name][1])) /
+                                                                            np.max(np.abs(res[name][1])))}
+        log(f"  dense target, {name}: {dt:.2f} s, kernel {km:.1f} ms")
     f32, g32, h32 = res["fp32"]
     f64, g64, h64 = res["fp64"]
     out["rel_err_fp32_vs_fp64"] = {"f": float(abs(f32 - f64) / abs(f64)),

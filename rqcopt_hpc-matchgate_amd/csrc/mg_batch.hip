@@ -312,6 +312,36 @@ __global__ __launch_bounds__(T) void materialize_kernel(int L, const Inst* tinst
     }
 }
 
+// the same for columns too large for LDS: the state is the seed row itself (global memory,
+// plain offsets), one column per workgroup
+template <int T, int W>
+__global__ __launch_bounds__(T) void materialize_global_kernel(int L, const Inst* tinst, const cpx<double>* tcoef,
+                                                               const WBlock* tblocks, int ntblocks, cpx<double>* seed)
+{
+    const uint32_t S = 1u << L, ntup = S >> W;
+    const int tl = threadIdx.x;
+    for (uint32_t b = blockIdx.x; b < S; b += gridDim.x) {
+        cpx<double>* row = seed + (size_t)b * S;
+        for (uint32_t i = tl; i < S; i += T) row[i] = {double(i == b), 0.0};
+        __syncthreads();
+        win_apply_program<double, W, false>(row, tblocks, ntblocks, tinst, tcoef, ntup, tl, T, true);
+        for (uint32_t i = tl; i < S; i += T) {
+            const cpx<double> v = row[i];
+            row[i] = {-v.re, v.im};
+        }
+        __syncthreads();
+    }
+}
+
+hipError_t launch_materialize_global(int L, const Inst* tinst, const cpx<double>* tcoef, const WBlock* tblocks,
+                                     int ntblocks, cpx<double>* seed, int grid, hipStream_t s)
+{
+    constexpr int T = 256, W = 3;
+    hipLaunchKernelGGL((materialize_global_kernel<T, W>), dim3(grid), dim3(T), 0, s, L, tinst, tcoef, tblocks, ntblocks,
+                       seed);
+    return hipGetLastError();
+}
+
 hipError_t launch_materialize(int L, const Inst* tinst, const cpx<double>* tcoef, const WBlock* tblocks, int ntblocks,
                               cpx<double>* seed, int grid, size_t lds, hipStream_t s)
 {

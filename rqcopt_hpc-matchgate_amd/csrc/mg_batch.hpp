@@ -66,6 +66,10 @@ hipError_t launch_batch(const BatchInst<R>* insts, const BatchItem* items, int n
 hipError_t launch_materialize(int L, const Inst* tinst, const cpx<double>* tcoef, const WBlock* tblocks, int ntblocks,
                               cpx<double>* seed, int grid, size_t lds, hipStream_t s);
 
+// the same for 2^L columns whose states do not fit LDS (in place in the seed rows; plain offsets)
+hipError_t launch_materialize_global(int L, const Inst* tinst, const cpx<double>* tcoef, const WBlock* tblocks,
+                                     int ntblocks, cpx<double>* seed, int grid, hipStream_t s);
+
 hipError_t launch_batch_reduce(const ReduceSeg* segs, const int2* blocks, int nblocks, const double* partial, double* out,
                                hipStream_t s);
 

@@ -639,14 +639,22 @@ int target_densify(Target& t)
     MG_HIP(db.ensure(w.blk_lds.size() * sizeof(WBlock)));
     MG_HIP(hipMemcpy(di.p, w.inst.data(), w.inst.size() * sizeof(Inst), hipMemcpyHostToDevice));
     MG_HIP(hipMemcpy(dc.p, w.coef.data(), w.coef.size() * sizeof(cpx<double>), hipMemcpyHostToDevice));
-    MG_HIP(hipMemcpy(db.p, w.blk_lds.data(), w.blk_lds.size() * sizeof(WBlock), hipMemcpyHostToDevice));
     const long cgw = std::min<long>(256, std::max<long>(1, (long)(S >> 3)));
     const size_t lds = (size_t)(256 / cgw) * S * sizeof(cpx<double>);
-    if (lds > c->lds_limit - 256) { set_error("target too large to materialise"); return -1; }
+    const bool in_lds = lds <= c->lds_limit - 256;
+    // LDS-resident columns use the swizzled block offsets, in-place rows the plain ones
+    const std::vector<WBlock>& bl = in_lds ? w.blk_lds : w.blk_glob;
+    MG_HIP(hipMemcpy(db.p, bl.data(), bl.size() * sizeof(WBlock), hipMemcpyHostToDevice));
     if (t.seed64.ensure(S * S * sizeof(cpx<double>)) != hipSuccess) { set_error("out of device memory for the target"); return -1; }
-    const int grid = (int)std::max<long>(1, std::min<long>((long)(S / (256 / cgw)), (long)c->ncu * 2));
-    MG_HIP(launch_materialize(L, (const Inst*)di.p, (const cpx<double>*)dc.p, (const WBlock*)db.p, (int)w.blk_lds.size(),
-                              (cpx<double>*)t.seed64.p, grid, lds, c->s));
+    if (in_lds) {
+        const int grid = (int)std::max<long>(1, std::min<long>((long)(S / (256 / cgw)), (long)c->ncu * 2));
+        MG_HIP(launch_materialize(L, (const Inst*)di.p, (const cpx<double>*)dc.p, (const WBlock*)db.p, (int)bl.size(),
+                                  (cpx<double>*)t.seed64.p, grid, lds, c->s));
+    } else {
+        const int grid = (int)std::min<long>((long)S, (long)c->ncu * 4);
+        MG_HIP(launch_materialize_global(L, (const Inst*)di.p, (const cpx<double>*)dc.p, (const WBlock*)db.p,
+                                         (int)bl.size(), (cpx<double>*)t.seed64.p, grid, c->s));
+    }
     MG_HIP(hipStreamSynchronize(c->s));
     t.kind = 0;
     t.seed32_ready = false;

@@ -232,6 +232,14 @@ def test_large_instances_column_subset_match_oracle(mg, L):
         f, g, h = mg.evaluate_batch([{"target": tgt, "program": prog, "gates": V, "v": v, "what": mg.EVAL_F_GRAD_HVP,
                                       "precision": prec, "b0": b0, "b1": b1}])[0]
         assert abs(f - fo) <= tol * abs(fo) and rel(g, go) < tol and rel(h, ho) < tol, (prec, rel(g, go), rel(h, ho))
+    if L == 16:
+        say("case: the L=16 target materialised in HBM (2^32 complex rows, in place)")
+        tgt.densify()
+        for prec, tol in ((mg.MG_FP64, 1e-11), (mg.MG_FP32, 2e-5)):
+            f, g, h = mg.evaluate_batch([{"target": tgt, "program": prog, "gates": V, "v": v,
+                                          "what": mg.EVAL_F_GRAD_HVP, "precision": prec, "b0": b0, "b1": b1}])[0]
+            assert abs(f - fo) <= tol * abs(fo) and rel(g, go) < tol and rel(h, ho) < tol, (prec, rel(g, go))
+        del tgt
 
 
 def test_batch_routes_dense_L12_to_single_request_kernels(mg):
