@@ -32,11 +32,11 @@ template <typename R>
 struct WinShape;
 template <>
 struct WinShape<double> {
-    static constexpr int T = 512, W = 3, NST = 2;
+    static constexpr int T = 512, W = 3;
 };
 template <>
 struct WinShape<float> {
-    static constexpr int T = 256, W = 4, NST = 4;
+    static constexpr int T = 256, W = 4;
 };
 
 // (A) uncompute psi (and dpsi), gradient outer products
