@@ -147,7 +147,7 @@ def served_bench(mg, njobs, seed=1):
     jps, jpsb, jpsp = njobs / st["wall_s"], njobs / stb["wall_s"], njobs / stp["wall_s"]
     return {"jobs": njobs, "jobs_per_s": jps, "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
             "launches": st["rounds"], "evaluations": st["requests"], "peak_admitted": st["max_active"],
-            "kernel_s": st["kernel_ms"] * 1e-3, "wall_s": st["wall_s"],
+            "kernel_s": st["kernel_ms"] * 1e-3, "wall_s": st["wall_s"], "graph_replays": st["graph_replays"],
             "baseline_fp64_one_job_plain": {"jobs_per_s": jpsp, "p50_ms": float(np.percentile(latp, 50)),
                                             "p99_ms": float(np.percentile(latp, 99)),
                                             "note": "circuit targets applied per column, one launch per evaluation"},

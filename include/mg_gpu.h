@@ -121,8 +121,16 @@ void mg_batch_set_grid_per_cu(int n);
  * of two) and basis columns per work item; 0 restores the default */
 void mg_batch_set_shape(int precision, int nqubits, int threads_per_column, int columns_per_item);
 void mg_batch_get_shape(int precision, int nqubits, int* threads_per_column, int* columns_per_item);
-/* duration of the last mg_evaluate_batch kernels in ms (needs mg_set_kernel_timing(1)) */
+/* duration of the last mg_evaluate_batch kernels in ms (needs mg_set_kernel_timing(1); a launch
+ * replayed from a graph is timed whole, upload and download included) */
 float mg_batch_last_kernel_ms(void);
+/* HIP graphs for batched launches (default on): a launch whose arguments, buffers and sizes
+ * repeat an earlier launch is captured once (upload, persistent kernel, reduction, download) and
+ * replayed from then on; the request descriptors travel in the upload, so one graph serves every
+ * trust-region step of a fixed set of requests.  Results are identical with and without graphs. */
+void mg_batch_set_graphs(int on);
+/* replayed and newly captured launches of the last mg_evaluate_batch call */
+void mg_batch_last_graph_stats(int* replays, int* captures);
 /* autotuner: times every compiled (threads, window bits) variant with 1 and 2 workgroups per
  * CU on the given calibration requests (their precision field is ignored; `precision` is
  * tuned), reps timed launches each, and keeps the fastest for that precision.  Outputs
@@ -175,6 +183,7 @@ typedef struct mg_serve_stats {
     double peak_footprint; /* peak device bytes held by admitted jobs */
     double eval_s;         /* wall time inside evaluations (host preparation, launches, waits) */
     double prep_s;         /* host time preparing batched launches */
+    long graph_replays;    /* batched launches replayed from captured HIP graphs */
 } mg_serve_stats;
 
 int mg_serve(mg_job* jobs, int njobs, const mg_serve_options* opt, mg_serve_stats* stats);

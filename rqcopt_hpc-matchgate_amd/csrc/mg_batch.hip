@@ -358,6 +358,13 @@ hipError_t launch_materialize(int L, const Inst* tinst, const cpx<double>* tcoef
 
 __global__ void batch_reset(int* counter) { *counter = 0; }
 
+// the dynamic-LDS attribute is set before a capture, not while one is recorded
+static bool hip_stream_capturing(hipStream_t s)
+{
+    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+    return hipStreamIsCapturing(s, &st) == hipSuccess && st == hipStreamCaptureStatusActive;
+}
+
 // out[seg.dst + k] = sum over items j of partial[seg.src + j * seg.stride + k], in j order
 // (thread row ty sums j = ty, ty + 16, ...; row 0 adds the 16 row sums in order)
 __global__ __launch_bounds__(1024) void batch_reduce(const ReduceSeg* __restrict__ segs, const int2* __restrict__ blocks,
@@ -386,7 +393,10 @@ static hipError_t launch_bt(const BatchInst<R>* insts, const BatchItem* items, i
                             hipStream_t s)
 {
     auto k = batch_kernel<R, T, W>;
-    if (lds > 65536) {
+    // a negative grid only sets the kernel's LDS attribute (before a stream capture)
+    if (grid < 0) return lds > 65536 ? hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds)
+                                     : hipSuccess;
+    if (lds > 65536 && !hip_stream_capturing(s)) {
         hipError_t e = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }

@@ -53,7 +53,9 @@ inline size_t batch_lds_fixed_bytes(int T)
     return (b + 255) & ~(size_t)255;
 }
 
-// compiled (threads, window bits) variants: FP32 (256, 2..4) and (512, 2..3); FP64 (256, 2..3) and (512, 2)
+// compiled (threads, window bits) variants: FP32 (256, 2..4) and (512, 2..3); FP64 (256, 2..3) and (512, 2).
+// launch_batch with grid < 0 only sets the kernel's dynamic-LDS attribute (done before a stream
+// capture; during a capture the launch itself does not set it).
 bool batch_variant_exists(int precision_fp32, int threads, int wbits);
 
 template <typename R>

@@ -2,6 +2,7 @@
 #include "mg_runtime.hpp"
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -70,6 +71,54 @@ struct Context {
     size_t b_up_hn = 0;
     void* b_out_h = nullptr;  // pinned staging of the outputs
     size_t b_out_hn = 0;
+
+    // captured batched launches, least recently used evicted; keys launched once (eagerly)
+    using GraphKey = std::array<long long, 21>;
+    struct Graph {
+        GraphKey key;
+        hipGraphExec_t exec;
+        long long used;
+    };
+    static constexpr size_t kGraphs = 16, kSeen = 64;
+    std::vector<Graph> graphs;
+    std::vector<GraphKey> seen;
+    long long graph_tick = 0;
+
+    hipGraphExec_t graph_find(const GraphKey& k)
+    {
+        for (Graph& g : graphs)
+            if (g.key == k) {
+                g.used = ++graph_tick;
+                return g.exec;
+            }
+        return nullptr;
+    }
+    // true if k was launched before (and forgets it); otherwise remembers it
+    bool graph_seen(const GraphKey& k)
+    {
+        auto it = std::find(seen.begin(), seen.end(), k);
+        if (it != seen.end()) {
+            seen.erase(it);
+            return true;
+        }
+        if (seen.size() == kSeen) seen.erase(seen.begin());
+        seen.push_back(k);
+        return false;
+    }
+    void graph_insert(const GraphKey& k, hipGraphExec_t e)
+    {
+        if (graphs.size() == kGraphs) {
+            auto lru = std::min_element(graphs.begin(), graphs.end(),
+                                        [](const Graph& a, const Graph& b) { return a.used < b.used; });
+            (void)hipGraphExecDestroy(lru->exec);
+            graphs.erase(lru);
+        }
+        graphs.push_back({k, e, ++graph_tick});
+    }
+    ~Context()
+    {
+        for (Graph& g : graphs) (void)hipGraphExecDestroy(g.exec);
+    }
 };
 
 static Context* ctx()
@@ -825,15 +874,69 @@ static int run_batch(Context* c, const std::vector<int>& idx, BatchReq* reqs, Ba
     }
 
     const double prep_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
-    MG_HIP(hipMemcpyAsync(c->b_up.p, c->b_up_h, up_bytes, hipMemcpyHostToDevice, s));
-    if (kn.time_kernels) MG_HIP(hipEventRecord(c->ev0, s));
-    MG_HIP(launch_batch<R>((const BatchInst<R>*)(d + o_insts), (const BatchItem*)(d + o_items), nitems, (int*)c->b_counter.p,
-                           (double*)c->b_partial.p, scratch_stride ? (cpx<R>*)c->b_scratch.p : nullptr,
-                           (long long)scratch_stride, T, W, grid, lds, s));
-    if (kn.time_kernels) MG_HIP(hipEventRecord(c->ev1, s));
-    MG_HIP(launch_batch_reduce((const ReduceSeg*)(d + o_segs), (const int2*)(d + o_rblk), (int)blocks.size(),
-                               (const double*)c->b_partial.p, (double*)c->b_out.p, s));
-    MG_HIP(hipMemcpyAsync(c->b_out_h, c->b_out.p, (size_t)ooff * sizeof(double), hipMemcpyDeviceToHost, s));
+    // upload, counter reset + persistent kernel, reduction, download; `events` brackets the kernel
+    auto enqueue = [&](bool events) -> hipError_t {
+        hipError_t e = hipMemcpyAsync(c->b_up.p, c->b_up_h, up_bytes, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess && events) e = hipEventRecord(c->ev0, s);
+        if (e == hipSuccess)
+            e = launch_batch<R>((const BatchInst<R>*)(d + o_insts), (const BatchItem*)(d + o_items), nitems,
+                                (int*)c->b_counter.p, (double*)c->b_partial.p,
+                                scratch_stride ? (cpx<R>*)c->b_scratch.p : nullptr, (long long)scratch_stride, T, W, grid,
+                                lds, s);
+        if (e == hipSuccess && events) e = hipEventRecord(c->ev1, s);
+        if (e == hipSuccess)
+            e = launch_batch_reduce((const ReduceSeg*)(d + o_segs), (const int2*)(d + o_rblk), (int)blocks.size(),
+                                    (const double*)c->b_partial.p, (double*)c->b_out.p, s);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(c->b_out_h, c->b_out.p, (size_t)ooff * sizeof(double), hipMemcpyDeviceToHost, s);
+        return e;
+    };
+    // Every node argument of the sequence (sizes, offsets, buffer and staging addresses) is in the
+    // key; the request descriptors travel in the upload.  A key launched a second time is captured
+    // once and replayed from then on.
+    hipGraphExec_t graph = nullptr;
+    bool captured = false;
+    if (batch_tuning().graphs) {
+        const Context::GraphKey key{prec, T, W, nitems, grid, (long long)lds, (long long)up_bytes, ooff,
+                                    (long long)blocks.size(), (long long)scratch_stride, (long long)o_insts,
+                                    (long long)o_items, (long long)o_segs, (long long)o_rblk, (long long)(uintptr_t)c->b_up.p,
+                                    (long long)(uintptr_t)c->b_up_h, (long long)(uintptr_t)c->b_partial.p,
+                                    (long long)(uintptr_t)c->b_out.p, (long long)(uintptr_t)c->b_out_h,
+                                    (long long)(uintptr_t)c->b_counter.p, (long long)(uintptr_t)c->b_scratch.p};
+        graph = c->graph_find(key);
+        if (!graph && c->graph_seen(key)) {
+            // the dynamic-LDS attribute is a function attribute, set outside the capture
+            MG_HIP(launch_batch<R>(nullptr, nullptr, 0, nullptr, nullptr, nullptr, 0, T, W, -1, lds, s));
+            MG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            const hipError_t e = enqueue(false);
+            hipGraph_t g = nullptr;
+            const hipError_t ee = hipStreamEndCapture(s, &g);
+            if (e != hipSuccess || ee != hipSuccess) {
+                if (g) (void)hipGraphDestroy(g);
+                set_error(std::string("batched evaluation: graph capture failed: ") +
+                          hipGetErrorString(e != hipSuccess ? e : ee));
+                return -1;
+            }
+            const hipError_t ei = hipGraphInstantiate(&graph, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (ei != hipSuccess) {
+                set_error(std::string("batched evaluation: graph instantiation failed: ") + hipGetErrorString(ei));
+                return -1;
+            }
+            c->graph_insert(key, graph);
+            captured = true;
+        }
+    }
+    if (graph) {
+        // the kernel's dynamic-LDS limit as at capture (another launch may have changed it)
+        if (!captured) MG_HIP(launch_batch<R>(nullptr, nullptr, 0, nullptr, nullptr, nullptr, 0, T, W, -1, lds, s));
+        // timing brackets the whole graph (upload and download included)
+        if (kn.time_kernels) MG_HIP(hipEventRecord(c->ev0, s));
+        MG_HIP(hipGraphLaunch(graph, s));
+        if (kn.time_kernels) MG_HIP(hipEventRecord(c->ev1, s));
+    } else {
+        MG_HIP(enqueue(kn.time_kernels));
+    }
     MG_HIP(hipStreamSynchronize(s));
 
     const double* o = (const double*)c->b_out_h;
@@ -857,6 +960,8 @@ static int run_batch(Context* c, const std::vector<int>& idx, BatchReq* reqs, Ba
         st->grid = grid;
         st->lds_bytes = std::max(st->lds_bytes, lds);
         st->prep_ms += prep_ms;
+        st->graph_replays += graph ? 1 : 0;
+        st->graph_captures += captured ? 1 : 0;
         float ms = 0.f;
         if (kn.time_kernels) MG_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
         st->kernel_ms += ms;

@@ -125,6 +125,8 @@ struct BatchStats {
     size_t lds_bytes = 0;
     float kernel_ms = 0.f;
     double prep_ms = 0.0;  // host time preparing the launches (instances, window blocks, upload staging)
+    int graph_replays = 0;   // launches replayed from a captured graph
+    int graph_captures = 0;  // graphs captured
 };
 
 // Per (precision, L) shape of the batched kernel: column-group width cgw (threads per
@@ -141,6 +143,8 @@ struct BatchTuning {
     // they run on the single-request kernels (register gate-window / LDS sweeps) in their own
     // launches instead of inside the persistent batch (0 / 0: never)
     int single_min = 12, single_max = 12;
+    // replay repeated launches (same arguments, buffers and sizes) from captured HIP graphs
+    bool graphs = true;
 };
 BatchTuning& batch_tuning();
 void batch_shape(int L, int precision, int* cgw, int* cpi);

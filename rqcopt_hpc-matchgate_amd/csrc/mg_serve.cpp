@@ -75,6 +75,7 @@ struct Server {
     bool batched = true;
     long rounds = 0, requests = 0;
     double kernel_ms = 0.0, eval_s = 0.0, prep_ms = 0.0;
+    long graph_replays = 0;
 };
 
 int job_precision(const JobRt* j)
@@ -308,6 +309,14 @@ void mg_batch_get_shape(int precision, int nqubits, int* threads_per_column, int
 
 float mg_batch_last_kernel_ms(void) { return mg::last_batch_stats().kernel_ms; }
 
+void mg_batch_set_graphs(int on) { mg::batch_tuning().graphs = on != 0; }
+
+void mg_batch_last_graph_stats(int* replays, int* captures)
+{
+    if (replays) *replays = mg::last_batch_stats().graph_replays;
+    if (captures) *captures = mg::last_batch_stats().graph_captures;
+}
+
 int mg_batch_autotune(mg_request* reqs, int n, int precision, int reps, int* threads, int* window_bits, int* per_cu,
                       float* kernel_ms)
 {
@@ -336,6 +345,7 @@ int mg_batch_autotune(mg_request* reqs, int n, int precision, int reps, int* thr
     const mg::BatchTuning saved = bt;
     const bool timing = mg::knobs().time_kernels;
     mg::knobs().time_kernels = true;
+    bt.graphs = false;  // time the kernels themselves, not graph replays
     float best = 1e30f;
     int bT = saved.threads[pi], bW = saved.wbits[pi], bP = saved.grid_per_cu;
     for (int T : {256, 512})
@@ -497,6 +507,7 @@ int mg_serve(mg_job* jobs, int njobs, const mg_serve_options* opt, mg_serve_stat
                 srv.kernel_ms += st.kernel_ms;
                 srv.eval_s += dt;
                 srv.prep_ms += st.prep_ms;
+                srv.graph_replays += st.graph_replays;
                 for (Posted* p : batch) {
                     p->status = r;
                     p->done = true;
@@ -526,6 +537,7 @@ int mg_serve(mg_job* jobs, int njobs, const mg_serve_options* opt, mg_serve_stat
         stats->peak_footprint = (double)peak_held;
         stats->eval_s = srv.eval_s;
         stats->prep_s = srv.prep_ms * 1e-3;
+        stats->graph_replays = srv.graph_replays;
     }
     return rc;
 }

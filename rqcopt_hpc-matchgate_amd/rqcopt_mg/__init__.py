@@ -79,7 +79,8 @@ class ServeOptions(ctypes.Structure):
 class ServeStats(ctypes.Structure):
     """struct mg_serve_stats (include/mg_gpu.h)"""
     _fields_ = [("wall_s", _D), ("rounds", ctypes.c_long), ("requests", ctypes.c_long), ("kernel_ms", _D),
-                ("max_active", ctypes.c_int), ("peak_footprint", _D), ("eval_s", _D), ("prep_s", _D)]
+                ("max_active", ctypes.c_int), ("peak_footprint", _D), ("eval_s", _D), ("prep_s", _D),
+                ("graph_replays", ctypes.c_long)]
 
 
 def rtr_default_params(n):
@@ -132,6 +133,8 @@ _sig("mg_batch_set_grid_per_cu", [_I], None)
 _sig("mg_batch_set_shape", [_I, _I, _I, _I], None)
 _sig("mg_batch_get_shape", [_I, _I, _P, _P], None)
 _sig("mg_batch_last_kernel_ms", [], ctypes.c_float)
+_sig("mg_batch_set_graphs", [ctypes.c_int], None)
+_sig("mg_batch_last_graph_stats", [_P, _P], None)
 _sig("mg_batch_autotune", [ctypes.POINTER(Request), _I, _I, _I, _P, _P, _P, _P])
 _sig("mg_serve", [ctypes.POINTER(Job), _I, ctypes.POINTER(ServeOptions), ctypes.POINTER(ServeStats)])
 # reference entry points (include/matchgate_target.h, mg_brickwall_opt.h)
@@ -421,6 +424,18 @@ def batch_last_kernel_ms():
     return float(lib.mg_batch_last_kernel_ms())
 
 
+def batch_set_graphs(on):
+    """replay repeated batched launches from captured HIP graphs (default on)"""
+    lib.mg_batch_set_graphs(int(bool(on)))
+
+
+def batch_last_graph_stats():
+    """(replayed, newly captured) launches of the last evaluate_batch call"""
+    r, c = ctypes.c_int(0), ctypes.c_int(0)
+    lib.mg_batch_last_graph_stats(ctypes.byref(r), ctypes.byref(c))
+    return r.value, c.value
+
+
 def serve(jobs, batched=True, precision=MG_FP32, precision_switch=3, max_active=0, hbm_budget=0.0,
           realtime=True, densify_qubits=0):
     """Run a stream of trust-region optimisation jobs (mg_serve).
@@ -540,3 +555,82 @@ def statevector_array(data):
     sa = StatevectorArray(data.ctypes.data, int(np.log2(data.shape[0])), data.shape[1])
     sa._keep = data
     return sa
+
+
+# ---- HDF5 files of the reference drivers (include/mg_hdf5.h, util.h) -------------------------
+_H5T = {np.dtype(np.float64): 1, np.dtype(np.float32): 2, np.dtype(np.int32): 3, np.dtype(np.int64): 4,
+        np.dtype(np.uint8): 5, np.dtype(np.int8): 6, np.dtype(np.int16): 7, np.dtype(np.uint32): 8,
+        np.dtype(np.uint64): 9}
+_H5T_BASE = 0x4d47485400000000
+_HID = ctypes.c_int64
+_sig("mg_h5_fopen", [ctypes.c_char_p, ctypes.c_uint], _HID)
+_sig("mg_h5_fcreate", [ctypes.c_char_p, ctypes.c_uint], _HID)
+_sig("mg_h5_fclose", [_HID])
+_sig("get_hdf5_dataset_ndims", [_HID, ctypes.c_char_p, _P])
+_sig("get_hdf5_dataset_dims", [_HID, ctypes.c_char_p, _P])
+_sig("read_hdf5_dataset", [_HID, ctypes.c_char_p, _HID, _P])
+_sig("read_hdf5_attribute", [_HID, ctypes.c_char_p, _HID, _P])
+_sig("write_hdf5_dataset", [_HID, ctypes.c_char_p, ctypes.c_int, _P, _HID, _HID, _P])
+_sig("write_hdf5_scalar_attribute", [_HID, ctypes.c_char_p, _HID, _HID, _P])
+
+
+class H5Error(RuntimeError):
+    pass
+
+
+def _h5t(dtype):
+    return _HID(_H5T_BASE + _H5T[np.dtype(dtype)])
+
+
+def h5_read(path, datasets=(), attributes=(), dtype=np.float64, attr_dtype=None):
+    """Read named datasets and root attributes of an HDF5 file with the library's reader
+    (the reference's read_hdf5_dataset / read_hdf5_attribute, src/util.c:115-161).
+    Returns (dict name -> ndarray of `dtype`, dict name -> scalar of `attr_dtype` or `dtype`)."""
+    f = lib.mg_h5_fopen(os.fsencode(path), 0)
+    if f < 0:
+        raise H5Error(f"cannot open {path}")
+    try:
+        out, att = {}, {}
+        for name in datasets:
+            nd = ctypes.c_int(0)
+            if lib.get_hdf5_dataset_ndims(f, name.encode(), ctypes.byref(nd)) != 0:
+                raise H5Error(f"{path}: no dataset {name}")
+            dims = (ctypes.c_ulonglong * max(1, nd.value))()
+            lib.get_hdf5_dataset_dims(f, name.encode(), dims)
+            a = np.zeros(tuple(dims[i] for i in range(nd.value)), dtype)
+            if lib.read_hdf5_dataset(f, name.encode(), _h5t(dtype), _p(a)) != 0:
+                raise H5Error(f"{path}: reading {name} failed")
+            out[name] = a
+        for name in attributes:
+            a = np.zeros(1, attr_dtype or dtype)
+            if lib.read_hdf5_attribute(f, name.encode(), _h5t(a.dtype), _p(a)) != 0:
+                raise H5Error(f"{path}: reading attribute {name} failed")
+            att[name] = a[0]
+        return out, att
+    finally:
+        lib.mg_h5_fclose(f)
+
+
+def h5_write(path, datasets=None, attributes=None, store=None):
+    """Write an HDF5 file with the library's writer (the reference's write_hdf5_dataset /
+    write_hdf5_scalar_attribute, src/util.c:168-236).  datasets / attributes: dict name ->
+    array / scalar (numpy dtypes); `store` maps a name to its file type (default: its own)."""
+    store = store or {}
+    f = lib.mg_h5_fcreate(os.fsencode(path), 2)
+    if f < 0:
+        raise H5Error(f"cannot create {path}")
+    ok = True
+    try:
+        for name, a in (datasets or {}).items():
+            a = np.ascontiguousarray(a)
+            dims = (ctypes.c_ulonglong * max(1, a.ndim))(*a.shape)
+            ok &= lib.write_hdf5_dataset(f, name.encode(), a.ndim, dims, _h5t(store.get(name, a.dtype)), _h5t(a.dtype),
+                                         _p(a)) == 0
+        for name, v in (attributes or {}).items():
+            a = np.asarray([v])
+            ok &= lib.write_hdf5_scalar_attribute(f, name.encode(), _h5t(store.get(name, a.dtype)), _h5t(a.dtype),
+                                                  _p(a)) == 0
+    finally:
+        ok &= lib.mg_h5_fclose(f) == 0
+    if not ok:
+        raise H5Error(f"writing {path} failed")

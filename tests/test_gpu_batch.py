@@ -270,3 +270,40 @@ def test_batch_routes_dense_L12_to_single_request_kernels(mg):
             mg.batch_set_single_range(12, 12)
         assert abs(o2[0][0] - single[0]) <= tol * abs(single[0])
         assert rel(o2[0][1], single[1]) < tol and rel(o2[0][2], single[2]) < tol
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_batch_graph_replays_equal_eager(mg, prec):
+    """A repeated batched launch is captured into a HIP graph and replayed: every replay,
+    with new gates and directions each time and single-request launches in between, gives
+    the bits of the eager launch of the same requests."""
+    from rqcopt_mg.workloads import haar_matchgates
+
+    rng = np.random.default_rng(21)
+    P = mg.MG_FP64 if prec == "fp64" else mg.MG_FP32
+    inst = [make_instance(mg, rng, L, d, dense) for L, d, dense in MIX]
+
+    def step_requests(step):
+        r2 = np.random.default_rng(1000 + step)
+        return [{"target": t, "program": p, "gates": haar_matchgates(r2, j["G"]), "v": r2.standard_normal(8 * j["G"]),
+                 "what": mg.EVAL_F_GRAD_HVP, "precision": P} for j, p, t, v in inst]
+
+    say(f"case: graph replays, {prec}, 5 steps")
+    mg.batch_set_graphs(False)
+    try:
+        eager = [mg.evaluate_batch(step_requests(s)) for s in range(5)]
+    finally:
+        mg.batch_set_graphs(True)
+    replays = 0
+    for s in range(5):
+        got = mg.evaluate_batch(step_requests(s))
+        r, c = mg.batch_last_graph_stats()
+        replays += r
+        say(f"  step {s}: replayed {r}, captured {c}")
+        for k, (a, b) in enumerate(zip(got, eager[s])):
+            for x, y in zip(a, b):
+                assert np.array_equal(np.asarray(x), np.asarray(y)), (s, k)
+        # eager work on the same stream between replays
+        j, p, t, v = inst[1]
+        mg.riemannian_gradient_hvp(t, p, j["gates"], v, P)
+    assert replays >= 3

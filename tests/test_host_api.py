@@ -2,8 +2,10 @@
 returns (before any device work), and the host FP64 matchgate algebra against the
 oracle and the reference's conventions (src/matchgate.c)."""
 import ctypes
+import os
 
 import numpy as np
+import pytest
 
 import oracle as orc
 from golden_io import haar_matchgates
@@ -76,3 +78,35 @@ def test_inverse_of_singular_block_fails_like_the_reference():
     b = np.asarray([2, 1j, 0, 1], np.complex128)
     assert mg.lib.inverse_mat2x2(b.ctypes.data, inv.ctypes.data) == 0
     assert np.max(np.abs(inv.reshape(2, 2) @ b.reshape(2, 2) - np.eye(2))) < 1e-15
+
+
+def test_ctypes_structs_match_the_c_header(tmp_path):
+    """The Python binding's ctypes mirrors of the public structs have the C header's layout
+    (every field offset and the size), compiled here with gcc against include/mg_gpu.h."""
+    import shutil
+    import subprocess
+
+    import rqcopt_mg as mg
+
+    if shutil.which("gcc") is None:
+        pytest.skip("no C compiler")
+    structs = {"mg_request": mg.Request, "mg_job": mg.Job, "mg_serve_options": mg.ServeOptions,
+               "mg_serve_stats": mg.ServeStats}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "mg_gpu.h"', "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in py._fields_:
+            lines.append(f'printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines.append("return 0; }")
+    src = tmp_path / "abi.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "abi"
+    inc = os.path.join(os.path.dirname(os.path.abspath(__file__)), "..", "include")
+    r = subprocess.run(["gcc", "-I", inc, str(src), "-o", str(exe)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60).stdout.split("\n")
+    got = {tuple(l.split()[:2]): int(l.split()[2]) for l in out if l.strip()}
+    for cname, py in structs.items():
+        assert got[(cname, "size")] == ctypes.sizeof(py), cname
+        for fname, _ in py._fields_:
+            assert got[(cname, fname)] == getattr(py, fname).offset, (cname, fname)
