@@ -160,6 +160,44 @@ def served_bench(mg, njobs, seed=1):
                         "circuit targets of 3x depth; FP32 sweeps, last 3 iterations FP64"}
 
 
+def batched_bench(mg, niter, seed=0):
+    """The batched workload: 64 independent instances, L uniform in {8, 10, 12, 14}, odd depth
+    3..11, targets seeded random brickwalls of 3x the depth (materialised in HBM at admission),
+    FP64; every instance runs `niter` trust-region iterations and all of them step together, each
+    round of evaluations one persistent launch per precision over all instances and basis
+    columns, repeated launch shapes replayed from captured HIP graphs."""
+    from rqcopt_mg.workloads import batched_instances
+
+    inst = batched_instances(64, seed=seed, niter=niter)
+    jobs = [{"program": mg.Program.create(j["L"], j["G"], j["param"], j["wires"]), "gates": j["gates"],
+             "niter": j["niter"], "target_gates": j["target_gates"], "target_wires": j["target_wires"],
+             "arrival": 0.0} for j in inst]
+    mg.set_kernel_timing(True)
+    log(f"batched workload: 64 instances, {niter} trust-region iterations each (FP64) ...")
+    res, st = mg.serve(jobs, batched=True, precision=mg.MG_FP64, precision_switch=0, densify_qubits=14,
+                       realtime=False)
+    iters = sum(len(r["f_iter"]) - 1 for r in res)
+    log(f"  {iters / st['wall_s']:.1f} trust-region iterations/s, {st['rounds']} launches, "
+        f"{st['graph_replays']} from graphs")
+    # final gradients of the smaller instances against the FP64 single-request path
+    err = 0.0
+    rng = np.random.default_rng(seed)
+    for j, r in list(zip(jobs, res))[:8]:
+        t = mg.Target.circuit(j["target_gates"], j["target_wires"], j["program"].L)
+        v = rng.standard_normal(8 * j["program"].P)
+        _, g0, _ = mg.riemannian_gradient_hvp(t, j["program"], r["gates"], v, mg.MG_FP64)
+        o = mg.evaluate_batch([{"target": t, "program": j["program"], "gates": r["gates"], "v": v,
+                                "what": mg.EVAL_F_GRAD_HVP, "precision": mg.MG_FP64}])[0]
+        err = max(err, float(np.max(np.abs(o[1] - g0)) / max(np.max(np.abs(g0)), 1e-300)))
+    return {"instances": 64, "iterations_per_instance": niter, "tr_iters_per_s": iters / st["wall_s"],
+            "evaluations_per_s": st["requests"] / st["wall_s"], "launches": st["rounds"],
+            "evaluations": st["requests"], "graph_replays": st["graph_replays"], "kernel_s": st["kernel_ms"] * 1e-3,
+            "wall_s": st["wall_s"], "grad_rel_err_vs_fp64_single_path": err,
+            "workload": "64 instances, L uniform in {8,10,12,14}, odd depth 3..11, targets random brickwalls of 3x "
+                        "depth (materialised at admission), FP64, seed 0" +
+                        ("" if niter == 20 else f"; {niter} of the 20 iterations per instance")}
+
+
 def large_bench(mg):
     """The large single instance: L=16, open brickwall depth 11 (83 gates), target = 50
     second-order Trotter steps of the XXZ chain with random fields (758 matrix-free gates);
@@ -233,6 +271,9 @@ def main():
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-sweep variant")
     ap.add_argument("--serve-jobs", type=int, default=32, help="jobs of the served stream (0 skips it)")
     ap.add_argument("--large", action="store_true", help="also run the L=16 instance (about 20 s of GPU time)")
+    ap.add_argument("--batched-iters", type=int, default=2,
+                    help="trust-region iterations per instance of the 64-instance batched workload (20 is the "
+                         "full workload, about 2.5 min of GPU time; 0 skips it)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 5 times (reference kernel for HBM byte counters)")
     args = ap.parse_args()
@@ -294,6 +335,8 @@ def main():
 
     if args.serve_jobs > 0:
         extra["served"] = served_bench(mg, args.serve_jobs)
+    if args.batched_iters > 0:
+        extra["batched"] = batched_bench(mg, args.batched_iters)
     if args.large:
         mg.set_kernel_timing(True)
         extra["large_L16"] = large_bench(mg)
