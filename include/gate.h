@@ -26,6 +26,9 @@ void apply_matchgate_backward_array(const struct matchgate* gate, const int i, c
 /* src/gate.c:982 */
 void apply_matchgate_placeholder(const int i, const int j, const struct statevector* psi,
                                  struct statevector_array* psi_out);
+/* src/gate.c:1066 (general two-qubit gate: the 16 states d(G psi)/dG[r][c], psi_out->nstates == 16) */
+void apply_gate_placeholder(const int i, const int j, const struct statevector* psi,
+                            struct statevector_array* psi_out);
 
 #ifdef __cplusplus
 }

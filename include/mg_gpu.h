@@ -66,6 +66,13 @@ int mg_evaluate_columns(const mg_target* t, const mg_program* p, const struct ma
                         const struct matchgate Z[], int what, int precision, long b0, long b1, numeric* f,
                         struct matchgate dgrad[], struct matchgate dhz[]);
 
+/* the Riemannian projection alone (host, FP64): summed partial sums of mg_evaluate_columns
+ * (after adding them across devices) -> Riemannian gradient (8P reals) and, if v != NULL, the
+ * Hessian-vector product H v (8P reals) of the direction v whose tangent was passed as Z;
+ * mg_riemannian_gradient_hvp applies the same projection to a whole-trace evaluation */
+int mg_riemannian_project(const struct matchgate gates[], int nparams, const struct matchgate dgrad[], const double* v,
+                          const struct matchgate dhz[], double* grad_vec, double* hvp);
+
 /* Riemannian gradient (8P reals, src/matchgate_target.c:903-908) and, if v != NULL,
  * the Riemannian Hessian-vector product H v (8P reals), one sweep */
 int mg_riemannian_gradient_hvp(const mg_target* t, const mg_program* p, const struct matchgate gates[], const double* v,

@@ -579,6 +579,24 @@ int mg_riemannian_gradient_hvp(const mg_target* t, const mg_program* p, const st
     return 0;
 }
 
+int mg_riemannian_project(const struct matchgate gates[], int nparams, const struct matchgate dgrad[], const double* v,
+                          const struct matchgate dhz[], double* grad_vec, double* hvp)
+{
+    if (!gates || nparams < 1 || !dgrad || (v && !dhz)) {
+        mg::set_error("mg_riemannian_project: null argument");
+        return -1;
+    }
+    const zd* g = Zp(dgrad);
+    if (grad_vec) riemannian_gradient(gates, nparams, g, grad_vec);
+    if (v && hvp)
+        for (int i = 0; i < nparams; i++) {
+            struct matchgate Z;
+            real_to_tangent_matchgate(v + 8 * i, &gates[i], &Z);
+            weingarten_block(&gates[i], g + 8 * i, &Z, Zp(dhz) + 8 * i, hvp + 8 * i);
+        }
+    return 0;
+}
+
 int mg_hessian(const mg_target* t, const mg_program* p, const struct matchgate gates[], int precision, numeric* f,
                struct matchgate dgrad[], numeric* hess)
 {

@@ -133,6 +133,29 @@ __global__ void k_placeholder(SortedGate s, const C* __restrict__ psi, C* __rest
     }
 }
 
+// the 16 states d(G psi)/dG[r][c] of a general two-qubit gate G (reference src/gate.c:1066-1143):
+// with r, c pair values in the caller's wire order (bit of wire i high, of wire j low), state
+// 4 r + c holds psi's amplitude of pair value c at the amplitude of pair value r, zero elsewhere
+__global__ void k_gate_placeholder(int lo, int hi, int flip, const C* __restrict__ psi, C* __restrict__ out, uint32_t Q)
+{
+    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < Q; q += gridDim.x * blockDim.x) {
+        const uint32_t b = quad_base(q, lo, hi);
+        // amplitude index of pair value p = 2 k + l (k: bit of wire i, l: bit of wire j)
+        uint32_t at[4];
+        for (int p = 0; p < 4; p++) {
+            const uint32_t k = p >> 1, l = p & 1;
+            at[p] = flip ? (b | (k << lo) | (l << hi)) : (b | (k << hi) | (l << lo));
+        }
+        C v[4];
+        for (int p = 0; p < 4; p++) v[p] = psi[at[p]];
+        const C zero = {0.0, 0.0};
+        for (int r = 0; r < 4; r++) {
+            C* o = out + 16 * (size_t)at[r];
+            for (int e = 0; e < 16; e++) o[e] = (e >> 2) == r ? v[e & 3] : zero;
+        }
+    }
+}
+
 struct Dev {
     void* p = nullptr;
     ~Dev() { if (p) (void)hipFree(p); }
@@ -329,6 +352,26 @@ void apply_matchgate_placeholder(const int i, const int j, const struct statevec
     identity_matchgate(&unit);
     if (d_placeholder(sort_gate(&unit, i, j, L), (C*)a.p, (C*)b.p, L, st) < 0) return;
     down(psi_out->data, b.p, 8 * S * sizeof(C), st);
+}
+
+void apply_gate_placeholder(const int i, const int j, const struct statevector* psi, struct statevector_array* psi_out)
+{
+    const int L = psi->nqubits;
+    if (psi_out->nqubits != L || psi_out->nstates != 16 || i < 0 || j < 0 || i >= L || j >= L || i == j) {
+        set_error("apply_gate_placeholder: invalid wires or output array (16 states of the same qubit count)");
+        return;
+    }
+    const size_t S = (size_t)1 << L;
+    hipStream_t st = stream();
+    Dev a, b;
+    if (dalloc(a, S * sizeof(C)) < 0 || dalloc(b, 16 * S * sizeof(C)) < 0) { set_error("device allocation failed"); return; }
+    if (up(a.p, psi->data, S * sizeof(C), st) < 0) return;
+    const int flip = j < i;
+    const int hi = L - 1 - std::min(i, j), lo = L - 1 - std::max(i, j);
+    const uint32_t Q = 1u << (L - 2);
+    hipLaunchKernelGGL(k_gate_placeholder, dim3(grid_for(Q)), dim3(256), 0, st, lo, hi, flip, (const C*)a.p, (C*)b.p, Q);
+    if (hipGetLastError() != hipSuccess) { set_error("apply_gate_placeholder: launch failed"); return; }
+    down(psi_out->data, b.p, 16 * S * sizeof(C), st);
 }
 
 // ---------------- quantum_circuit.h ----------------

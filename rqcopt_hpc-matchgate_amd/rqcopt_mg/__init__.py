@@ -110,6 +110,7 @@ _sig("mg_program_ninst", [_P])
 _sig("mg_evaluate", [_P, _P, _P, _P, _I, _I, _P, _P, _P])
 _sig("mg_evaluate_columns", [_P, _P, _P, _P, _I, _I, _L, _L, _P, _P, _P])
 _sig("mg_riemannian_gradient_hvp", [_P, _P, _P, _P, _I, _P, _P, _P])
+_sig("mg_riemannian_project", [_P, _I, _P, _P, _P, _P, _P])
 _sig("mg_hessian", [_P, _P, _P, _I, _P, _P, _P])
 _sig("mg_gradient_vector_hessian_matrix", [_P, _P, _P, _I, _P, _P, _P])
 _sig("mg_optimize", [_P, _P, _P, ctypes.POINTER(RTRParams), _I, _I, _I, _I, _P, _P, _P, _P])
@@ -158,6 +159,7 @@ _sig("apply_matchgate_backward", [_P, _I, _I, _SV, _SV, _SV, _P], None)
 _sig("apply_matchgate_to_array", [_P, _I, _I, _SA, _SA], None)
 _sig("apply_matchgate_backward_array", [_P, _I, _I, _SA, _SV, _P], None)
 _sig("apply_matchgate_placeholder", [_I, _I, _SV, _SA], None)
+_sig("apply_gate_placeholder", [_I, _I, _SV, _SA], None)
 _sig("apply_quantum_matchgate_circuit", [_P, _I, _P, _SV, _SV])
 _sig("quantum_matchgate_circuit_forward", [_P, _I, _P, _SV, _QC, _SV])
 _sig("quantum_matchgate_circuit_backward", [_P, _I, _P, _QC, _SV, _SV, _P])
@@ -293,6 +295,32 @@ def evaluate_columns(target, program, gates, Z, what, precision, b0, b1):
     _check(lib.mg_evaluate_columns(target.h, program.h, _p(gates), _p(Zc) if Zc is not None else None, what,
                                    precision, b0, b1, _p(f), _p(dg), _p(dh)), "mg_evaluate_columns")
     return f[0], dg, dh
+
+
+def real_to_directions(gates, v):
+    """complex tangent directions Z_i = V_i A(v_i) (P, 8) of real parameters v (8P)
+    (real_to_tangent_matchgate per gate, reference src/matchgate.c:296)"""
+    gates = _c(gates)
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    Z = np.empty_like(gates)
+    for k in range(gates.shape[0]):
+        lib.real_to_tangent_matchgate(_p(v[8 * k:8 * k + 8]), _p(gates[k]), _p(Z[k]))
+    return Z
+
+
+def riemannian_project(gates, dgrad, v=None, dhz=None):
+    """Riemannian gradient (8P) and, with v, H v (8P) from summed partial sums (P, 8) of
+    evaluate_columns (mg_riemannian_project; host FP64, no device work)."""
+    gates = _c(gates)
+    P = gates.shape[0]
+    dg = _c(dgrad).reshape(P, 8)
+    g = np.zeros(8 * P)
+    hv = np.zeros(8 * P)
+    vv = np.ascontiguousarray(v, dtype=np.float64) if v is not None else None
+    dh = _c(dhz).reshape(P, 8) if v is not None else None
+    _check(lib.mg_riemannian_project(_p(gates), P, _p(dg), _p(vv) if vv is not None else None,
+                                     _p(dh) if dh is not None else None, _p(g), _p(hv)), "mg_riemannian_project")
+    return g, (hv if v is not None else None)
 
 
 def riemannian_gradient_hvp(target, program, gates, v=None, precision=MG_FP64):

@@ -407,6 +407,11 @@ def test_single_column_kernels_fixtures(mg):
         sa = mg.statevector_array(out)
         mg.lib.apply_matchgate_placeholder(i, j, ctypes.byref(mg.statevector(psi)), ctypes.byref(sa))
         assert np.max(np.abs(sa._keep - cvec(d[key]).reshape(128, 16)[:, emap])) < 1e-12
+        # the general two-qubit gate placeholder (test/test_gate.c:376-380, 399-403): all 16 states
+        out16 = np.full((128, 16), np.nan + 0j)
+        sa16 = mg.statevector_array(out16)
+        mg.lib.apply_gate_placeholder(i, j, ctypes.byref(mg.statevector(psi)), ctypes.byref(sa16))
+        assert np.max(np.abs(sa16._keep - cvec(d[key]).reshape(128, 16))) < 1e-12
     d = load("test_apply_matchgate_to_array_cplx")
     V = gate(d, "v.")
     a = d["psi"]
