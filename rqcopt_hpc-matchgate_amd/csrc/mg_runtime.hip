@@ -447,11 +447,12 @@ static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, in
     }
     // register-resident gate-window kernel (FP64, 512 threads x 8 amplitudes, dense targets)
     size_t win_acc = 0, win_buf = 0, win_lds = 0;
+    int win_rs = 0;
     bool use_win = false;
     constexpr int fp32 = std::is_same<R, float>::value ? 1 : 0;
     if (!ckpt && b.defect <= 1e-12 && t.kind == 0 && win_sweep_fits(L, fp32) && kn.force_win != 0 && kn.force_reg != 0 &&
         kn.force_lds != 0 && (!fp32 || kn.force_win == 1)) {
-        win_lds = win_sweep_lds_bytes(L, p.P, fp32, &win_acc, &win_buf);
+        win_lds = win_sweep_lds_bytes(L, p.P, fp32, c->lds_limit, &win_acc, &win_buf, &win_rs);
         use_win = win_lds <= c->lds_limit;
     }
     if (use_win) reg_T = 0;
@@ -533,6 +534,7 @@ static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, in
         wa.partial = (double*)c->partial.p;
         wa.acc_off = (int)win_acc;
         wa.buf_off = (int)win_buf;
+        wa.rs = win_rs;
         // the copies above read pageable host vectors: complete them before w goes away
         MG_HIP(hipStreamSynchronize(s));
     }

@@ -6,7 +6,8 @@
 // window tuple in VGPRs (FP64: 512 threads x 8, FP32: 256 x 16 = 2^12); a block's gates are
 // applied there, and only between blocks are the states re-laid out through LDS
 // (write in the block's tuple order, barrier, read in the next block's order),
-// two FP64 or four FP32 states per round.  Compared with a re-layout per gate this halves the LDS
+// two FP64 (one when the accumulator of many parameter gates leaves no room for two) or four FP32
+// states per round.  Compared with a re-layout per gate this halves the LDS
 // traffic and the barriers of the brickwall sweeps.
 //
 // Reverse sweep per gate, in three passes over the tuple's quadruples so that one
@@ -97,13 +98,13 @@ __device__ inline void stage8(const cpx<R>* s, double* slot, int lane)
 
 }  // namespace
 
-template <typename R, int MODE>
+template <typename R, int MODE, int RS>
 __global__ __launch_bounds__(WinShape<R>::T) void sweep_win_kernel(WinSweepArgs a)
 {
     constexpr int kT = WinShape<R>::T, kW = WinShape<R>::W, kNW = kT / 64;
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr int NV = MODE == 2 ? 32 : 16;
-    constexpr bool ONE_ROUND = sizeof(R) == 4;  // all four FP32 states fit the re-layout buffer
+    // RS states per re-layout round (the buffer holds RS states): FP32 4, FP64 2 or 1
     const uint32_t S = 1u << a.L;
     const int tid0 = threadIdx.x, lane = tid0 & 63, wv = tid0 >> 6;
 
@@ -150,11 +151,17 @@ __global__ __launch_bounds__(WinShape<R>::T) void sweep_win_kernel(WinSweepArgs 
                 const uint32_t sb = lin<R, true>(deposit<kW>((uint32_t)tid, bk.pos));
                 const uint32_t sn = lin<R, true>(deposit<kW>((uint32_t)tid, nx.pos));
                 win_store<R, kW, true>(buf, sb, bk.off, P);
-                if constexpr (MODE == 2) win_store<R, kW, true>(buf + S, sb, bk.off, D);
+                if constexpr (MODE == 2 && RS >= 2) win_store<R, kW, true>(buf + S, sb, bk.off, D);
                 __syncthreads();
                 win_load<R, kW, true>(buf, sn, nx.off, P);
-                if constexpr (MODE == 2) win_load<R, kW, true>(buf + S, sn, nx.off, D);
+                if constexpr (MODE == 2 && RS >= 2) win_load<R, kW, true>(buf + S, sn, nx.off, D);
                 __syncthreads();
+                if constexpr (MODE == 2 && RS == 1) {
+                    win_store<R, kW, true>(buf, sb, bk.off, D);
+                    __syncthreads();
+                    win_load<R, kW, true>(buf, sn, nx.off, D);
+                    __syncthreads();
+                }
             }
         }
         // ---- seed lam = -conj(U e_b) in the last block's tuple order (after the forward sweep,
@@ -225,7 +232,7 @@ __global__ __launch_bounds__(WinShape<R>::T) void sweep_win_kernel(WinSweepArgs 
                     MG_WCASES(kW, MG_PC)
 #undef MG_PC
                 }
-                // re-layout to the previous block's order (two states per round); the staged
+                // re-layout to the previous block's order (RS states per round); the staged
                 // sums are consumed after the first barrier
                 const bool move = k > 0;
                 BlockRegs<kW> pv = bk;
@@ -234,8 +241,8 @@ __global__ __launch_bounds__(WinShape<R>::T) void sweep_win_kernel(WinSweepArgs 
                 const uint32_t sn = lin<R, true>(deposit<kW>((uint32_t)tid, pv.pos));
                 if (move) {
                     win_store<R, kW, true>(buf, sb, bk.off, Lm);
-                    win_store<R, kW, true>(buf + S, sb, bk.off, P);
-                    if constexpr (MODE == 2 && ONE_ROUND) {
+                    if constexpr (RS >= 2) win_store<R, kW, true>(buf + S, sb, bk.off, P);
+                    if constexpr (MODE == 2 && RS >= 4) {
                         win_store<R, kW, true>(buf + 2 * S, sb, bk.off, DL);
                         win_store<R, kW, true>(buf + 3 * S, sb, bk.off, D);
                     }
@@ -255,18 +262,34 @@ __global__ __launch_bounds__(WinShape<R>::T) void sweep_win_kernel(WinSweepArgs 
                 }
                 if (move) {
                     win_load<R, kW, true>(buf, sn, pv.off, Lm);
-                    win_load<R, kW, true>(buf + S, sn, pv.off, P);
-                    if constexpr (MODE == 2 && ONE_ROUND) {
+                    if constexpr (RS >= 2) win_load<R, kW, true>(buf + S, sn, pv.off, P);
+                    if constexpr (MODE == 2 && RS >= 4) {
                         win_load<R, kW, true>(buf + 2 * S, sn, pv.off, DL);
                         win_load<R, kW, true>(buf + 3 * S, sn, pv.off, D);
                     }
                     __syncthreads();
-                    if constexpr (MODE == 2 && !ONE_ROUND) {
+                    if constexpr (RS == 1) {
+                        win_store<R, kW, true>(buf, sb, bk.off, P);
+                        __syncthreads();
+                        win_load<R, kW, true>(buf, sn, pv.off, P);
+                        __syncthreads();
+                    }
+                    if constexpr (MODE == 2 && RS == 2) {
                         win_store<R, kW, true>(buf, sb, bk.off, DL);
                         win_store<R, kW, true>(buf + S, sb, bk.off, D);
                         __syncthreads();
                         win_load<R, kW, true>(buf, sn, pv.off, DL);
                         win_load<R, kW, true>(buf + S, sn, pv.off, D);
+                        __syncthreads();
+                    }
+                    if constexpr (MODE == 2 && RS == 1) {
+                        win_store<R, kW, true>(buf, sb, bk.off, DL);
+                        __syncthreads();
+                        win_load<R, kW, true>(buf, sn, pv.off, DL);
+                        __syncthreads();
+                        win_store<R, kW, true>(buf, sb, bk.off, D);
+                        __syncthreads();
+                        win_load<R, kW, true>(buf, sn, pv.off, D);
                         __syncthreads();
                     }
                 } else {
@@ -285,7 +308,7 @@ __global__ __launch_bounds__(WinShape<R>::T) void sweep_win_kernel(WinSweepArgs 
     for (int k = tid0; k < 32 * a.P; k += kT) out[2 + k] = acc[k];
 }
 
-size_t win_sweep_lds_bytes(int L, int P, int fp32, size_t* acc_off, size_t* buf_off)
+size_t win_sweep_lds_bytes(int L, int P, int fp32, size_t limit, size_t* acc_off, size_t* buf_off, int* rs)
 {
     const int nw = (fp32 ? WinShape<float>::T : WinShape<double>::T) / 64;
     const size_t fixed = sizeof(double) * ((size_t)kWinRegGates * nw * 32 + 2 * nw);
@@ -293,8 +316,12 @@ size_t win_sweep_lds_bytes(int L, int P, int fp32, size_t* acc_off, size_t* buf_
     const size_t b0 = (a0 + sizeof(double) * 32 * (size_t)P + 255) & ~(size_t)255;
     if (acc_off) *acc_off = a0;
     if (buf_off) *buf_off = b0;
-    // two FP64 or four FP32 states per re-layout round
-    return b0 + (fp32 ? 4 * sizeof(cpx<float>) : 2 * sizeof(cpx<double>)) * ((size_t)1 << L);
+    const size_t state = (fp32 ? sizeof(cpx<float>) : sizeof(cpx<double>)) * ((size_t)1 << L);
+    // four FP32 states per re-layout round; FP64 two, or one when two do not fit
+    int n = fp32 ? 4 : 2;
+    if (!fp32 && b0 + 2 * state > limit) n = 1;
+    if (rs) *rs = n;
+    return b0 + (size_t)n * state;
 }
 
 bool win_sweep_fits(int L, int fp32)
@@ -305,25 +332,28 @@ bool win_sweep_fits(int L, int fp32)
 
 int win_sweep_window_bits(int fp32) { return fp32 ? WinShape<float>::W : WinShape<double>::W; }
 
-template <typename R>
+template <typename R, int RS>
 static hipError_t launch_win_r(const WinSweepArgs& a, int mode, int grid, size_t lds, hipStream_t s)
 {
-    const void* k = mode == 2 ? (const void*)sweep_win_kernel<R, 2>
-                  : mode == 1 ? (const void*)sweep_win_kernel<R, 1> : (const void*)sweep_win_kernel<R, 0>;
+    const void* k = mode == 2 ? (const void*)sweep_win_kernel<R, 2, RS>
+                  : mode == 1 ? (const void*)sweep_win_kernel<R, 1, RS> : (const void*)sweep_win_kernel<R, 0, RS>;
     if (lds > 65536) {
         hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
     constexpr int T = WinShape<R>::T;
-    if (mode == 2) hipLaunchKernelGGL((sweep_win_kernel<R, 2>), dim3(grid), dim3(T), lds, s, a);
-    else if (mode == 1) hipLaunchKernelGGL((sweep_win_kernel<R, 1>), dim3(grid), dim3(T), lds, s, a);
-    else hipLaunchKernelGGL((sweep_win_kernel<R, 0>), dim3(grid), dim3(T), lds, s, a);
+    if (mode == 2) hipLaunchKernelGGL((sweep_win_kernel<R, 2, RS>), dim3(grid), dim3(T), lds, s, a);
+    else if (mode == 1) hipLaunchKernelGGL((sweep_win_kernel<R, 1, RS>), dim3(grid), dim3(T), lds, s, a);
+    else hipLaunchKernelGGL((sweep_win_kernel<R, 0, RS>), dim3(grid), dim3(T), lds, s, a);
     return hipGetLastError();
 }
 
 hipError_t launch_sweep_win(const WinSweepArgs& a, int mode, int fp32, int grid, size_t lds, hipStream_t s)
 {
-    return fp32 ? launch_win_r<float>(a, mode, grid, lds, s) : launch_win_r<double>(a, mode, grid, lds, s);
+    if (fp32) return a.rs == 4 ? launch_win_r<float, 4>(a, mode, grid, lds, s) : hipErrorInvalidValue;
+    if (a.rs == 2) return launch_win_r<double, 2>(a, mode, grid, lds, s);
+    if (a.rs == 1) return launch_win_r<double, 1>(a, mode, grid, lds, s);
+    return hipErrorInvalidValue;
 }
 
 }  // namespace mg

@@ -17,10 +17,13 @@ struct WinSweepArgs {
     long b0, b1;               // column range
     double* partial;           // [gridDim.x][2 + 32 P]
     int acc_off, buf_off;      // byte offsets in dynamic LDS
+    int rs;                    // states per re-layout round (FP32: 4; FP64: 2, or 1 for many parameters)
 };
 
-// dynamic LDS of the kernel (and the accumulator and buffer offsets)
-size_t win_sweep_lds_bytes(int L, int P, int fp32, size_t* acc_off, size_t* buf_off);
+// dynamic LDS of the kernel within `limit` bytes (and the accumulator and buffer offsets, and the
+// states per re-layout round that fit: FP64 re-lays two states out per round, or one when the
+// accumulator of many parameter gates leaves no room for two)
+size_t win_sweep_lds_bytes(int L, int P, int fp32, size_t limit, size_t* acc_off, size_t* buf_off, int* rs);
 // the kernel's threads x window amplitudes cover the column (L=12 in both precisions)
 bool win_sweep_fits(int L, int fp32);
 int win_sweep_window_bits(int fp32);

@@ -212,11 +212,13 @@ def test_register_kernel_matches_oracle(mg, L, layout, depth, threads, prec):
     assert abs(r1[0] - fo) / abs(fo) < tol and rel(r1[1], dgo) < tol and abs(r0[0] - fo) / abs(fo) < tol
 
 
-@pytest.mark.parametrize("layout,depth", [("tied", 3), ("open", 5), ("open", 9)])
+@pytest.mark.parametrize("layout,depth", [("tied", 3), ("open", 5), ("open", 9), ("open", 15)])
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
 def test_window_register_kernel_matches_oracle(mg, layout, depth, prec):
     """The register-resident gate-window kernel (L=12): tied brickwalls with random
-    permutations (windows of scattered qubits) and open brickwalls, all three modes."""
+    permutations (windows of scattered qubits) and open brickwalls, all three modes; at depth 15
+    (83 parameter gates) the FP64 kernel re-lays one state out per round (its accumulator leaves
+    no LDS room for two)."""
     L = 12
     say(f"case: register window kernel L={L} {layout} depth={depth} {prec}")
     V, Z, param, wires, P, U = build_case(L, layout, depth, seed=depth * 5 + 1)
@@ -226,7 +228,11 @@ def test_window_register_kernel_matches_oracle(mg, layout, depth, prec):
     mg.set_window_kernel(1)
     try:
         r = mg.evaluate_columns(t, p, V, Z, mg.EVAL_F_GRAD_HVP, precision, 7, 71)
-        assert mg.last_sweep_info()["kernel"] == "sweep_win_kernel"
+        info = mg.last_sweep_info()
+        assert info["kernel"] == "sweep_win_kernel"
+        if prec == "fp64":
+            # one state (64 KB) or two (128 KB) per re-layout round
+            assert (info["lds_bytes"] < 100 * 1024) == (P > 79), (P, info["lds_bytes"])
         r1 = mg.evaluate_columns(t, p, V, None, mg.EVAL_F_GRAD, precision, 7, 71)
         r0 = mg.evaluate_columns(t, p, V, None, mg.EVAL_F, precision, 7, 71)
     finally:
