@@ -175,7 +175,7 @@ def batched_bench(mg, niter, seed=0):
     mg.set_kernel_timing(True)
     log(f"batched workload: 64 instances, {niter} trust-region iterations each (FP64) ...")
     res, st = mg.serve(jobs, batched=True, precision=mg.MG_FP64, precision_switch=0, densify_qubits=14,
-                       realtime=False)
+                       realtime=False, progress_s=30.0)
     iters = sum(len(r["f_iter"]) - 1 for r in res)
     log(f"  {iters / st['wall_s']:.1f} trust-region iterations/s, {st['rounds']} launches, "
         f"{st['graph_replays']} from graphs")

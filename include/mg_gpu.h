@@ -118,9 +118,10 @@ int mg_evaluate_batch(mg_request* reqs, int n);
  * Compiled: FP32 (256, 2..4), (512, 2..3); FP64 (256, 2..3), (512, 2).  Returns -1 for others. */
 int mg_batch_set_kernel(int precision, int threads, int window_bits);
 void mg_batch_get_kernel(int precision, int* threads, int* window_bits);
-/* requests with dense targets of min_qubits..max_qubits qubits (default 12..12) fill the GPU on
+/* requests with dense targets of min_qubits..max_qubits qubits (default 12..30) fill the GPU on
  * their own: mg_evaluate_batch runs them on the single-request kernels (register gate-window,
- * LDS sweeps), one launch each, and the rest in the persistent batch; (0, 0) batches everything */
+ * LDS and tiled sweeps), one launch each, and the rest in the persistent batch; (0, 0) batches
+ * everything */
 void mg_batch_set_single_range(int min_qubits, int max_qubits);
 /* workgroups per CU of the batched launch (0 automatic) */
 void mg_batch_set_grid_per_cu(int n);
@@ -179,6 +180,8 @@ typedef struct mg_serve_options {
     int densify_qubits;   /* circuit targets of at most this many qubits are materialised once on the
                              GPU at admission (2^n x 2^n FP64, plus an FP32 copy) and read as dense
                              rows afterwards; 0: always applied gate by gate per column */
+    double progress_s;    /* > 0: after a launch round (or a finished job, one job at a time), print a
+                             progress line to stderr if this many seconds passed since the last one */
 } mg_serve_options;
 
 typedef struct mg_serve_stats {

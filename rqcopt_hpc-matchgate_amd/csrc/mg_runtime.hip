@@ -12,6 +12,7 @@
 
 #include "mg_batch.hpp"
 #include "mg_sweep.hpp"
+#include "mg_sweep_tile.hpp"
 #include "mg_sweep_win.hpp"
 
 namespace mg {
@@ -349,6 +350,9 @@ static int ensure_fp32_target(Target& t, hipStream_t s)
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
+// states per column of the tiled kernel: psi (f), + lam (gradient), + dpsi, dlam (HVP)
+static int nstate_tile(int mode) { return mode == 2 ? 4 : mode == 1 ? 2 : 1; }
+
 // window blocks of an instance list, reordered instances (lo = window-local pair
 // index) and coefficients, block offsets for LDS (swizzled) and global states
 struct Windowed {
@@ -456,6 +460,17 @@ static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, in
         use_win = win_lds <= c->lds_limit;
     }
     if (use_win) reg_T = 0;
+    // tiled kernel: dense targets whose states do not fit the LDS (global state slices, one LDS
+    // tile of 2^kw amplitudes per state and pass)
+    const int tile_kw = tile_window_bits(fp32);
+    size_t tile_off = 0, tile_lds = 0;
+    bool use_tile = false;
+    if (!use_win && !ckpt && !lds_states && t.kind == 0 && L > tile_kw && kn.force_tile != 0 && kn.force_lds == -1 &&
+        kn.force_reg != 1) {
+        tile_lds = tile_sweep_lds_bytes(p.P, tile_kw, fp32, nstate_tile(mode), &tile_off);
+        use_tile = tile_lds <= c->lds_limit;
+    }
+    if (use_tile) reg_T = 0;
     const int nw_reg = reg_T / 64;
     const size_t head_reg = reg_T ? sweep_lds_head_bytes(p.P, nw_reg) : 0;
     int reg_nst = 0;
@@ -465,15 +480,18 @@ static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, in
         if (reg_nst < 1) reg_T = 0;
     }
     const size_t lds = use_win ? win_lds
+                     : use_tile ? tile_lds
                      : reg_T ? head_reg + (size_t)reg_nst * S * sizeof(cpx<R>) : (lds_states ? head + state_bytes : head);
 
     const long ncols = b1 - b0;
     int per_cu = kn.grid_per_cu > 0 ? kn.grid_per_cu : std::max<int>(1, std::min<size_t>(4, (160 * 1024) / std::max<size_t>(lds, 1)));
-    if ((reg_T || use_win) && kn.grid_per_cu <= 0) per_cu = std::max<int>(1, std::min<size_t>(2, (160 * 1024) / lds));
+    if ((reg_T || use_win || use_tile) && kn.grid_per_cu <= 0) per_cu = std::max<int>(1, std::min<size_t>(2, (160 * 1024) / lds));
     long grid = std::min<long>(ncols, (long)c->ncu * per_cu);
     // bound the global scratch and checkpoint footprint (2 GiB each)
     const size_t lim = (size_t)2 << 30;
-    if (!lds_states && !reg_T && !use_win) grid = std::min<long>(grid, std::max<long>(1, lim / state_bytes));
+    const size_t tile_state_bytes = (size_t)nstate_tile(mode) * S * sizeof(cpx<R>);
+    if (use_tile) grid = std::min<long>(grid, std::max<long>(1, lim / tile_state_bytes));
+    else if (!lds_states && !reg_T && !use_win) grid = std::min<long>(grid, std::max<long>(1, lim / state_bytes));
     const size_t ck_per = ckpt ? (size_t)2 * (p.G + 1) * S * sizeof(cpx<R>) : 0;
     if (ckpt) grid = std::min<long>(grid, std::max<long>(1, lim / ck_per));
     grid = std::max<long>(grid, 1);
@@ -481,7 +499,8 @@ static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, in
     const int stride = 2 + 32 * p.P;
     MG_HIP(c->partial.ensure((size_t)grid * stride * sizeof(double)));
     MG_HIP(c->out.ensure((size_t)stride * sizeof(double)));
-    if (!lds_states && !reg_T && !use_win) MG_HIP(c->scratch.ensure((size_t)grid * state_bytes));
+    if (use_tile) MG_HIP(c->scratch.ensure((size_t)grid * tile_state_bytes));
+    else if (!lds_states && !reg_T && !use_win) MG_HIP(c->scratch.ensure((size_t)grid * state_bytes));
     if (ckpt) MG_HIP(c->ckpt.ensure((size_t)grid * ck_per));
 
     SweepArgs<R> a{};
@@ -538,8 +557,63 @@ static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, in
         // the copies above read pageable host vectors: complete them before w goes away
         MG_HIP(hipStreamSynchronize(s));
     }
+    TileArgs<R> ta{};
+    if (use_tile) {
+        std::vector<int> order;
+        std::vector<TBlock> blocks;
+        tile_blocks(b.inst, L, tile_kw, &order, &blocks);
+        if (order.size() != b.inst.size()) {
+            set_error("tiled sweep: gate blocking failed");
+            return -1;
+        }
+        std::vector<Inst> ti(order.size());
+        std::vector<cpx<R>> tc(order.size() * kCoefPerInst);
+        for (const TBlock& bk : blocks)
+            for (int k = bk.g0; k < bk.g1; k++) {
+                Inst in = b.inst[order[k]];
+                // window-local bit positions (the window's positions are ascending)
+                const int lo = (int)(std::find(bk.pos, bk.pos + tile_kw, in.lo) - bk.pos);
+                const int hi = (int)(std::find(bk.pos, bk.pos + tile_kw, in.hi) - bk.pos);
+                if (lo >= tile_kw || hi >= tile_kw) {
+                    set_error("tiled sweep: gate outside its window");
+                    return -1;
+                }
+                in.lo = lo;
+                in.hi = hi;
+                ti[k] = in;
+                for (int e = 0; e < kCoefPerInst; e++) {
+                    const cpx<double> v = b.coef[(size_t)order[k] * kCoefPerInst + e];
+                    tc[(size_t)k * kCoefPerInst + e] = {(R)v.re, (R)v.im};
+                }
+            }
+        MG_HIP(c->w_inst.ensure(ti.size() * sizeof(Inst)));
+        MG_HIP(c->w_coef.ensure(tc.size() * sizeof(cpx<R>)));
+        MG_HIP(c->w_blk.ensure(blocks.size() * sizeof(TBlock)));
+        MG_HIP(hipMemcpyAsync(c->w_inst.p, ti.data(), ti.size() * sizeof(Inst), hipMemcpyHostToDevice, s));
+        MG_HIP(hipMemcpyAsync(c->w_coef.p, tc.data(), tc.size() * sizeof(cpx<R>), hipMemcpyHostToDevice, s));
+        MG_HIP(hipMemcpyAsync(c->w_blk.p, blocks.data(), blocks.size() * sizeof(TBlock), hipMemcpyHostToDevice, s));
+        ta.L = L;
+        ta.G = p.G;
+        ta.P = p.P;
+        ta.kw = tile_kw;
+        ta.nblocks = (int)blocks.size();
+        ta.inst = (const Inst*)c->w_inst.p;
+        ta.coef = (const cpx<R>*)c->w_coef.p;
+        ta.blocks = (const TBlock*)c->w_blk.p;
+        ta.seed = fp32 ? (const cpx<R>*)t.seed32.p : (const cpx<R>*)t.seed64.p;
+        ta.b0 = b0;
+        ta.b1 = b1;
+        ta.partial = (double*)c->partial.p;
+        ta.scratch = (cpx<R>*)c->scratch.p;
+        ta.nstate = nstate_tile(mode);
+        ta.tile_off = (int)tile_off;
+        // the copies above read pageable host vectors: complete them before they go away
+        MG_HIP(hipStreamSynchronize(s));
+    }
     if (kn.time_kernels) MG_HIP(hipEventRecord(c->ev0, s));
-    if (use_win)
+    if (use_tile)
+        MG_HIP(launch_sweep_tile<R>(ta, mode, (int)grid, lds, s));
+    else if (use_win)
         MG_HIP(launch_sweep_win(wa, mode, fp32, (int)grid, lds, s));
     else if (reg_T)
         MG_HIP(launch_sweep_reg<R>(a, mode, reg_T, reg_qpt, (int)grid, lds, s));
@@ -561,11 +635,11 @@ static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, in
             out->dhz[8 * q + e] = zd(o[2 + 32 * q + 16 + 2 * e], o[2 + 32 * q + 16 + 2 * e + 1]);
         }
     if (st) {
-        st->lds_states = lds_states && !reg_T && !use_win;
+        st->lds_states = lds_states && !reg_T && !use_win && !use_tile;
         st->ckpt = ckpt;
         st->reg_threads = use_win ? 512 : reg_T;
-        st->threads = use_win ? 512 : reg_T ? reg_T : lds_T;
-        st->kernel = use_win ? "sweep_win_kernel" : reg_T ? "sweep_reg_kernel" : "sweep_kernel";
+        st->threads = use_win || use_tile ? 512 : reg_T ? reg_T : lds_T;  // the tiled kernel: 512 in both precisions
+        st->kernel = use_win ? "sweep_win_kernel" : use_tile ? "sweep_tile_kernel" : reg_T ? "sweep_reg_kernel" : "sweep_kernel";
         st->grid = (int)grid;
         st->lds_bytes = lds;
         st->kernel_ms = 0.f;

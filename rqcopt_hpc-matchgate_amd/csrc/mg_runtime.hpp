@@ -99,6 +99,7 @@ struct Knobs {
     int reg_threads = 0;      // 0 auto, else 512 or 1024 threads for the register-resident kernel
     int lds_threads = 0;      // 0 auto, else 256, 512 or 1024 threads for the LDS/global kernel
     int force_win = -1;       // -1 auto (register gate-window kernel where it applies), 0 never
+    int force_tile = -1;      // -1 auto (tiled kernel for dense targets whose states exceed the LDS), 0 never
     bool time_kernels = false;
 };
 Knobs& knobs();
@@ -140,9 +141,9 @@ struct BatchTuning {
     int cgw[2][31] = {};
     int cpi[2][31] = {};
     // requests with dense targets and single_min <= L <= single_max fill the GPU on their own:
-    // they run on the single-request kernels (register gate-window / LDS sweeps) in their own
-    // launches instead of inside the persistent batch (0 / 0: never)
-    int single_min = 12, single_max = 12;
+    // they run on the single-request kernels (register gate-window / LDS / tiled sweeps) in
+    // their own launches instead of inside the persistent batch (0 / 0: never)
+    int single_min = 12, single_max = 30;
     // replay repeated launches (same arguments, buffers and sizes) from captured HIP graphs
     bool graphs = true;
 };

@@ -411,6 +411,17 @@ int mg_serve(mg_job* jobs, int njobs, const mg_serve_options* opt, mg_serve_stat
     std::vector<std::unique_ptr<JobRt>> rt(njobs);
     int next = 0, done = 0, peak_active = 0, rc = 0;
     size_t held = 0, peak_held = 0;
+    double last_report = 0.0;
+    // progress line after a round or a finished job (callers with long runs ask for it)
+    auto report = [&](long rounds, long requests) {
+        if (opt->progress_s <= 0.0) return;
+        const double t = now_s();
+        if (t - last_report < opt->progress_s) return;
+        last_report = t;
+        std::fprintf(stderr, "[mg_serve] %.1f s: %d of %d jobs done, %ld launches, %ld evaluations\n", t, done, njobs,
+                     rounds, requests);
+        std::fflush(stderr);
+    };
 
     auto start_job = [&](int i) -> bool {
         auto j = std::make_unique<JobRt>();
@@ -443,6 +454,8 @@ int mg_serve(mg_job* jobs, int njobs, const mg_serve_options* opt, mg_serve_stat
             rt[i]->target.reset();
             jobs[i].t_done = now_s();
             if (jobs[i].status != 0) rc = -1;
+            done++;
+            report(srv.rounds, srv.requests);
         }
     } else {
         std::unique_lock<std::mutex> lk(srv.mu);
@@ -504,6 +517,7 @@ int mg_serve(mg_job* jobs, int njobs, const mg_serve_options* opt, mg_serve_stat
                 lk.lock();
                 srv.rounds++;
                 srv.requests += (long)batch.size();
+                report(srv.rounds, srv.requests);
                 srv.kernel_ms += st.kernel_ms;
                 srv.eval_s += dt;
                 srv.prep_ms += st.prep_ms;

@@ -1,0 +1,308 @@
+// Tiled column sweep for columns larger than the LDS (L >= 13; FP64 from L = 12 when the
+// register window kernel does not apply).
+//
+// The gate program is regrouped on the host (tile_blocks) into blocks of consecutive gates that
+// all act inside one window of kw qubits (kw = 11 FP64, 12 FP32), a window that always holds the 3
+// lowest index bits so that a tile's amplitudes come in runs of 8 contiguous elements.  A column's
+// states live in a per-workgroup global slice; per block, the workgroup walks the 2^(L-kw) tiles
+// of the window (the amplitudes sharing the bits outside it): load the tile of every live state
+// into LDS (2^kw amplitudes each, 128 KB in all), apply the block's gates there one after the
+// other as the LDS-resident kernel does (quadruples, one barrier per gate, per-gate outer
+// products through the transposing wave butterfly into the workgroup's FP64 accumulator), store
+// the tile back.  A brickwall needs a handful of such passes (4 for L=14, depth 11, FP64)
+// where one-gate or 3/4-qubit-window passes need one per gate or per 2-4 gates.
+//
+// Per column: forward blocks (psi, dpsi), f = seed . psi_G while the last forward block is
+// stored, then the reverse blocks (lam seeded from the dense target row in the first one, psi
+// and dpsi uncomputed with V^{-1}).  Partial sums per workgroup as the other sweep kernels.
+//
+// Reference: per-column loop src/matchgate_target.c:788-837, forward
+// src/matchgate_brickwall.c:52-78, backward kernel src/gate.c:659-772.
+#include <algorithm>
+
+#include "mg_quad.hpp"
+#include "mg_sweep_tile.hpp"
+
+#ifndef MG_TILE_LT64
+#define MG_TILE_LT64 9
+#endif
+
+namespace mg {
+
+namespace {
+
+// tile bits: FP64 11 (2^11 amplitudes x 4 states x 16 bytes = 128 KB of LDS), FP32 12
+template <typename R>
+constexpr int tile_kw() { return sizeof(R) == 8 ? 11 : 12; }
+
+// threads per workgroup (log2): 512 (256 measured the same for FP64)
+template <typename R>
+constexpr int tile_lt() { return sizeof(R) == 8 ? MG_TILE_LT64 : 9; }
+
+__device__ inline uint32_t scatter_bits(uint32_t x, const int* pos, int n)
+{
+    uint32_t r = 0;
+    for (int i = 0; i < n; i++) r |= ((x >> i) & 1u) << pos[i];
+    return r;
+}
+
+// the amplitude index of tile position tid + m T of a block's tiles: win_tid | wm[m] | tile base
+template <int PER>
+struct TileMap {
+    uint32_t win_tid;
+    uint32_t wm[PER];
+};
+
+template <int KW, int LT, int PER>
+__device__ inline void tile_map(const TBlock* blk, int tid, TileMap<PER>* m)
+{
+    m->win_tid = scatter_bits((uint32_t)tid, blk->pos, LT);  // T = 2^LT threads
+#pragma unroll
+    for (int q = 0; q < PER; q++) m->wm[q] = scatter_bits((uint32_t)q << LT, blk->pos, KW);
+}
+
+}  // namespace
+
+template <typename R, int MODE>
+__global__ __launch_bounds__(1 << tile_lt<R>()) void sweep_tile_kernel(TileArgs<R> a)
+{
+    constexpr int LT = tile_lt<R>(), T = 1 << LT, NW = T / 64;
+    constexpr int NV = MODE == 2 ? 32 : 16;
+    constexpr int SHIFT = 6 - log2c<NV>();
+    constexpr int KW = tile_kw<R>();
+    constexpr uint32_t TS = 1u << KW, Qt = TS >> 2;
+    constexpr int PER = (int)(TS / T);
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int L = a.L;
+    const uint32_t S = 1u << L;
+    const uint32_t ntiles = 1u << (L - KW);
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+
+    double* acc = reinterpret_cast<double*>(smem);  // [P][32]
+    double* stage = acc + 32 * a.P;                 // [2][NW][NV]
+    double* fst = stage + 2 * NW * NV;              // [NW][2]
+    cpx<R>* tl = reinterpret_cast<cpx<R>*>(smem + a.tile_off);
+    // tile slots and global slices: psi, dpsi (MODE 2), lam (MODE >= 1), dlam (MODE 2)
+    constexpr int LAM = MODE == 2 ? 2 : 1;
+    cpx<R>* t_psi = tl;
+    cpx<R>* t_dpsi = tl + TS;
+    cpx<R>* t_lam = tl + LAM * TS;
+    cpx<R>* t_dlam = tl + 3 * TS;
+    cpx<R>* sc = a.scratch + (size_t)blockIdx.x * a.nstate * S;
+    cpx<R>* g_psi = sc;
+    cpx<R>* g_dpsi = sc + S;
+    cpx<R>* g_lam = sc + (size_t)LAM * S;
+    cpx<R>* g_dlam = sc + 3 * (size_t)S;
+
+    for (int k = tid; k < 32 * a.P; k += T) acc[k] = 0.0;
+    double fre = 0.0, fim = 0.0;
+    __syncthreads();
+
+    const int nb = a.nblocks;
+    for (long b = a.b0 + blockIdx.x; b < a.b1; b += gridDim.x) {
+        const cpx<R>* row = a.seed + (size_t)b * S;
+        double fr = 0.0, fi = 0.0;
+        // ---- forward: psi, dpsi ----------------------------------------------------------
+        for (int k = 0; k < nb; k++) {
+            const TBlock* bk = a.blocks + k;
+            TileMap<PER> mp;
+            tile_map<KW, LT, PER>(bk, tid, &mp);
+            const int g0 = bk->g0, g1 = bk->g1;
+            for (uint32_t t = 0; t < ntiles; t++) {
+                const uint32_t ot = scatter_bits(t, bk->other, L - KW);
+#pragma unroll
+                for (int m = 0; m < PER; m++) {
+                    const uint32_t j = tid + m * T, gi = ot | mp.win_tid | mp.wm[m];
+                    if (k == 0) {
+                        sel<R, true>(t_psi, j) = {R(gi == (uint32_t)b), R(0)};
+                        if constexpr (MODE == 2) sel<R, true>(t_dpsi, j) = {R(0), R(0)};
+                    } else {
+                        sel<R, true>(t_psi, j) = g_psi[gi];
+                        if constexpr (MODE == 2) sel<R, true>(t_dpsi, j) = g_dpsi[gi];
+                    }
+                }
+                __syncthreads();
+                for (int g = g0; g < g1; g++) {
+                    const Inst in = a.inst[g];
+                    cpx<R> cf[24];
+                    load_coef<R, 8>(a.coef + (size_t)g * kCoefPerInst, cf);
+                    if constexpr (MODE == 2) load_coef<R, 8>(a.coef + (size_t)g * kCoefPerInst + 16, cf + 16);
+                    for (uint32_t q = tid; q < Qt; q += T) {
+                        const uint32_t base = quad_base(q, in.lo, in.hi);
+                        const Quad<R> p = load_quad<R, true>(t_psi, base, in.lo, in.hi);
+                        if constexpr (MODE == 2) {
+                            const Quad<R> d = load_quad<R, true>(t_dpsi, base, in.lo, in.hi);
+                            store_quad<R, true>(t_dpsi, base, in.lo, in.hi, gapply_add(cf, d, gapply(cf + 16, p)));
+                        }
+                        store_quad<R, true>(t_psi, base, in.lo, in.hi, gapply(cf, p));
+                    }
+                    __syncthreads();
+                }
+                // store (each thread its own positions, so the next tile's load needs no barrier);
+                // f_b = s_b . psi_G over the last block's tiles
+#pragma unroll
+                for (int m = 0; m < PER; m++) {
+                    const uint32_t j = tid + m * T, gi = ot | mp.win_tid | mp.wm[m];
+                    const cpx<R> p = sel<R, true>(t_psi, j);
+                    if (k == nb - 1) {
+                        const cpx<R> tt = cmul(row[gi], p);
+                        fr += (double)tt.re;
+                        fi += (double)tt.im;
+                        if constexpr (MODE == 0) continue;
+                    }
+                    g_psi[gi] = p;
+                    if constexpr (MODE == 2) g_dpsi[gi] = sel<R, true>(t_dpsi, j);
+                }
+            }
+        }
+        {
+            const double sr = wave_sum(fr), si = wave_sum(fi);
+            if (lane == 0) {
+                fst[2 * wv] = sr;
+                fst[2 * wv + 1] = si;
+            }
+            __syncthreads();
+            if (tid == 0) {
+                double tr = 0.0, ti = 0.0;
+                for (int w = 0; w < NW; w++) {
+                    tr += fst[2 * w];
+                    ti += fst[2 * w + 1];
+                }
+                fre += tr;
+                fim += ti;
+            }
+        }
+        if constexpr (MODE >= 1) {
+            // ---- reverse: lam (seeded), psi uncomputed, dlam, dpsi ---------------------------
+            for (int k = nb - 1; k >= 0; k--) {
+                const TBlock* bk = a.blocks + k;
+                TileMap<PER> mp;
+                tile_map<KW, LT, PER>(bk, tid, &mp);
+                const int g0 = bk->g0, g1 = bk->g1;
+                for (uint32_t t = 0; t < ntiles; t++) {
+                    const uint32_t ot = scatter_bits(t, bk->other, L - KW);
+#pragma unroll
+                    for (int m = 0; m < PER; m++) {
+                        const uint32_t j = tid + m * T, gi = ot | mp.win_tid | mp.wm[m];
+                        sel<R, true>(t_psi, j) = g_psi[gi];
+                        sel<R, true>(t_lam, j) = k == nb - 1 ? row[gi] : g_lam[gi];
+                        if constexpr (MODE == 2) {
+                            sel<R, true>(t_dpsi, j) = g_dpsi[gi];
+                            sel<R, true>(t_dlam, j) = k == nb - 1 ? cpx<R>{R(0), R(0)} : g_dlam[gi];
+                        }
+                    }
+                    __syncthreads();
+                    for (int g = g1 - 1; g >= g0; g--) {
+                        const Inst in = a.inst[g];
+                        cpx<R> cf[24];
+                        load_coef<R, MODE == 2 ? 24 : 16>(a.coef + (size_t)g * kCoefPerInst, cf);
+                        cpx<R> ga[8], ha[8];
+#pragma unroll
+                        for (int e = 0; e < 8; e++) {
+                            ga[e] = {R(0), R(0)};
+                            ha[e] = {R(0), R(0)};
+                        }
+                        for (uint32_t q = tid; q < Qt; q += T) {
+                            const uint32_t base = quad_base(q, in.lo, in.hi);
+                            const Quad<R> l = load_quad<R, true>(t_lam, base, in.lo, in.hi);
+                            const Quad<R> p = gapply(cf + 8, load_quad<R, true>(t_psi, base, in.lo, in.hi));
+                            store_quad<R, true>(t_psi, base, in.lo, in.hi, p);
+                            outer_acc(ga, l, p);
+                            if constexpr (MODE == 2) {
+                                const Quad<R> dn = load_quad<R, true>(t_dpsi, base, in.lo, in.hi);
+                                const Quad<R> dp = gapply(cf + 8, qsub(dn, gapply(cf + 16, p)));
+                                store_quad<R, true>(t_dpsi, base, in.lo, in.hi, dp);
+                                const Quad<R> dl = load_quad<R, true>(t_dlam, base, in.lo, in.hi);
+                                outer_acc(ha, dl, p);
+                                outer_acc(ha, l, dp);
+                                store_quad<R, true>(t_dlam, base, in.lo, in.hi, gapplyT_add(cf, dl, gapplyT(cf + 16, l)));
+                            }
+                            store_quad<R, true>(t_lam, base, in.lo, in.hi, gapplyT(cf, l));
+                        }
+                        R v[NV];
+#pragma unroll
+                        for (int e = 0; e < 8; e++) {
+                            v[2 * e] = ga[e].re;
+                            v[2 * e + 1] = ga[e].im;
+                            if constexpr (MODE == 2) {
+                                v[16 + 2 * e] = ha[e].re;
+                                v[16 + 2 * e + 1] = ha[e].im;
+                            }
+                        }
+                        wave_halve<NV, 32, R>(v, lane);
+                        const int buf = g & 1;
+                        if ((lane & ((1 << SHIFT) - 1)) == 0) stage[(buf * NW + wv) * NV + (lane >> SHIFT)] = (double)v[0];
+                        __syncthreads();
+                        if (wv == g % NW && lane < NV) {
+                            double s = 0.0;
+                            for (int w = 0; w < NW; w++) s += stage[(buf * NW + w) * NV + lane];
+                            const int ci = lane >> 1;
+                            int e = ci & 7;
+                            if (in.flip && e < 4) e = 3 - e;
+                            acc[32 * in.param + 2 * ((ci & 8) | e) + (lane & 1)] += s;
+                        }
+                    }
+                    if (k > 0) {
+#pragma unroll
+                        for (int m = 0; m < PER; m++) {
+                            const uint32_t j = tid + m * T, gi = ot | mp.win_tid | mp.wm[m];
+                            g_psi[gi] = sel<R, true>(t_psi, j);
+                            g_lam[gi] = sel<R, true>(t_lam, j);
+                            if constexpr (MODE == 2) {
+                                g_dpsi[gi] = sel<R, true>(t_dpsi, j);
+                                g_dlam[gi] = sel<R, true>(t_dlam, j);
+                            }
+                        }
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+    __syncthreads();
+    double* out = a.partial + (size_t)blockIdx.x * (2 + 32 * a.P);
+    if (tid == 0) {
+        out[0] = fre;
+        out[1] = fim;
+    }
+    for (int k = tid; k < 32 * a.P; k += T) out[2 + k] = acc[k];
+}
+
+int tile_window_bits(int fp32) { return fp32 ? tile_kw<float>() : tile_kw<double>(); }
+
+size_t tile_sweep_lds_bytes(int P, int kw, int fp32, int nstate, size_t* tile_off)
+{
+    const int NW = (1 << (fp32 ? tile_lt<float>() : tile_lt<double>())) / 64;
+    const size_t head = sizeof(double) * (32 * (size_t)P + 2 * NW * 32 + 2 * NW);
+    const size_t off = (head + 255) & ~(size_t)255;
+    if (tile_off) *tile_off = off;
+    return off + (size_t)nstate * ((size_t)1 << kw) * (fp32 ? sizeof(cpx<float>) : sizeof(cpx<double>));
+}
+
+template <typename R, int MODE>
+static hipError_t launch_tile_m(const TileArgs<R>& a, int grid, size_t lds, hipStream_t s)
+{
+    auto k = sweep_tile_kernel<R, MODE>;
+    if (lds > 65536) {
+        hipError_t e = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(k, dim3(grid), dim3(1 << tile_lt<R>()), lds, s, a);
+    return hipGetLastError();
+}
+
+template <typename R>
+hipError_t launch_sweep_tile(const TileArgs<R>& a, int mode, int grid, size_t lds, hipStream_t s)
+{
+    // the kernel's tile is 2^kw amplitudes, kw fixed per precision, and the column larger
+    if (a.kw != tile_kw<R>() || a.kw >= a.L || a.L > 30 || a.L - a.kw > 32 - kTileMaxBits) return hipErrorInvalidValue;
+    if (mode == 2) return launch_tile_m<R, 2>(a, grid, lds, s);
+    if (mode == 1) return launch_tile_m<R, 1>(a, grid, lds, s);
+    if (mode == 0) return launch_tile_m<R, 0>(a, grid, lds, s);
+    return hipErrorInvalidValue;
+}
+
+template hipError_t launch_sweep_tile<double>(const TileArgs<double>&, int, int, size_t, hipStream_t);
+template hipError_t launch_sweep_tile<float>(const TileArgs<float>&, int, int, size_t, hipStream_t);
+
+}  // namespace mg
