@@ -219,6 +219,9 @@ const char* mg_last_sweep_kernel(void);
 /* register-resident gate-window kernel (12 qubits, dense targets, unitary gates): -1 automatic
  * (used for FP64 sweeps where it applies), 0 never, 1 also for FP32 sweeps */
 void mg_set_window_kernel(int mode);
+/* tiled kernel for dense targets whose states exceed the LDS (L >= 13; FP64 L=12 for gates not
+ * unitary to 1e-12): mode -1 automatic (default), 0 never (global-scratch or register kernels) */
+void mg_set_tile_kernel(int mode);
 /* record HIP events around every sweep kernel on the library's stream */
 void mg_set_kernel_timing(int on);
 /* duration of the last sweep kernel in ms (needs mg_set_kernel_timing(1)) */

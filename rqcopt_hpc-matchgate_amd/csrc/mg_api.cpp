@@ -642,6 +642,8 @@ void mg_set_sweep_threads(int threads) { mg::knobs().lds_threads = threads; }
 int mg_last_sweep_threads(void) { return g_last_stats.reg_threads; }
 const char* mg_last_sweep_kernel(void) { return g_last_stats.kernel ? g_last_stats.kernel : ""; }
 void mg_set_window_kernel(int mode) { mg::knobs().force_win = mode; }
+
+void mg_set_tile_kernel(int mode) { mg::knobs().force_tile = mode != 0 ? -1 : 0; }
 float mg_last_kernel_ms(void) { return g_last_stats.kernel_ms; }
 void mg_last_sweep_info(int* lds_states, int* ckpt, int* grid, long* lds_bytes)
 {

@@ -123,6 +123,7 @@ _sig("mg_set_sweep_threads", [_I], None)
 _sig("mg_last_sweep_threads", [])
 _sig("mg_last_sweep_kernel", [], ctypes.c_char_p)
 _sig("mg_set_window_kernel", [_I], None)
+_sig("mg_set_tile_kernel", [_I], None)
 _sig("mg_last_kernel_ms", [], ctypes.c_float)
 _sig("mg_last_sweep_info", [_P, _P, _P, _P], None)
 _sig("mg_target_cache_clear", [], None)
@@ -519,6 +520,11 @@ def set_register_kernel(mode, threads=0):
 
 def set_window_kernel(mode):
     lib.mg_set_window_kernel(int(mode))
+
+
+def set_tile_kernel(mode):
+    """-1: tiled kernel where it applies (default), 0: never"""
+    lib.mg_set_tile_kernel(int(mode))
 
 
 def last_sweep_kernel():
