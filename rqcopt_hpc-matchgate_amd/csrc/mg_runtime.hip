@@ -391,6 +391,79 @@ static void make_windowed(const std::vector<Inst>& inst, const std::vector<cpx<d
         }
 }
 
+// tile blocks (12-qubit windows) and, per tile block, its 3-qubit window blocks, for the tiled
+// register-window kernel: instances in the final order (lo = window-local pair index),
+// coefficients, tile blocks, window blocks with LDS (swizzled tile-local) and global
+// (amplitude-index) offsets
+struct WindowedTiles {
+    std::vector<Inst> inst;
+    std::vector<cpx<double>> coef;
+    std::vector<WTBlock> tblocks;
+    std::vector<WBlock> blk_lds, blk_glob;
+};
+
+static bool make_windowed_tiles(const std::vector<Inst>& inst, const std::vector<cpx<double>>& coef, int L,
+                                WindowedTiles* w)
+{
+    constexpr int TB = 12, W = 3;
+    std::vector<int> order1;
+    std::vector<TBlock> tbs;
+    tile_blocks(inst, L, TB, &order1, &tbs);
+    if (order1.size() != inst.size()) return false;
+    w->inst.clear();
+    w->coef.clear();
+    w->tblocks.clear();
+    w->blk_lds.clear();
+    w->blk_glob.clear();
+    for (const TBlock& tb : tbs) {
+        auto local = [&](int q) { return (int)(std::find(tb.pos, tb.pos + TB, q) - tb.pos); };
+        std::vector<Inst> li;
+        for (int k = tb.g0; k < tb.g1; k++) {
+            Inst in = inst[order1[k]];
+            in.lo = local(in.lo);
+            in.hi = local(in.hi);
+            if (in.lo >= TB || in.hi >= TB) return false;
+            li.push_back(in);
+        }
+        std::vector<int> order2;
+        std::vector<WBlock> wbs;
+        window_blocks(li, TB, W, kWinRegGates, &order2, &wbs);
+        if (order2.size() != li.size()) return false;
+        WTBlock t{};
+        t.k0 = (int)w->blk_lds.size();
+        const int base = (int)w->inst.size();
+        for (int i = 0; i < TB; i++) t.tpos[i] = tb.pos[i];
+        for (int i = 0; i < L - TB; i++) t.other[i] = tb.other[i];
+        for (const WBlock& wb : wbs) {
+            for (int k = wb.g0; k < wb.g1; k++) {
+                const Inst& l = li[order2[k]];
+                Inst f = inst[order1[tb.g0 + order2[k]]];
+                f.lo = window_case(wb, W, l.lo, l.hi);
+                f.hi = 0;
+                w->inst.push_back(f);
+                const size_t src = (size_t)order1[tb.g0 + order2[k]] * kCoefPerInst;
+                w->coef.insert(w->coef.end(), coef.begin() + src, coef.begin() + src + kCoefPerInst);
+            }
+            WBlock bl = wb, bg = wb;
+            bl.g0 = bg.g0 = base + wb.g0;
+            bl.g1 = bg.g1 = base + wb.g1;
+            for (int j = 0; j < (1 << W); j++) {
+                uint32_t o = 0;
+                for (int i = 0; i < W; i++) o |= ((uint32_t)(j >> i) & 1u) << wb.pos[i];
+                bl.off[j] = win_swz<batch_bank_bits<double>()>(o);
+                uint32_t gl = 0;
+                for (int i = 0; i < TB; i++) gl |= ((o >> i) & 1u) << tb.pos[i];
+                bg.off[j] = gl;
+            }
+            w->blk_lds.push_back(bl);
+            w->blk_glob.push_back(bg);
+        }
+        t.k1 = (int)w->blk_lds.size();
+        w->tblocks.push_back(t);
+    }
+    return true;
+}
+
 // ---- evaluation -------------------------------------------------------------------------------
 template <typename R>
 static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, int mode, long b0, long b1, EvalOut* out,
@@ -471,6 +544,14 @@ static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, in
         use_tile = tile_lds <= c->lds_limit;
     }
     if (use_tile) reg_T = 0;
+    // tiled register-window kernel (FP64, unitary gates): preferred over the LDS tiles
+    bool use_wtile = false;
+    size_t wt_acc = 0, wt_buf = 0, wt_lds = 0;
+    int wt_rs = 0;
+    if (use_tile && !fp32 && b.defect <= 1e-12 && kn.force_win != 0 && L > 12 && L - 12 <= 20) {
+        wt_lds = win_sweep_lds_bytes(12, p.P, 0, c->lds_limit, &wt_acc, &wt_buf, &wt_rs);
+        use_wtile = wt_lds <= c->lds_limit;
+    }
     const int nw_reg = reg_T / 64;
     const size_t head_reg = reg_T ? sweep_lds_head_bytes(p.P, nw_reg) : 0;
     int reg_nst = 0;
@@ -480,6 +561,7 @@ static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, in
         if (reg_nst < 1) reg_T = 0;
     }
     const size_t lds = use_win ? win_lds
+                     : use_wtile ? wt_lds
                      : use_tile ? tile_lds
                      : reg_T ? head_reg + (size_t)reg_nst * S * sizeof(cpx<R>) : (lds_states ? head + state_bytes : head);
 
@@ -557,8 +639,46 @@ static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, in
         // the copies above read pageable host vectors: complete them before w goes away
         MG_HIP(hipStreamSynchronize(s));
     }
+    WinTileArgs wta{};
+    if (use_wtile) {
+        WindowedTiles w;
+        if (!make_windowed_tiles(b.inst, b.coef, L, &w)) {
+            set_error("tiled window sweep: gate blocking failed");
+            return -1;
+        }
+        MG_HIP(c->w_inst.ensure(w.inst.size() * sizeof(Inst)));
+        MG_HIP(c->w_coef.ensure(w.coef.size() * sizeof(cpx<double>)));
+        MG_HIP(c->w_blk.ensure(2 * w.blk_lds.size() * sizeof(WBlock) + w.tblocks.size() * sizeof(WTBlock)));
+        unsigned char* blk = (unsigned char*)c->w_blk.p;
+        MG_HIP(hipMemcpyAsync(c->w_inst.p, w.inst.data(), w.inst.size() * sizeof(Inst), hipMemcpyHostToDevice, s));
+        MG_HIP(hipMemcpyAsync(c->w_coef.p, w.coef.data(), w.coef.size() * sizeof(cpx<double>), hipMemcpyHostToDevice, s));
+        MG_HIP(hipMemcpyAsync(blk, w.blk_lds.data(), w.blk_lds.size() * sizeof(WBlock), hipMemcpyHostToDevice, s));
+        MG_HIP(hipMemcpyAsync(blk + w.blk_lds.size() * sizeof(WBlock), w.blk_glob.data(), w.blk_glob.size() * sizeof(WBlock),
+                              hipMemcpyHostToDevice, s));
+        MG_HIP(hipMemcpyAsync(blk + 2 * w.blk_lds.size() * sizeof(WBlock), w.tblocks.data(),
+                              w.tblocks.size() * sizeof(WTBlock), hipMemcpyHostToDevice, s));
+        wta.L = L;
+        wta.G = p.G;
+        wta.P = p.P;
+        wta.ntb = (int)w.tblocks.size();
+        wta.inst = (const Inst*)c->w_inst.p;
+        wta.coef = c->w_coef.p;
+        wta.tblocks = (const WTBlock*)(blk + 2 * w.blk_lds.size() * sizeof(WBlock));
+        wta.blk_lds = (const WBlock*)blk;
+        wta.blk_glob = (const WBlock*)(blk + w.blk_lds.size() * sizeof(WBlock));
+        wta.seed = t.seed64.p;
+        wta.b0 = b0;
+        wta.b1 = b1;
+        wta.partial = (double*)c->partial.p;
+        wta.scratch = c->scratch.p;
+        wta.nstate = nstate_tile(mode);
+        wta.acc_off = (int)wt_acc;
+        wta.buf_off = (int)wt_buf;
+        wta.rs = wt_rs;
+        MG_HIP(hipStreamSynchronize(s));
+    }
     TileArgs<R> ta{};
-    if (use_tile) {
+    if (use_tile && !use_wtile) {
         std::vector<int> order;
         std::vector<TBlock> blocks;
         tile_blocks(b.inst, L, tile_kw, &order, &blocks);
@@ -611,7 +731,9 @@ static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, in
         MG_HIP(hipStreamSynchronize(s));
     }
     if (kn.time_kernels) MG_HIP(hipEventRecord(c->ev0, s));
-    if (use_tile)
+    if (use_wtile)
+        MG_HIP(launch_sweep_win_tile(wta, mode, (int)grid, lds, s));
+    else if (use_tile)
         MG_HIP(launch_sweep_tile<R>(ta, mode, (int)grid, lds, s));
     else if (use_win)
         MG_HIP(launch_sweep_win(wa, mode, fp32, (int)grid, lds, s));
@@ -639,7 +761,11 @@ static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, in
         st->ckpt = ckpt;
         st->reg_threads = use_win ? 512 : reg_T;
         st->threads = use_win || use_tile ? 512 : reg_T ? reg_T : lds_T;  // the tiled kernel: 512 in both precisions
-        st->kernel = use_win ? "sweep_win_kernel" : use_tile ? "sweep_tile_kernel" : reg_T ? "sweep_reg_kernel" : "sweep_kernel";
+        st->kernel = use_win     ? "sweep_win_kernel"
+                     : use_wtile ? "sweep_win_tile_kernel"
+                     : use_tile  ? "sweep_tile_kernel"
+                     : reg_T     ? "sweep_reg_kernel"
+                                 : "sweep_kernel";
         st->grid = (int)grid;
         st->lds_bytes = lds;
         st->kernel_ms = 0.f;

@@ -154,6 +154,8 @@ __global__ __launch_bounds__(1 << tile_lt<R>()) void sweep_tile_kernel(TileArgs<
                     if constexpr (MODE == 2) g_dpsi[gi] = sel<R, true>(t_dpsi, j);
                 }
             }
+            // the next block reads, in its own mapping, what other threads stored here
+            __syncthreads();
         }
         {
             const double sr = wave_sum(fr), si = wave_sum(fi);
@@ -255,6 +257,8 @@ __global__ __launch_bounds__(1 << tile_lt<R>()) void sweep_tile_kernel(TileArgs<
                         }
                     }
                 }
+                // the next block reads, in its own mapping, what other threads stored here
+                __syncthreads();
             }
         }
         __syncthreads();

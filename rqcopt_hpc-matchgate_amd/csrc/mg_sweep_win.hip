@@ -308,6 +308,305 @@ __global__ __launch_bounds__(WinShape<R>::T) void sweep_win_kernel(WinSweepArgs 
     for (int k = tid0; k < 32 * a.P; k += kT) out[2 + k] = acc[k];
 }
 
+// ---- tiled variant (FP64, L > 12) ------------------------------------------------------------
+namespace {
+
+// bits of x scattered to the positions p[0..n) (wave-uniform positions, scalar reads)
+__device__ inline uint32_t scatter_c(uint32_t x, const int* p, int n)
+{
+    const MG_CONST int* q = cptr(p);
+    uint32_t r = 0;
+    for (int i = 0; i < n; i++) r |= ((x >> i) & 1u) << q[i];
+    return r;
+}
+
+}  // namespace
+
+template <int MODE, int RS>
+__global__ __launch_bounds__(512) void sweep_win_tile_kernel(WinTileArgs a)
+{
+    using R = double;
+    constexpr int kT = 512, kW = 3, kNW = kT / 64, TB = 12;
+    extern __shared__ __align__(16) unsigned char smem[];
+    constexpr int NV = MODE == 2 ? 32 : 16;
+    constexpr uint32_t TSZ = 1u << TB;  // tile amplitudes (the re-layout buffer's index space)
+    const int L = a.L;
+    const uint32_t S = 1u << L, ntiles = 1u << (L - TB);
+    const int tid0 = threadIdx.x, lane = tid0 & 63, wv = tid0 >> 6;
+
+    double* stage = reinterpret_cast<double*>(smem);  // [kWinRegGates][kNW][32]
+    double* fst = stage + kWinRegGates * kNW * 32;    // [kNW][2]
+    double* acc = reinterpret_cast<double*>(smem + a.acc_off);
+    cpx<R>* buf = reinterpret_cast<cpx<R>*>(smem + a.buf_off);  // [RS][TSZ] re-layout buffer
+    // the column's states in global memory: psi, dpsi (MODE 2), lam (MODE >= 1), dlam (MODE 2)
+    constexpr int LAM = MODE == 2 ? 2 : 1;
+    cpx<R>* sc = reinterpret_cast<cpx<R>*>(a.scratch) + (size_t)blockIdx.x * a.nstate * S;
+    cpx<R>* g_psi = sc;
+    cpx<R>* g_dpsi = sc + S;
+    cpx<R>* g_lam = sc + (size_t)LAM * S;
+    cpx<R>* g_dlam = sc + 3 * (size_t)S;
+    const cpx<R>* coef = reinterpret_cast<const cpx<R>*>(a.coef);
+    const cpx<R>* seed = reinterpret_cast<const cpx<R>*>(a.seed);
+
+    for (int k = tid0; k < 32 * a.P; k += kT) acc[k] = 0.0;
+    double fre = 0.0, fim = 0.0;
+    __syncthreads();
+
+    const int ntb = a.ntb;
+    for (long b = a.b0 + blockIdx.x; b < a.b1; b += gridDim.x) {
+        int tid = tid0;
+        asm volatile("" : "+v"(tid));
+        Win<R, kW> Lm, P, DL, D;
+        double fr = 0.0, fi = 0.0;
+        const cpx<R>* row = seed + (size_t)b * S;
+        // ---- forward: psi, dpsi, tile block by tile block ------------------------------------
+        for (int tb = 0; tb < ntb; tb++) {
+            const WTBlock* tbk = a.tblocks + tb;
+            const int k0 = cptr(tbk)->k0, k1 = cptr(tbk)->k1;
+            for (uint32_t t = 0; t < ntiles; t++) {
+                const uint32_t ot = scatter_c(t, tbk->other, L - TB);
+                {
+                    const BlockRegs<kW> bf = load_block<kW>(a.blk_glob + k0);
+                    const uint32_t gb = ot | scatter_c(deposit<kW>((uint32_t)tid, bf.pos), tbk->tpos, TB);
+                    if (tb == 0) {
+#pragma unroll
+                        for (int j = 0; j < (1 << kW); j++) {
+                            P.a[j] = {(gb ^ bf.off[j]) == (uint32_t)b ? R(1) : R(0), R(0)};
+                            D.a[j] = {R(0), R(0)};
+                        }
+                    } else {
+                        win_load<R, kW, false>(g_psi, gb, bf.off, P);
+                        if constexpr (MODE == 2) win_load<R, kW, false>(g_dpsi, gb, bf.off, D);
+                    }
+                }
+                for (int k = k0; k < k1; k++) {
+                    const BlockRegs<kW> bk = load_block<kW>(a.blk_lds + k);
+                    for (int g = bk.g0; g < bk.g1; g++) {
+                        const int wcase = cptr(a.inst)[g].lo;
+                        cpx<R> cf[24];
+                        load_coef_c<R, 8>(coef + (size_t)g * kCoefPerInst, cf);
+                        if constexpr (MODE == 2) load_coef_c<R, 8>(coef + (size_t)g * kCoefPerInst + 16, cf + 16);
+#define MG_FWD(WW, A, B) wg_forward<R, WW, A, B, MODE == 2 ? 2 : 0>(cf, P, D)
+                        MG_WCASES(kW, MG_FWD)
+#undef MG_FWD
+                    }
+                    if (k + 1 < k1) {
+                        const BlockRegs<kW> nx = load_block<kW>(a.blk_lds + k + 1);
+                        const uint32_t sb = lin<R, true>(deposit<kW>((uint32_t)tid, bk.pos));
+                        const uint32_t sn = lin<R, true>(deposit<kW>((uint32_t)tid, nx.pos));
+                        win_store<R, kW, true>(buf, sb, bk.off, P);
+                        if constexpr (MODE == 2 && RS >= 2) win_store<R, kW, true>(buf + TSZ, sb, bk.off, D);
+                        __syncthreads();
+                        win_load<R, kW, true>(buf, sn, nx.off, P);
+                        if constexpr (MODE == 2 && RS >= 2) win_load<R, kW, true>(buf + TSZ, sn, nx.off, D);
+                        __syncthreads();
+                        if constexpr (MODE == 2 && RS == 1) {
+                            win_store<R, kW, true>(buf, sb, bk.off, D);
+                            __syncthreads();
+                            win_load<R, kW, true>(buf, sn, nx.off, D);
+                            __syncthreads();
+                        }
+                    }
+                }
+                // back to global memory (the last tile block also forms f_b = s_b . psi_G)
+                {
+                    const BlockRegs<kW> bl = load_block<kW>(a.blk_glob + k1 - 1);
+                    const uint32_t gb = ot | scatter_c(deposit<kW>((uint32_t)tid, bl.pos), tbk->tpos, TB);
+                    if (tb == ntb - 1) {
+#pragma unroll
+                        for (int j = 0; j < (1 << kW); j++) {
+                            const cpx<R> tt = cmul(row[gb ^ bl.off[j]], P.a[j]);
+                            fr += tt.re;
+                            fi += tt.im;
+                        }
+                    }
+                    if (MODE >= 1 || tb < ntb - 1) {
+                        win_store<R, kW, false>(g_psi, gb, bl.off, P);
+                        if constexpr (MODE == 2) win_store<R, kW, false>(g_dpsi, gb, bl.off, D);
+                    }
+                }
+            }
+            // the next tile block reads, in its own mapping, what other threads stored here
+            __syncthreads();
+        }
+        {
+            const double sr = wave_sum(fr), si = wave_sum(fi);
+            if (lane == 0) {
+                fst[2 * wv] = sr;
+                fst[2 * wv + 1] = si;
+            }
+            __syncthreads();
+            if (tid == 0) {
+                double tr = 0.0, ti = 0.0;
+                for (int w = 0; w < kNW; w++) {
+                    tr += fst[2 * w];
+                    ti += fst[2 * w + 1];
+                }
+                fre += tr;
+                fim += ti;
+            }
+        }
+        if constexpr (MODE >= 1) {
+            // ---- reverse, tile block by tile block ------------------------------------------
+            for (int tb = ntb - 1; tb >= 0; tb--) {
+                const WTBlock* tbk = a.tblocks + tb;
+                const int k0 = cptr(tbk)->k0, k1 = cptr(tbk)->k1;
+                for (uint32_t t = 0; t < ntiles; t++) {
+                    const uint32_t ot = scatter_c(t, tbk->other, L - TB);
+                    {
+                        const BlockRegs<kW> bl = load_block<kW>(a.blk_glob + k1 - 1);
+                        const uint32_t gb = ot | scatter_c(deposit<kW>((uint32_t)tid, bl.pos), tbk->tpos, TB);
+                        win_load<R, kW, false>(g_psi, gb, bl.off, P);
+                        if constexpr (MODE == 2) win_load<R, kW, false>(g_dpsi, gb, bl.off, D);
+                        if (tb == ntb - 1) {
+#pragma unroll
+                            for (int j = 0; j < (1 << kW); j++) {
+                                Lm.a[j] = row[gb ^ bl.off[j]];
+                                DL.a[j] = {R(0), R(0)};
+                            }
+                        } else {
+                            win_load<R, kW, false>(g_lam, gb, bl.off, Lm);
+                            if constexpr (MODE == 2) win_load<R, kW, false>(g_dlam, gb, bl.off, DL);
+                        }
+                    }
+                    for (int k = k1 - 1; k >= k0; k--) {
+                        const BlockRegs<kW> bk = load_block<kW>(a.blk_lds + k);
+                        for (int g = bk.g1 - 1; g >= bk.g0; g--) {
+                            const int wcase = cptr(a.inst)[g].lo;
+                            double* slot = stage + ((g - bk.g0) * kNW + wv) * 32;
+                            cpx<R> V[8], Z[8];
+                            load_coef_c<R, 8>(coef + (size_t)g * kCoefPerInst, V);
+                            if constexpr (MODE == 2) load_coef_c<R, 8>(coef + (size_t)g * kCoefPerInst + 16, Z);
+                            {
+                                cpx<R> ga[8];
+#pragma unroll
+                                for (int e = 0; e < 8; e++) ga[e] = {R(0), R(0)};
+#define MG_PA(WW, A, B) pass_a<R, WW, A, B, MODE>(V, Z, Lm, P, D, ga)
+                                MG_WCASES(kW, MG_PA)
+#undef MG_PA
+                                stage8(ga, slot, lane);
+                            }
+                            if constexpr (MODE == 2) {
+                                cpx<R> ha[8];
+#pragma unroll
+                                for (int e = 0; e < 8; e++) ha[e] = {R(0), R(0)};
+#define MG_PB(WW, A, B) pass_b<R, WW, A, B>(Lm, P, DL, D, ha)
+                                MG_WCASES(kW, MG_PB)
+#undef MG_PB
+                                stage8(ha, slot + 16, lane);
+                            }
+#define MG_PC(WW, A, B) pass_c<R, WW, A, B, MODE>(V, Z, Lm, DL)
+                            MG_WCASES(kW, MG_PC)
+#undef MG_PC
+                        }
+                        // re-layout to the previous window block's order; the staged sums are
+                        // consumed after the first barrier
+                        const bool move = k > k0;
+                        BlockRegs<kW> pv = bk;
+                        if (move) pv = load_block<kW>(a.blk_lds + k - 1);
+                        const uint32_t sb = lin<R, true>(deposit<kW>((uint32_t)tid, bk.pos));
+                        const uint32_t sn = lin<R, true>(deposit<kW>((uint32_t)tid, pv.pos));
+                        if (move) {
+                            win_store<R, kW, true>(buf, sb, bk.off, Lm);
+                            if constexpr (RS >= 2) win_store<R, kW, true>(buf + TSZ, sb, bk.off, P);
+                        }
+                        __syncthreads();
+                        for (int g = bk.g0; g < bk.g1; g++) {
+                            if (wv == (g - bk.g0) % kNW && lane < NV) {
+                                const double* sg = stage + (size_t)(g - bk.g0) * kNW * 32;
+                                double s = 0.0;
+                                for (int w = 0; w < kNW; w++) s += sg[w * 32 + lane];
+                                const int gparam = cptr(a.inst)[g].param, gflip = cptr(a.inst)[g].flip;
+                                const int ci = lane >> 1;
+                                int e = ci & 7;
+                                if (gflip && e < 4) e = 3 - e;
+                                acc[32 * gparam + 2 * ((ci & 8) | e) + (lane & 1)] += s;
+                            }
+                        }
+                        if (move) {
+                            win_load<R, kW, true>(buf, sn, pv.off, Lm);
+                            if constexpr (RS >= 2) win_load<R, kW, true>(buf + TSZ, sn, pv.off, P);
+                            __syncthreads();
+                            if constexpr (RS == 1) {
+                                win_store<R, kW, true>(buf, sb, bk.off, P);
+                                __syncthreads();
+                                win_load<R, kW, true>(buf, sn, pv.off, P);
+                                __syncthreads();
+                            }
+                            if constexpr (MODE == 2 && RS == 2) {
+                                win_store<R, kW, true>(buf, sb, bk.off, DL);
+                                win_store<R, kW, true>(buf + TSZ, sb, bk.off, D);
+                                __syncthreads();
+                                win_load<R, kW, true>(buf, sn, pv.off, DL);
+                                win_load<R, kW, true>(buf + TSZ, sn, pv.off, D);
+                                __syncthreads();
+                            }
+                            if constexpr (MODE == 2 && RS == 1) {
+                                win_store<R, kW, true>(buf, sb, bk.off, DL);
+                                __syncthreads();
+                                win_load<R, kW, true>(buf, sn, pv.off, DL);
+                                __syncthreads();
+                                win_store<R, kW, true>(buf, sb, bk.off, D);
+                                __syncthreads();
+                                win_load<R, kW, true>(buf, sn, pv.off, D);
+                                __syncthreads();
+                            }
+                        } else {
+                            __syncthreads();
+                        }
+                    }
+                    // back to global memory in the first window block's order (the next,
+                    // lower tile block reads it)
+                    if (tb > 0) {
+                        const BlockRegs<kW> bf = load_block<kW>(a.blk_glob + k0);
+                        const uint32_t gb = ot | scatter_c(deposit<kW>((uint32_t)tid, bf.pos), tbk->tpos, TB);
+                        win_store<R, kW, false>(g_psi, gb, bf.off, P);
+                        win_store<R, kW, false>(g_lam, gb, bf.off, Lm);
+                        if constexpr (MODE == 2) {
+                            win_store<R, kW, false>(g_dpsi, gb, bf.off, D);
+                            win_store<R, kW, false>(g_dlam, gb, bf.off, DL);
+                        }
+                    }
+                }
+                // the next (lower) tile block reads, in its own mapping, what other threads stored
+                __syncthreads();
+            }
+        }
+        __syncthreads();
+    }
+    __syncthreads();
+    double* out = a.partial + (size_t)blockIdx.x * (2 + 32 * a.P);
+    if (tid0 == 0) {
+        out[0] = fre;
+        out[1] = fim;
+    }
+    for (int k = tid0; k < 32 * a.P; k += kT) out[2 + k] = acc[k];
+}
+
+template <int RS>
+static hipError_t launch_wtile_rs(const WinTileArgs& a, int mode, int grid, size_t lds, hipStream_t s)
+{
+    const void* k = mode == 2 ? (const void*)sweep_win_tile_kernel<2, RS>
+                  : mode == 1 ? (const void*)sweep_win_tile_kernel<1, RS> : (const void*)sweep_win_tile_kernel<0, RS>;
+    if (lds > 65536) {
+        hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    if (mode == 2) hipLaunchKernelGGL((sweep_win_tile_kernel<2, RS>), dim3(grid), dim3(512), lds, s, a);
+    else if (mode == 1) hipLaunchKernelGGL((sweep_win_tile_kernel<1, RS>), dim3(grid), dim3(512), lds, s, a);
+    else hipLaunchKernelGGL((sweep_win_tile_kernel<0, RS>), dim3(grid), dim3(512), lds, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_sweep_win_tile(const WinTileArgs& a, int mode, int grid, size_t lds, hipStream_t s)
+{
+    // 12-qubit tiles of a larger column, the other bits listed in WTBlock::other
+    if (a.L <= 12 || a.L - 12 > 20 || a.ntb < 1) return hipErrorInvalidValue;
+    if (a.rs == 2) return launch_wtile_rs<2>(a, mode, grid, lds, s);
+    if (a.rs == 1) return launch_wtile_rs<1>(a, mode, grid, lds, s);
+    return hipErrorInvalidValue;
+}
+
 size_t win_sweep_lds_bytes(int L, int P, int fp32, size_t limit, size_t* acc_off, size_t* buf_off, int* rs)
 {
     const int nw = (fp32 ? WinShape<float>::T : WinShape<double>::T) / 64;

@@ -20,6 +20,32 @@ struct WinSweepArgs {
     int rs;                    // states per re-layout round (FP32: 4; FP64: 2, or 1 for many parameters)
 };
 
+// Tiled variant (FP64, L > 12): the program is cut into tile blocks acting inside 12-qubit
+// windows (tile_blocks); a tile (the 2^12 amplitudes sharing the bits outside the window) is
+// loaded from the column's global slice into registers and swept through the tile block's own
+// 3-qubit window blocks exactly as the L=12 kernel sweeps a whole column, then stored back.
+struct WTBlock {
+    int k0, k1;                         // window blocks [k0, k1) of this tile block
+    int tpos[12];                       // the tile window's bit positions, ascending
+    int other[20];                      // the other L - 12 bit positions, ascending
+};
+
+struct WinTileArgs {
+    int L, G, P, ntb;
+    const Inst* inst;          // [G] in block order; lo = window-local pair index
+    const void* coef;          // cpx<double> [G][kCoefPerInst] in block order
+    const WTBlock* tblocks;    // [ntb]
+    const WBlock* blk_lds;     // window blocks: tile-local positions, offsets swizzled for the LDS buffer
+    const WBlock* blk_glob;    // the same blocks with the amplitude-index images of the offsets
+    const void* seed;          // cpx<double> [S][S]: row b = -conj(U e_b)
+    long b0, b1;               // column range
+    double* partial;           // [gridDim.x][2 + 32 P]
+    void* scratch;             // cpx<double> [gridDim.x][nstate][S]
+    int nstate;                // 1 (f), 2 (+ gradient), 4 (+ HVP)
+    int acc_off, buf_off, rs;  // as WinSweepArgs (12-qubit tiles)
+};
+hipError_t launch_sweep_win_tile(const WinTileArgs& a, int mode, int grid, size_t lds, hipStream_t s);
+
 // dynamic LDS of the kernel within `limit` bytes (and the accumulator and buffer offsets, and the
 // states per re-layout round that fit: FP64 re-lays two states out per round, or one when the
 // accumulator of many parameter gates leaves no room for two)

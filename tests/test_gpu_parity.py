@@ -244,13 +244,15 @@ def test_window_register_kernel_matches_oracle(mg, layout, depth, prec):
 
 
 @pytest.mark.parametrize("L,layout,depth,prec", [(13, "open", 11, "fp64"), (13, "tied", 4, "fp32"), (14, "open", 7, "fp64"),
-                                                  (13, "open", 15, "fp32"), (14, "open", 5, "fp32"), (12, "open", 5, "fp64")])
-def test_tile_kernel_matches_oracle(mg, L, layout, depth, prec):
-    """The tiled kernel (dense targets whose states exceed the LDS: 2^11-amplitude FP64 or
-    2^12-amplitude FP32 tiles per state, a few passes per sweep), all three modes on a column
-    subset against the oracle; at L=12 FP64 it takes the gates the register window kernel
-    does not (not unitary to 1e-12, but invertible)."""
-    say(f"case: tiled kernel L={L} {layout} depth={depth} {prec}")
+                                                  (13, "open", 15, "fp32"), (13, "open", 15, "fp64"),
+                                                  (14, "open", 5, "fp32"), (12, "open", 5, "fp64")])
+def test_tile_kernels_match_oracle(mg, L, layout, depth, prec):
+    """The tiled kernels for dense targets whose states exceed the LDS, all three modes on a
+    column subset against the oracle: FP64 unitary gates run the tiled register-window kernel
+    (12-qubit tiles swept in 3-qubit register windows; one state per re-layout round at depth 15)
+    and, with it switched off, the LDS-tile kernel; FP32 and the FP64 gates that are only
+    invertible (L=12, unitarity defect 2e-11) run the LDS-tile kernel (2^12 / 2^11 amplitudes)."""
+    say(f"case: tiled kernels L={L} {layout} depth={depth} {prec}")
     V, Z, param, wires, P, U = build_case(L, layout, depth, seed=L * 7 + depth)
     if L == 12:
         V = V * (1.0 + 1e-11)  # unitarity defect 2e-11: no register window kernel, no checkpoints
@@ -258,17 +260,25 @@ def test_tile_kernel_matches_oracle(mg, L, layout, depth, prec):
     p = mg.Program.create(L, P, param, wires)
     precision = mg.MG_FP64 if prec == "fp64" else mg.MG_FP32
     c0, c1 = 3, 3 + 24
-    r = mg.evaluate_columns(t, p, V, Z, mg.EVAL_F_GRAD_HVP, precision, c0, c1)
-    info = mg.last_sweep_info()
-    assert info["kernel"] == "sweep_tile_kernel", info
-    r1 = mg.evaluate_columns(t, p, V, None, mg.EVAL_F_GRAD, precision, c0, c1)
-    # f + gradient needs two states, which fit the LDS at FP64 L=12 and FP32 L=13
-    assert mg.last_sweep_info()["kernel"] == "sweep_tile_kernel" or L == 12 or (L == 13 and prec == "fp32")
-    r0 = mg.evaluate_columns(t, p, V, None, mg.EVAL_F, precision, c0, c1)
     fo, dgo, dho = orc.target_gradient_hvp(U, V, Z, param, wires, L, c0, c1)
     tol = 1e-11 if prec == "fp64" else 2e-5
-    assert abs(r[0] - fo) / abs(fo) < tol and rel(r[1], dgo) < tol and rel(r[2], dho) < tol
-    assert abs(r1[0] - fo) / abs(fo) < tol and rel(r1[1], dgo) < tol and abs(r0[0] - fo) / abs(fo) < tol
+    kernels = ["sweep_win_tile_kernel", "sweep_tile_kernel"] if prec == "fp64" and L > 12 else ["sweep_tile_kernel"]
+    results = []
+    for kname in kernels:
+        mg.set_window_kernel(0 if kname == "sweep_tile_kernel" else -1)
+        try:
+            r = mg.evaluate_columns(t, p, V, Z, mg.EVAL_F_GRAD_HVP, precision, c0, c1)
+            assert mg.last_sweep_info()["kernel"] == kname, (kname, mg.last_sweep_info())
+            r1 = mg.evaluate_columns(t, p, V, None, mg.EVAL_F_GRAD, precision, c0, c1)
+            # f + gradient needs two states, which fit the LDS at FP64 L=12 and FP32 L=13
+            k1 = mg.last_sweep_info()["kernel"]
+            assert k1 == kname or L == 12 or (L == 13 and prec == "fp32"), k1
+            r0 = mg.evaluate_columns(t, p, V, None, mg.EVAL_F, precision, c0, c1)
+        finally:
+            mg.set_window_kernel(-1)
+        assert abs(r[0] - fo) / abs(fo) < tol and rel(r[1], dgo) < tol and rel(r[2], dho) < tol, kname
+        assert abs(r1[0] - fo) / abs(fo) < tol and rel(r1[1], dgo) < tol and abs(r0[0] - fo) / abs(fo) < tol, kname
+        results.append(r)
     # the same columns through the kernels without tiles (global-scratch or register states)
     mg.set_lds_states(0)
     try:
@@ -277,7 +287,8 @@ def test_tile_kernel_matches_oracle(mg, L, layout, depth, prec):
     finally:
         mg.set_lds_states(-1)
     tol_g = 1e-12 if prec == "fp64" else 2e-5
-    assert abs(r[0] - g[0]) / abs(g[0]) < tol_g and rel(r[1], g[1]) < tol_g and rel(r[2], g[2]) < tol_g
+    for r in results:
+        assert abs(r[0] - g[0]) / abs(g[0]) < tol_g and rel(r[1], g[1]) < tol_g and rel(r[2], g[2]) < tol_g
 
 
 def test_register_kernel_brickwall_target(mg):
