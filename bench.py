@@ -100,8 +100,10 @@ def served_bench(mg, njobs, seed=1):
     """The served stream: Poisson arrivals (200 jobs/s, seed 1) of trust-region jobs with
     L in {8, 10, 12}, odd depth 3..15, budget 5..50 iterations, circuit targets of 3x the
     depth; FP32 sweeps with the last 3 iterations in FP64, continuous batching, targets
-    materialised at admission; against the same stream run one job at a time in FP64 with
-    one launch per evaluation."""
+    materialised at admission, each job's trust region reusing its own products (tr_reuse);
+    against the same stream run one job at a time in FP64 with one launch per evaluation and the
+    reference trust-region control flow (and, to separate the gains, one job at a time with
+    tr_reuse)."""
     from rqcopt_mg.workloads import served_stream
 
     stream = served_stream(njobs, seed=seed, rate=200.0, Ls=(8, 10, 12))
@@ -120,13 +122,16 @@ def served_bench(mg, njobs, seed=1):
     mg.set_kernel_timing(True)
     log(f"serving {njobs} jobs with continuous batching ...")
     res, st = mg.serve(jobs, batched=True, precision=mg.MG_FP32, precision_switch=3, densify_qubits=12,
-                       realtime=True)
+                       realtime=True, tr_reuse=True)
     log(f"  {njobs / st['wall_s']:.2f} jobs/s, {st['rounds']} launches for {st['requests']} evaluations")
     log("the same stream one job at a time (FP64, one launch per evaluation, targets materialised) ...")
     base, stb = mg.serve(jobs, batched=False, precision=mg.MG_FP64, realtime=True, densify_qubits=12)
     log("the same stream one job at a time (FP64, one launch per evaluation, circuit targets as given) ...")
     plain, stp = mg.serve(jobs, batched=False, precision=mg.MG_FP64, realtime=True, densify_qubits=0)
+    log("the same stream one job at a time (FP64, targets materialised, trust region with tr_reuse) ...")
+    reuse1, stu = mg.serve(jobs, batched=False, precision=mg.MG_FP64, realtime=True, densify_qubits=12, tr_reuse=True)
     lat = np.array([r["latency"] for r in res]) * 1e3
+    latu = np.array([r["latency"] for r in reuse1]) * 1e3
     latb = np.array([r["latency"] for r in base]) * 1e3
     latp = np.array([r["latency"] for r in plain]) * 1e3
     # error of each job's final evaluation (FP64 sweeps in the last iterations) and of an FP32
@@ -144,7 +149,7 @@ def served_bench(mg, njobs, seed=1):
         err["grad_fp32_sweep"] = max(err["grad_fp32_sweep"], float(np.max(np.abs(o32[1] - g0)) / gn))
         err["hvp_fp32_sweep"] = max(err["hvp_fp32_sweep"], float(np.max(np.abs(o32[2] - h0)) / hn))
         err["f_vs_fp64_path"] = max(err["f_vs_fp64_path"], float(abs(r["f_iter"][-1] - b["f_iter"][-1]) / abs(b["f_iter"][-1])))
-    jps, jpsb, jpsp = njobs / st["wall_s"], njobs / stb["wall_s"], njobs / stp["wall_s"]
+    jps, jpsb, jpsp, jpsu = njobs / st["wall_s"], njobs / stb["wall_s"], njobs / stp["wall_s"], njobs / stu["wall_s"]
     return {"jobs": njobs, "jobs_per_s": jps, "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
             "launches": st["rounds"], "evaluations": st["requests"], "peak_admitted": st["max_active"],
             "kernel_s": st["kernel_ms"] * 1e-3, "wall_s": st["wall_s"], "graph_replays": st["graph_replays"],
@@ -154,10 +159,17 @@ def served_bench(mg, njobs, seed=1):
             "baseline_fp64_one_job_dense_targets": {"jobs_per_s": jpsb, "p50_ms": float(np.percentile(latb, 50)),
                                                     "p99_ms": float(np.percentile(latb, 99)),
                                                     "note": "targets materialised at admission, best single-job kernels"},
+            "baseline_fp64_one_job_dense_targets_tr_reuse": {
+                "jobs_per_s": jpsu, "p50_ms": float(np.percentile(latu, 50)), "p99_ms": float(np.percentile(latu, 99)),
+                "evaluations": stu["requests"],
+                "note": "as above with the served trust region (H eta from the truncated CG, f+grad kept on rejection)"},
+            "evaluations_reference_control_flow": stb["requests"],
+            "hvp_sweeps": int(sum(r["nhvp"] for r in res)), "hvp_sweeps_reference_control_flow": int(sum(b["nhvp"] for b in base)),
             "speedup_vs_plain": jps / jpsp, "speedup_vs_dense_targets": jps / jpsb,
+            "speedup_vs_dense_targets_tr_reuse": jps / jpsu,
             "rel_err_vs_fp64": err, "autotune": tune,
             "workload": "Poisson 200 jobs/s seed 1, L in {8,10,12}, odd depth 3..15, 5..50 iterations, "
-                        "circuit targets of 3x depth; FP32 sweeps, last 3 iterations FP64"}
+                        "circuit targets of 3x depth; FP32 sweeps, last 3 iterations FP64; trust region with tr_reuse"}
 
 
 def batched_bench(mg, niter, seed=0):

@@ -149,7 +149,7 @@ int mg_batch_autotune(mg_request* reqs, int n, int precision, int reps, int* thr
 
 /* ---- serving: a stream of trust-region optimisation jobs ----
  * Each job runs riemannian_trust_region_optimize_hvp with set_rtr_default_params(16 nparams)
- * on its circuit and target.  batched = 1: jobs are admitted as they arrive (and as slots and
+ * on its circuit and target (with tr_reuse, the variant described at mg_serve_options).  batched = 1: jobs are admitted as they arrive (and as slots and
  * device memory allow) and all admitted jobs' evaluations are merged into shared launches;
  * batched = 0: jobs run one at a time, one launch per evaluation.  The target is created on
  * the device at admission and freed when the job ends. */
@@ -182,6 +182,14 @@ typedef struct mg_serve_options {
                              rows afterwards; 0: always applied gate by gate per column */
     double progress_s;    /* > 0: after a launch round (or a finished job, one job at a time), print a
                              progress line to stderr if this many seconds passed since the last one */
+    int tr_reuse;         /* 1: each job's trust region reuses what it evaluates: H eta of the ratio
+                             test is summed from the truncated CG's own products (one HVP per
+                             iteration fewer; equal up to rounding); f at a trial point is evaluated
+                             with its gradient while iterations remain, and an iteration that starts
+                             at gates evaluated last (an accepted trial point, or the same iterate
+                             after a rejected step; same precision) takes f + gradient from there (one
+                             evaluation per iteration fewer); 0: the control flow of
+                             riemannian_trust_region_optimize_hvp */
 } mg_serve_options;
 
 typedef struct mg_serve_stats {

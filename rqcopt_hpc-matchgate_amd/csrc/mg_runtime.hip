@@ -897,6 +897,54 @@ int target_densify(Target& t)
     const std::vector<WBlock>& bl = in_lds ? w.blk_lds : w.blk_glob;
     MG_HIP(hipMemcpy(db.p, bl.data(), bl.size() * sizeof(WBlock), hipMemcpyHostToDevice));
     if (t.seed64.ensure(S * S * sizeof(cpx<double>)) != hipSuccess) { set_error("out of device memory for the target"); return -1; }
+    // beyond the LDS: 12-qubit tiles of each row in registers (the tiled window kernel's
+    // forward sweep), unless the knob asks for the windowed in-place rows
+    if (!in_lds && L > 12 && L - 12 <= 20 && knobs().force_win != 0) {
+        WindowedTiles wt;
+        if (!make_windowed_tiles(t.tinst_h, t.tcoef_h, L, &wt)) {
+            set_error("target materialisation: gate blocking failed");
+            return -1;
+        }
+        DevBuf ti, tc, tb;
+        MG_HIP(ti.ensure(wt.inst.size() * sizeof(Inst)));
+        MG_HIP(tc.ensure(wt.coef.size() * sizeof(cpx<double>)));
+        const size_t nb = wt.blk_lds.size() * sizeof(WBlock);
+        MG_HIP(tb.ensure(2 * nb + wt.tblocks.size() * sizeof(WTBlock)));
+        unsigned char* blk = (unsigned char*)tb.p;
+        MG_HIP(hipMemcpy(ti.p, wt.inst.data(), wt.inst.size() * sizeof(Inst), hipMemcpyHostToDevice));
+        MG_HIP(hipMemcpy(tc.p, wt.coef.data(), wt.coef.size() * sizeof(cpx<double>), hipMemcpyHostToDevice));
+        MG_HIP(hipMemcpy(blk, wt.blk_lds.data(), nb, hipMemcpyHostToDevice));
+        MG_HIP(hipMemcpy(blk + nb, wt.blk_glob.data(), nb, hipMemcpyHostToDevice));
+        MG_HIP(hipMemcpy(blk + 2 * nb, wt.tblocks.data(), wt.tblocks.size() * sizeof(WTBlock), hipMemcpyHostToDevice));
+        size_t acc_off = 0, buf_off = 0;
+        int rs = 0;
+        win_sweep_lds_bytes(12, 0, 0, c->lds_limit, &acc_off, &buf_off, &rs);
+        const size_t lds = buf_off + ((size_t)1 << 12) * sizeof(cpx<double>);
+        WinTileArgs a{};
+        a.L = L;
+        a.G = (int)wt.inst.size();
+        a.P = 0;
+        a.ntb = (int)wt.tblocks.size();
+        a.inst = (const Inst*)ti.p;
+        a.coef = tc.p;
+        a.tblocks = (const WTBlock*)(blk + 2 * nb);
+        a.blk_lds = (const WBlock*)blk;
+        a.blk_glob = (const WBlock*)(blk + nb);
+        a.b0 = 0;
+        a.b1 = (long)S;
+        a.nstate = 1;
+        a.acc_off = (int)acc_off;
+        a.buf_off = (int)buf_off;
+        a.rs = 1;
+        a.rows = t.seed64.p;
+        const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(2, c->lds_limit / lds));
+        const int grid = (int)std::min<long>((long)S, (long)c->ncu * per_cu);
+        MG_HIP(launch_materialize_win_tile(a, grid, lds, c->s));
+        MG_HIP(hipStreamSynchronize(c->s));
+        t.kind = 0;
+        t.seed32_ready = false;
+        return 0;
+    }
     if (in_lds) {
         const int grid = (int)std::max<long>(1, std::min<long>((long)(S / (256 / cgw)), (long)c->ncu * 2));
         MG_HIP(launch_materialize(L, (const Inst*)di.p, (const cpx<double>*)dc.p, (const WBlock*)db.p, (int)bl.size(),

@@ -63,6 +63,16 @@ struct JobRt {
     std::unique_ptr<mg::Target> target;
     size_t footprint = 0;
     int precision = MG_FP64, switch_last = 0, niter = 0, iter = -1, nhvp = 0, failed = 0;
+    bool reuse = false;
+    // the last two evaluations of f (+ Riemannian gradient): gates, precision, results
+    struct Kept {
+        std::vector<double> x, grad;
+        int prec = -1;
+        bool has_grad = false;
+        mg::zd f{0.0, 0.0};
+    };
+    Kept kept[2];
+    int kept_next = 0;
     std::thread th;
     bool finished = false;
 };
@@ -114,14 +124,68 @@ int job_eval(JobRt* j, const zd* gates, const zd* Z, int mode, mg::EvalOut* out)
     return p.status;
 }
 
+// with reuse: f (+ Riemannian gradient) at x from one of the last two evaluations when it had
+// the same gates and precision (bitwise: evaluations are deterministic)
+const JobRt::Kept* job_kept(const JobRt* j, const double* x, bool need_grad)
+{
+    if (!j->reuse) return nullptr;
+    const size_t m = 16 * (size_t)j->spec->program->p.P;
+    for (const JobRt::Kept& k : j->kept)
+        if ((k.has_grad || !need_grad) && k.prec == job_precision(j) && k.x.size() == m &&
+            std::memcmp(k.x.data(), x, m * sizeof(double)) == 0)
+            return &k;
+    return nullptr;
+}
+
+JobRt::Kept& job_keep(JobRt* j, const double* x, const mg::zd& f)
+{
+    JobRt::Kept& k = j->kept[j->kept_next];
+    j->kept_next ^= 1;
+    k.x.assign(x, x + 16 * (size_t)j->spec->program->p.P);
+    k.prec = job_precision(j);
+    k.f = f;
+    k.has_grad = false;
+    return k;
+}
+
+// f + Riemannian gradient at x (one evaluation), kept
+const JobRt::Kept* job_f_grad(JobRt* j, const double* x)
+{
+    const int P = j->spec->program->p.P;
+    mg::EvalOut out;
+    if (job_eval(j, reinterpret_cast<const zd*>(x), nullptr, mg::kFGrad, &out) < 0) return nullptr;
+    JobRt::Kept& k = job_keep(j, x, out.f);
+    k.grad.resize(8 * (size_t)P);
+    riemannian_gradient(reinterpret_cast<const struct matchgate*>(x), P, out.grad.data(), k.grad.data());
+    k.has_grad = true;
+    return &k;
+}
+
 double job_f(const double* x, void* fdata, numeric* f_complex)
 {
     JobRt* j = (JobRt*)fdata;
+    if (const JobRt::Kept* k = job_kept(j, x, false)) {
+        *f_complex = k->f;
+        return k->f.real();
+    }
+    // with reuse, f at a trial point comes with its gradient while iterations remain: the
+    // next iteration starts there when the step is accepted
+    if (j->reuse && j->iter + 1 < j->niter) {
+        const JobRt::Kept* k = job_f_grad(j, x);
+        if (!k) {
+            j->failed = 1;
+            return NAN;
+        }
+        *f_complex = k->f;
+        return k->f.real();
+    }
     mg::EvalOut out;
     if (job_eval(j, reinterpret_cast<const zd*>(x), nullptr, mg::kF, &out) < 0) {
         j->failed = 1;
         return NAN;
     }
+    // the final f after the last iteration is asked at these gates when the step is accepted
+    if (j->reuse) job_keep(j, x, out.f);
     *f_complex = out.f;
     return out.f.real();
 }
@@ -130,12 +194,25 @@ double job_f_deriv(const double* x, void* fdata, double* grad)
 {
     JobRt* j = (JobRt*)fdata;
     j->iter++;
+    const int P = j->spec->program->p.P;
+    // after a rejected step the trust region asks again at the same gates; after an
+    // accepted one, at the trial point whose f came with its gradient
+    if (j->reuse) {
+        const JobRt::Kept* k = job_kept(j, x, true);
+        if (!k) k = job_f_grad(j, x);
+        if (!k) {
+            j->failed = 1;
+            return NAN;
+        }
+        std::memcpy(grad, k->grad.data(), 8 * (size_t)P * sizeof(double));
+        return k->f.real();
+    }
     mg::EvalOut out;
     if (job_eval(j, reinterpret_cast<const zd*>(x), nullptr, mg::kFGrad, &out) < 0) {
         j->failed = 1;
         return NAN;
     }
-    riemannian_gradient(reinterpret_cast<const struct matchgate*>(x), j->spec->program->p.P, out.grad.data(), grad);
+    riemannian_gradient(reinterpret_cast<const struct matchgate*>(x), P, out.grad.data(), grad);
     return out.f.real();
 }
 
@@ -171,9 +248,9 @@ void run_job(JobRt* j)
     int P = s->program->p.P;
     struct rtr_params prm;
     set_rtr_default_params(16 * P, &prm);
-    riemannian_trust_region_optimize_hvp(job_f, job_f_deriv, job_hvp, j, job_retract, &P, 8 * P,
-                                         reinterpret_cast<const double*>(s->gates_start), 16 * P, &prm, s->niter,
-                                         s->f_iter, reinterpret_cast<double*>(s->gates_opt));
+    mg::rtr_optimize_hvp(job_f, job_f_deriv, job_hvp, j, job_retract, &P, 8 * P,
+                         reinterpret_cast<const double*>(s->gates_start), 16 * P, &prm, s->niter, s->f_iter,
+                         reinterpret_cast<double*>(s->gates_opt), j->reuse);
     s->nhvp = j->nhvp;
     s->status = j->failed ? -1 : 0;
 }
@@ -430,6 +507,7 @@ int mg_serve(mg_job* jobs, int njobs, const mg_serve_options* opt, mg_serve_stat
         j->precision = opt->precision == MG_FP32 ? MG_FP32 : MG_FP64;
         j->switch_last = std::max(0, opt->precision_switch);
         j->niter = jobs[i].niter;
+        j->reuse = opt->tr_reuse != 0;
         j->footprint = job_footprint(jobs[i], opt->densify_qubits);
         jobs[i].t_admit = now_s();
         j->target.reset(make_target(jobs[i], opt->densify_qubits));

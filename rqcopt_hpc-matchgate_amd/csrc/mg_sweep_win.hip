@@ -322,6 +322,11 @@ __device__ inline uint32_t scatter_c(uint32_t x, const int* p, int n)
 
 }  // namespace
 
+// Columns of 2^L > 2^12 amplitudes in 12-qubit tiles (512 threads x 8 amplitudes in
+// registers), each tile block's gates applied in 3-qubit register windows.  MODE 0/1/2: f,
+// + gradient, + HVP against the dense target rows (states between tile blocks in the
+// per-workgroup scratch).  MODE 3: materialise a target program, row b of a.rows =
+// -conj(W e_b), each row doubling as its column's state between tile blocks.
 template <int MODE, int RS>
 __global__ __launch_bounds__(512) void sweep_win_tile_kernel(WinTileArgs a)
 {
@@ -358,7 +363,8 @@ __global__ __launch_bounds__(512) void sweep_win_tile_kernel(WinTileArgs a)
         asm volatile("" : "+v"(tid));
         Win<R, kW> Lm, P, DL, D;
         double fr = 0.0, fi = 0.0;
-        const cpx<R>* row = seed + (size_t)b * S;
+        const cpx<R>* row = MODE == 3 ? nullptr : seed + (size_t)b * S;
+        cpx<R>* f_psi = MODE == 3 ? reinterpret_cast<cpx<R>*>(a.rows) + (size_t)b * S : g_psi;
         // ---- forward: psi, dpsi, tile block by tile block ------------------------------------
         for (int tb = 0; tb < ntb; tb++) {
             const WTBlock* tbk = a.tblocks + tb;
@@ -375,7 +381,7 @@ __global__ __launch_bounds__(512) void sweep_win_tile_kernel(WinTileArgs a)
                             D.a[j] = {R(0), R(0)};
                         }
                     } else {
-                        win_load<R, kW, false>(g_psi, gb, bf.off, P);
+                        win_load<R, kW, false>(f_psi, gb, bf.off, P);
                         if constexpr (MODE == 2) win_load<R, kW, false>(g_dpsi, gb, bf.off, D);
                     }
                 }
@@ -412,7 +418,13 @@ __global__ __launch_bounds__(512) void sweep_win_tile_kernel(WinTileArgs a)
                 {
                     const BlockRegs<kW> bl = load_block<kW>(a.blk_glob + k1 - 1);
                     const uint32_t gb = ot | scatter_c(deposit<kW>((uint32_t)tid, bl.pos), tbk->tpos, TB);
-                    if (tb == ntb - 1) {
+                    if constexpr (MODE == 3) {
+                        if (tb == ntb - 1) {
+#pragma unroll
+                            for (int j = 0; j < (1 << kW); j++) P.a[j] = {-P.a[j].re, P.a[j].im};
+                        }
+                        win_store<R, kW, false>(f_psi, gb, bl.off, P);
+                    } else if (tb == ntb - 1) {
 #pragma unroll
                         for (int j = 0; j < (1 << kW); j++) {
                             const cpx<R> tt = cmul(row[gb ^ bl.off[j]], P.a[j]);
@@ -420,7 +432,7 @@ __global__ __launch_bounds__(512) void sweep_win_tile_kernel(WinTileArgs a)
                             fi += tt.im;
                         }
                     }
-                    if (MODE >= 1 || tb < ntb - 1) {
+                    if (MODE == 1 || MODE == 2 || (MODE == 0 && tb < ntb - 1)) {
                         win_store<R, kW, false>(g_psi, gb, bl.off, P);
                         if constexpr (MODE == 2) win_store<R, kW, false>(g_dpsi, gb, bl.off, D);
                     }
@@ -429,6 +441,7 @@ __global__ __launch_bounds__(512) void sweep_win_tile_kernel(WinTileArgs a)
             // the next tile block reads, in its own mapping, what other threads stored here
             __syncthreads();
         }
+        if constexpr (MODE == 3) continue;
         {
             const double sr = wave_sum(fr), si = wave_sum(fi);
             if (lane == 0) {
@@ -446,7 +459,7 @@ __global__ __launch_bounds__(512) void sweep_win_tile_kernel(WinTileArgs a)
                 fim += ti;
             }
         }
-        if constexpr (MODE >= 1) {
+        if constexpr (MODE == 1 || MODE == 2) {
             // ---- reverse, tile block by tile block ------------------------------------------
             for (int tb = ntb - 1; tb >= 0; tb--) {
                 const WTBlock* tbk = a.tblocks + tb;
@@ -574,6 +587,7 @@ __global__ __launch_bounds__(512) void sweep_win_tile_kernel(WinTileArgs a)
         }
         __syncthreads();
     }
+    if constexpr (MODE == 3) return;
     __syncthreads();
     double* out = a.partial + (size_t)blockIdx.x * (2 + 32 * a.P);
     if (tid0 == 0) {
@@ -581,6 +595,18 @@ __global__ __launch_bounds__(512) void sweep_win_tile_kernel(WinTileArgs a)
         out[1] = fim;
     }
     for (int k = tid0; k < 32 * a.P; k += kT) out[2 + k] = acc[k];
+}
+
+hipError_t launch_materialize_win_tile(const WinTileArgs& a, int grid, size_t lds, hipStream_t s)
+{
+    if (a.L <= 12 || a.L - 12 > 20 || a.ntb < 1 || a.rs != 1 || !a.rows) return hipErrorInvalidValue;
+    const void* k = (const void*)sweep_win_tile_kernel<3, 1>;
+    if (lds > 65536) {
+        hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL((sweep_win_tile_kernel<3, 1>), dim3(grid), dim3(512), lds, s, a);
+    return hipGetLastError();
 }
 
 template <int RS>

@@ -43,8 +43,12 @@ struct WinTileArgs {
     void* scratch;             // cpx<double> [gridDim.x][nstate][S]
     int nstate;                // 1 (f), 2 (+ gradient), 4 (+ HVP)
     int acc_off, buf_off, rs;  // as WinSweepArgs (12-qubit tiles)
+    void* rows;                // materialisation: cpx<double> [S][S] output rows (column b's state)
 };
 hipError_t launch_sweep_win_tile(const WinTileArgs& a, int mode, int grid, size_t lds, hipStream_t s);
+// rows[b] = -conj(W e_b) for the columns [b0, b1) of the program in `a` (the target's gates,
+// P = 0, one state per re-layout round: rs = 1); seed, partial and scratch are not used
+hipError_t launch_materialize_win_tile(const WinTileArgs& a, int grid, size_t lds, hipStream_t s);
 
 // dynamic LDS of the kernel within `limit` bytes (and the accumulator and buffer offsets, and the
 // states per re-layout round that fit: FP64 re-lays two states out per round, or one when the

@@ -5,6 +5,8 @@
 // src/trust_region.c:19-306; the HVP variant implements the optimiser that the
 // reference declares (src/trust_region.h:84-86) but leaves commented out
 // (src/trust_region.c:319-381).  Dense level-1/2 kernels are plain loops (no CBLAS).
+// mg::rtr_optimize_hvp adds the option to take H eta for the ratio test from the truncated
+// CG's own products (one Hessian-vector product per iteration fewer).
 #include <cassert>
 #include <cmath>
 #include <cstdio>
@@ -13,6 +15,7 @@
 #include <vector>
 
 #include "../../include/trust_region.h"
+#include "mg_host.hpp"
 
 namespace {
 
@@ -54,14 +57,16 @@ double boundary_step(const double* b, const double* d, int n, double radius)
 }
 
 // minimise <grad, z> + 1/2 <z, H z> subject to ||z|| <= radius; returns true when
-// the iterate was moved to the boundary
+// the iterate was moved to the boundary.  Hz (nullable) receives H z, accumulated from the
+// products H d of the CG directions with the same steps as z.
 bool steihaug(const double* grad, const std::function<void(const double*, double*)>& hv, int n, double radius,
-              const truncated_cg_params* prm, double* z)
+              const truncated_cg_params* prm, double* z, double* Hz = nullptr)
 {
     std::vector<double> r(grad, grad + n), d(n), Hd(n);
     double rsq = dot(n, r.data(), r.data());
     const double stoptol = std::fmax(prm->abstol, prm->reltol * std::sqrt(rsq));
     std::memset(z, 0, n * sizeof(double));
+    if (Hz) std::memset(Hz, 0, n * sizeof(double));
     for (int k = 0; k < n; k++) d[k] = -r[k];
     for (int it = 0; it < prm->maxiter; it++) {
         hv(d.data(), Hd.data());
@@ -70,10 +75,14 @@ bool steihaug(const double* grad, const std::function<void(const double*, double
         const double alpha = rsq / dHd;
         if (dHd <= 0 || alpha > t) {
             for (int k = 0; k < n; k++) z[k] += t * d[k];
+            if (Hz)
+                for (int k = 0; k < n; k++) Hz[k] += t * Hd[k];
             return true;
         }
         for (int k = 0; k < n; k++) r[k] += alpha * Hd[k];
         for (int k = 0; k < n; k++) z[k] += alpha * d[k];
+        if (Hz)
+            for (int k = 0; k < n; k++) Hz[k] += alpha * Hd[k];
         const double rsq_next = dot(n, r.data(), r.data());
         if (std::sqrt(rsq_next) <= stoptol) break;
         const double beta = rsq_next / rsq;
@@ -143,11 +152,14 @@ void riemannian_trust_region_optimize_hmat(target_func f, target_gradient_hessia
     if (f_citer) f_citer[niter] = fc;
 }
 
-void riemannian_trust_region_optimize_hvp(const target_func f, const target_gradient_func f_deriv,
-                                          const hessian_vector_product_func f_hvp, void* fdata, retract_func retract,
-                                          void* rdata, const int n, const double* x_init, const int m,
-                                          const struct rtr_params* params, const int niter, double* f_iter,
-                                          double* x_final)
+}  // extern "C"
+
+namespace mg {
+
+void rtr_optimize_hvp(const target_func f, const target_gradient_func f_deriv, const hessian_vector_product_func f_hvp,
+                      void* fdata, retract_func retract, void* rdata, const int n, const double* x_init, const int m,
+                      const struct rtr_params* params, const int niter, double* f_iter, double* x_final,
+                      bool heta_from_cg)
 {
     assert(0 <= params->rho_trust && params->rho_trust < 0.25);
     std::vector<double> x(x_init, x_init + m), xn(m), grad(n), eta(n), Heta(n);
@@ -156,10 +168,12 @@ void riemannian_trust_region_optimize_hvp(const target_func f, const target_grad
     if (params->g_func) params->g_iter[0] = params->g_func(x.data(), params->g_data, &fc);
     for (int k = 0; k < niter; k++) {
         const double fx = f_deriv(x.data(), fdata, grad.data());
-        const bool bnd = truncated_cg_hvp(x.data(), grad.data(), f_hvp, fdata, n, radius, &params->tcg_params, eta.data());
+        const bool bnd = steihaug(
+            grad.data(), [&](const double* d, double* Hd) { f_hvp(x.data(), fdata, const_cast<double*>(d), Hd); }, n,
+            radius, &params->tcg_params, eta.data(), heta_from_cg ? Heta.data() : nullptr);
         retract(x.data(), eta.data(), rdata, xn.data());
         f_iter[k] = fx;
-        f_hvp(x.data(), fdata, eta.data(), Heta.data());
+        if (!heta_from_cg) f_hvp(x.data(), fdata, eta.data(), Heta.data());
         const double rho = (f(xn.data(), fdata, &fc) - fx) / (dot(n, grad.data(), eta.data()) + 0.5 * dot(n, eta.data(), Heta.data()));
         bool accept;
         tr_update(rho, bnd, params, &radius, &accept);
@@ -168,6 +182,19 @@ void riemannian_trust_region_optimize_hvp(const target_func f, const target_grad
     }
     std::memcpy(x_final, x.data(), m * sizeof(double));
     f_iter[niter] = f(x_final, fdata, &fc);
+}
+
+}  // namespace mg
+
+extern "C" {
+
+void riemannian_trust_region_optimize_hvp(const target_func f, const target_gradient_func f_deriv,
+                                          const hessian_vector_product_func f_hvp, void* fdata, retract_func retract,
+                                          void* rdata, const int n, const double* x_init, const int m,
+                                          const struct rtr_params* params, const int niter, double* f_iter,
+                                          double* x_final)
+{
+    mg::rtr_optimize_hvp(f, f_deriv, f_hvp, fdata, retract, rdata, n, x_init, m, params, niter, f_iter, x_final, false);
 }
 
 }  // extern "C"

@@ -6,7 +6,9 @@
 - a request gives the same bits alone and in any mix (the batched kernel's summation
   order depends only on the request's own size);
 - mg_serve: continuous batching of trust-region jobs reproduces each job run alone
-  bitwise, and the one-job-at-a-time path within 1e-9 (different kernels).
+  bitwise, and the one-job-at-a-time path within 1e-9 (different kernels); with tr_reuse
+  the trajectories stay within 1e-9 of the reference control flow with one HVP fewer per
+  iteration.
 """
 import sys
 
@@ -150,6 +152,32 @@ def test_densified_target_matches_program(mg):
     assert abs(ref[0] - fo) <= 1e-11 * abs(fo) and rel(ref[1], go) < 1e-11 and rel(ref[2], ho) < 1e-11
 
 
+@pytest.mark.parametrize("L", [13, 14])
+def test_densify_beyond_lds_matches_circuit_target(mg, L):
+    """Targets of L > 12 qubits are materialised in 12-qubit register tiles (each row its own
+    column's state between tile blocks); the alternative path (3-qubit windows applied in place
+    in the rows) must give the same rows.  Checked through f, gradient and HVP over all 2^L
+    columns (every row enters), FP64, against the circuit target applied per column."""
+    rng = np.random.default_rng(40 + L)
+    job, prog, tgt, v = make_instance(mg, rng, L, 3)
+    say(f"case: densify L={L} ({job['Gt']} target gates): tiles vs in-place windows vs circuit target")
+    ref = mg.riemannian_gradient_hvp(tgt, prog, job["gates"], v, mg.MG_FP64)
+    outs = []
+    try:
+        for knob in (-1, 0):
+            mg.set_window_kernel(knob)
+            dense = mg.Target.circuit(job["target_gates"], job["target_wires"], L).densify()
+            mg.set_window_kernel(-1)
+            outs.append(mg.riemannian_gradient_hvp(dense, prog, job["gates"], v, mg.MG_FP64))
+            del dense
+    finally:
+        mg.set_window_kernel(-1)
+    for out in outs:
+        assert abs(out[0] - ref[0]) <= 1e-12 * abs(ref[0])
+        assert rel(out[1], ref[1]) < 1e-12 and rel(out[2], ref[2]) < 1e-12
+    assert abs(outs[0][0] - outs[1][0]) <= 1e-13 * abs(ref[0])
+
+
 def _jobs(mg, seed, spec):
     from rqcopt_mg.workloads import brickwall_job
 
@@ -195,6 +223,33 @@ def test_serve_batched_equals_alone_and_single_path(mg):
     d64, _ = mg.serve(jobs, batched=True, precision=mg.MG_FP64, realtime=False, densify_qubits=10)
     for a, b in zip(d64, b64):
         assert np.max(np.abs(a["f_iter"] - b["f_iter"])) < 1e-9 and np.max(np.abs(a["gates"] - b["gates"])) < 1e-9
+
+
+def test_serve_tr_reuse_matches_reference_control_flow(mg):
+    """tr_reuse: H eta of the ratio test summed from the truncated CG's products, f + gradient
+    kept across rejected steps.  Same trajectories as the reference control flow (1e-9, FP64),
+    one HVP fewer per iteration, and still the same bits alone and in the mix."""
+    jobs = _jobs(mg, 23, SERVE_SPEC)
+    say("case: serve FP64 batched, reference control flow vs tr_reuse")
+    ref, st0 = mg.serve(jobs, batched=True, precision=mg.MG_FP64, realtime=False)
+    reu, st1 = mg.serve(jobs, batched=True, precision=mg.MG_FP64, realtime=False, tr_reuse=True)
+    niter = sum(int(j["niter"]) for j in jobs)
+    for a, b in zip(reu, ref):
+        assert a["status"] == 0
+        assert np.max(np.abs(a["f_iter"] - b["f_iter"])) < 1e-9
+        assert np.max(np.abs(a["gates"] - b["gates"])) < 1e-9
+        assert a["nhvp"] <= b["nhvp"]
+    saved_hvp = sum(b["nhvp"] for b in ref) - sum(a["nhvp"] for a in reu)
+    assert saved_hvp >= niter - 2, (saved_hvp, niter)
+    assert st0["requests"] - st1["requests"] >= saved_hvp
+    say("case: tr_reuse one job at a time vs batched, and a job alone vs in the mix")
+    one, _ = mg.serve(jobs, batched=False, precision=mg.MG_FP64, realtime=False, tr_reuse=True)
+    for a, b in zip(one, reu):
+        assert np.max(np.abs(a["f_iter"] - b["f_iter"])) < 1e-9
+    mix, _ = mg.serve(jobs, batched=True, precision=mg.MG_FP32, precision_switch=3, realtime=False, tr_reuse=True)
+    alone, _ = mg.serve([jobs[1]], batched=True, precision=mg.MG_FP32, precision_switch=3, realtime=False,
+                        tr_reuse=True)
+    assert np.array_equal(alone[0]["f_iter"], mix[1]["f_iter"]) and np.array_equal(alone[0]["gates"], mix[1]["gates"])
 
 
 def test_serve_admission_limits(mg):
