@@ -224,16 +224,18 @@ __device__ void batch_item(const BatchInst<R>& in, const BatchItem& it, unsigned
                     }
                 }
                 __syncthreads();
-                // one wave per gate adds the staged sums in wave order, then clears its slots
+                // one wave per gate adds the staged sums in wave order, then clears its slots;
+                // the wave is the one that owns the gate's parameter, so gates of one block
+                // that share a parameter (tied layers) are added by one wave, in order
                 for (int g = bk.g0; g < bk.g1; g++) {
-                    if (wv == (g - bk.g0) % NW && lane < NV) {
+                    const int gparam = cptr(in.inst)[g].param, gflip = cptr(in.inst)[g].flip;
+                    if (wv == gparam % NW && lane < NV) {
                         double* sg = stg + (size_t)(g - bk.g0) * NW * 32;
                         double s = 0.0;
                         for (int w = 0; w < NW; w++) {
                             s += sg[w * 32 + lane];
                             sg[w * 32 + lane] = 0.0;
                         }
-                        const int gparam = cptr(in.inst)[g].param, gflip = cptr(in.inst)[g].flip;
                         const int ci = lane >> 1;
                         int e = ci & 7;
                         if (gflip && e < 4) e = 3 - e;

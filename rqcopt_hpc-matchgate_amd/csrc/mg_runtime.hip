@@ -476,10 +476,14 @@ static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, in
     // threads of the LDS/global kernel: more waves per CU when one workgroup fills the LDS
     int lds_T = kn.lds_threads;
     {
-        // 512 threads when the states fill the LDS (one workgroup per CU), else 256
+        // 512 threads when the states fill the LDS (one workgroup per CU), else 256: two
+        // 256-thread workgroups per CU when their states fit twice (measured at L=12 FP32:
+        // f 0.66 -> 0.51 ms, f + gradient 1.94 -> 1.41 ms; each hides the other's barriers)
         const int nst0 = (mode == 2 ? 4 : 2);
-        const bool fits = sweep_lds_head_bytes(p.P, 8) + (size_t)nst0 * S * sizeof(cpx<R>) <= c->lds_limit;
-        if (lds_T == 0) lds_T = fits && S >= 2048 ? 512 : 256;
+        const size_t st0 = (size_t)nst0 * S * sizeof(cpx<R>);
+        const bool fits = sweep_lds_head_bytes(p.P, 8, mode) + st0 <= c->lds_limit;
+        const bool two = 2 * (sweep_lds_head_bytes(p.P, 4, mode) + st0) <= c->lds_limit;
+        if (lds_T == 0) lds_T = fits && S >= 2048 && !two ? 512 : 256;
     }
     const int NW = lds_T / 64;
 
@@ -499,7 +503,7 @@ static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, in
         else ckpt = !b.invertible || b.defect > 1e-10;
     }
     const int nstate = ckpt ? (mode == 2 ? 2 : 1) : (mode == 2 ? 4 : 2);
-    const size_t head = sweep_lds_head_bytes(p.P, NW);
+    const size_t head = sweep_lds_head_bytes(p.P, NW, mode);
     const size_t state_bytes = (size_t)nstate * S * sizeof(cpx<R>);
     if (head > c->lds_limit) { set_error("too many parameter gates for the LDS accumulator"); return -1; }
     bool lds_states = head + state_bytes <= c->lds_limit;
@@ -520,7 +524,7 @@ static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, in
                 break;
             }
         }
-        if (reg_T && head + S * sizeof(cpx<R>) > c->lds_limit) reg_T = 0;
+        if (reg_T && sweep_lds_head_bytes(p.P, NW) + S * sizeof(cpx<R>) > c->lds_limit) reg_T = 0;
     }
     // register-resident gate-window kernel (FP64, 512 threads x 8 amplitudes, dense targets)
     size_t win_acc = 0, win_buf = 0, win_lds = 0;
@@ -529,7 +533,7 @@ static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, in
     constexpr int fp32 = std::is_same<R, float>::value ? 1 : 0;
     if (!ckpt && b.defect <= 1e-12 && t.kind == 0 && win_sweep_fits(L, fp32) && kn.force_win != 0 && kn.force_reg != 0 &&
         kn.force_lds != 0 && (!fp32 || kn.force_win == 1)) {
-        win_lds = win_sweep_lds_bytes(L, p.P, fp32, c->lds_limit, &win_acc, &win_buf, &win_rs);
+        win_lds = win_sweep_lds_bytes(L, p.P, fp32, c->lds_limit, &win_acc, &win_buf, &win_rs, mode);
         use_win = win_lds <= c->lds_limit;
     }
     if (use_win) reg_T = 0;
@@ -549,7 +553,7 @@ static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, in
     size_t wt_acc = 0, wt_buf = 0, wt_lds = 0;
     int wt_rs = 0;
     if (use_tile && !fp32 && b.defect <= 1e-12 && kn.force_win != 0 && L > 12 && L - 12 <= 20) {
-        wt_lds = win_sweep_lds_bytes(12, p.P, 0, c->lds_limit, &wt_acc, &wt_buf, &wt_rs);
+        wt_lds = win_sweep_lds_bytes(12, p.P, 0, c->lds_limit, &wt_acc, &wt_buf, &wt_rs, mode);
         use_wtile = wt_lds <= c->lds_limit;
     }
     const int nw_reg = reg_T / 64;

@@ -32,14 +32,14 @@ __global__ __launch_bounds__(T) void sweep_kernel(SweepArgs<R> a)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr int NW = T / 64;
-    constexpr int NV = MODE == 2 ? 32 : 16;  // doubles reduced per gate
+    constexpr int NV = MODE == 2 ? 32 : 16;  // doubles reduced per gate (and kept per parameter)
     constexpr int SHIFT = 6 - log2c<NV>();
     const int L = a.L;
     const uint32_t S = 1u << L, Q = S >> 2;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
 
-    double* acc = reinterpret_cast<double*>(smem);    // [P][32]
-    double* stage = acc + 32 * a.P;                   // [2][NW][NV]
+    double* acc = reinterpret_cast<double*>(smem);    // [P][NV]
+    double* stage = acc + NV * a.P;                   // [2][NW][NV]
     double* fst = stage + 2 * NW * NV;                // [NW][2]
     cpx<R>* st;
     if constexpr (LDS) {
@@ -55,7 +55,7 @@ __global__ __launch_bounds__(T) void sweep_kernel(SweepArgs<R> a)
     cpx<R>* ck = nullptr;
     if constexpr (CKPT) ck = a.ckpt + (size_t)blockIdx.x * 2 * (a.G + 1) * S;  // [G+1][psi | dpsi]
 
-    for (int k = tid; k < 32 * a.P; k += T) acc[k] = 0.0;
+    for (int k = tid; k < NV * a.P; k += T) acc[k] = 0.0;
     double fre = 0.0, fim = 0.0;
     __syncthreads();
 
@@ -198,7 +198,7 @@ __global__ __launch_bounds__(T) void sweep_kernel(SweepArgs<R> a)
                     const int ci = lane >> 1;
                     int e = ci & 7;
                     if (in.flip && e < 4) e = 3 - e;
-                    acc[32 * in.param + 2 * ((ci & 8) | e) + (lane & 1)] += s;
+                    acc[NV * in.param + 2 * ((ci & 8) | e) + (lane & 1)] += s;
                 }
             }
         }
@@ -210,7 +210,7 @@ __global__ __launch_bounds__(T) void sweep_kernel(SweepArgs<R> a)
         out[0] = fre;
         out[1] = fim;
     }
-    for (int k = tid; k < 32 * a.P; k += T) out[2 + k] = acc[k];
+    for (int k = tid; k < 32 * a.P; k += T) out[2 + k] = (k & 31) < NV ? acc[(k >> 5) * NV + (k & 31)] : 0.0;
 }
 
 // out[k] = sum over workgroups w of partial[w][k], in a fixed order: thread row ty

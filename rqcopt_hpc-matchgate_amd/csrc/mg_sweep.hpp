@@ -24,10 +24,11 @@ struct SweepArgs {
     int lds_state_offset;   // byte offset of the states inside dynamic LDS (LDS mode)
 };
 
-// doubles of LDS bookkeeping in front of the states: acc [P][32], stage [2][NW][32], f [NW][2]
-inline size_t sweep_lds_head_bytes(int P, int nwaves)
+// doubles of LDS bookkeeping in front of the states: acc [P][32] (sweep_kernel: [P][16] below
+// MODE 2, which has no D^2 f[Z] half), stage [2][NW][32], f [NW][2]
+inline size_t sweep_lds_head_bytes(int P, int nwaves, int mode = 2)
 {
-    size_t b = sizeof(double) * (32 * (size_t)P + 2 * nwaves * 32 + 2 * nwaves);
+    size_t b = sizeof(double) * ((mode == 2 ? 32 : 16) * (size_t)P + 2 * nwaves * 32 + 2 * nwaves);
     return (b + 255) & ~(size_t)255;
 }
 

@@ -113,7 +113,7 @@ __global__ __launch_bounds__(WinShape<R>::T) void sweep_win_kernel(WinSweepArgs 
     double* acc = reinterpret_cast<double*>(smem + a.acc_off);
     cpx<R>* buf = reinterpret_cast<cpx<R>*>(smem + a.buf_off);  // [2 or 4][S] re-layout buffer
 
-    for (int k = tid0; k < 32 * a.P; k += kT) acc[k] = 0.0;
+    for (int k = tid0; k < NV * a.P; k += kT) acc[k] = 0.0;
     double fre = 0.0, fim = 0.0;
     __syncthreads();
 
@@ -248,16 +248,18 @@ __global__ __launch_bounds__(WinShape<R>::T) void sweep_win_kernel(WinSweepArgs 
                     }
                 }
                 __syncthreads();
+                // the wave that owns a gate's parameter adds its sums: gates of one block that
+                // share a parameter (tied layers) are added by one wave, in order
                 for (int g = bk.g0; g < bk.g1; g++) {
-                    if (wv == (g - bk.g0) % kNW && lane < NV) {
+                    const int gparam = cptr(a.inst)[g].param, gflip = cptr(a.inst)[g].flip;
+                    if (wv == gparam % kNW && lane < NV) {
                         const double* sg = stage + (size_t)(g - bk.g0) * kNW * 32;
                         double s = 0.0;
                         for (int w = 0; w < kNW; w++) s += sg[w * 32 + lane];
-                        const int gparam = cptr(a.inst)[g].param, gflip = cptr(a.inst)[g].flip;
                         const int ci = lane >> 1;
                         int e = ci & 7;
                         if (gflip && e < 4) e = 3 - e;
-                        acc[32 * gparam + 2 * ((ci & 8) | e) + (lane & 1)] += s;
+                        acc[NV * gparam + 2 * ((ci & 8) | e) + (lane & 1)] += s;
                     }
                 }
                 if (move) {
@@ -305,7 +307,7 @@ __global__ __launch_bounds__(WinShape<R>::T) void sweep_win_kernel(WinSweepArgs 
         out[0] = fre;
         out[1] = fim;
     }
-    for (int k = tid0; k < 32 * a.P; k += kT) out[2 + k] = acc[k];
+    for (int k = tid0; k < 32 * a.P; k += kT) out[2 + k] = (k & 31) < NV ? acc[(k >> 5) * NV + (k & 31)] : 0.0;
 }
 
 // ---- tiled variant (FP64, L > 12) ------------------------------------------------------------
@@ -353,7 +355,7 @@ __global__ __launch_bounds__(512) void sweep_win_tile_kernel(WinTileArgs a)
     const cpx<R>* coef = reinterpret_cast<const cpx<R>*>(a.coef);
     const cpx<R>* seed = reinterpret_cast<const cpx<R>*>(a.seed);
 
-    for (int k = tid0; k < 32 * a.P; k += kT) acc[k] = 0.0;
+    for (int k = tid0; k < NV * a.P; k += kT) acc[k] = 0.0;
     double fre = 0.0, fim = 0.0;
     __syncthreads();
 
@@ -524,16 +526,17 @@ __global__ __launch_bounds__(512) void sweep_win_tile_kernel(WinTileArgs a)
                             if constexpr (RS >= 2) win_store<R, kW, true>(buf + TSZ, sb, bk.off, P);
                         }
                         __syncthreads();
+                        // one wave per parameter: gates sharing it are added in order
                         for (int g = bk.g0; g < bk.g1; g++) {
-                            if (wv == (g - bk.g0) % kNW && lane < NV) {
+                            const int gparam = cptr(a.inst)[g].param, gflip = cptr(a.inst)[g].flip;
+                            if (wv == gparam % kNW && lane < NV) {
                                 const double* sg = stage + (size_t)(g - bk.g0) * kNW * 32;
                                 double s = 0.0;
                                 for (int w = 0; w < kNW; w++) s += sg[w * 32 + lane];
-                                const int gparam = cptr(a.inst)[g].param, gflip = cptr(a.inst)[g].flip;
                                 const int ci = lane >> 1;
                                 int e = ci & 7;
                                 if (gflip && e < 4) e = 3 - e;
-                                acc[32 * gparam + 2 * ((ci & 8) | e) + (lane & 1)] += s;
+                                acc[NV * gparam + 2 * ((ci & 8) | e) + (lane & 1)] += s;
                             }
                         }
                         if (move) {
@@ -594,7 +597,7 @@ __global__ __launch_bounds__(512) void sweep_win_tile_kernel(WinTileArgs a)
         out[0] = fre;
         out[1] = fim;
     }
-    for (int k = tid0; k < 32 * a.P; k += kT) out[2 + k] = acc[k];
+    for (int k = tid0; k < 32 * a.P; k += kT) out[2 + k] = (k & 31) < NV ? acc[(k >> 5) * NV + (k & 31)] : 0.0;
 }
 
 hipError_t launch_materialize_win_tile(const WinTileArgs& a, int grid, size_t lds, hipStream_t s)
@@ -633,12 +636,13 @@ hipError_t launch_sweep_win_tile(const WinTileArgs& a, int mode, int grid, size_
     return hipErrorInvalidValue;
 }
 
-size_t win_sweep_lds_bytes(int L, int P, int fp32, size_t limit, size_t* acc_off, size_t* buf_off, int* rs)
+size_t win_sweep_lds_bytes(int L, int P, int fp32, size_t limit, size_t* acc_off, size_t* buf_off, int* rs, int mode)
 {
     const int nw = (fp32 ? WinShape<float>::T : WinShape<double>::T) / 64;
     const size_t fixed = sizeof(double) * ((size_t)kWinRegGates * nw * 32 + 2 * nw);
     const size_t a0 = (fixed + 255) & ~(size_t)255;
-    const size_t b0 = (a0 + sizeof(double) * 32 * (size_t)P + 255) & ~(size_t)255;
+    // per parameter gate 16 complex sums (gradient, D^2 f[Z]); 8 below MODE 2
+    const size_t b0 = (a0 + sizeof(double) * (mode == 2 ? 32 : 16) * (size_t)P + 255) & ~(size_t)255;
     if (acc_off) *acc_off = a0;
     if (buf_off) *buf_off = b0;
     const size_t state = (fp32 ? sizeof(cpx<float>) : sizeof(cpx<double>)) * ((size_t)1 << L);

@@ -53,7 +53,8 @@ hipError_t launch_materialize_win_tile(const WinTileArgs& a, int grid, size_t ld
 // dynamic LDS of the kernel within `limit` bytes (and the accumulator and buffer offsets, and the
 // states per re-layout round that fit: FP64 re-lays two states out per round, or one when the
 // accumulator of many parameter gates leaves no room for two)
-size_t win_sweep_lds_bytes(int L, int P, int fp32, size_t limit, size_t* acc_off, size_t* buf_off, int* rs);
+size_t win_sweep_lds_bytes(int L, int P, int fp32, size_t limit, size_t* acc_off, size_t* buf_off, int* rs,
+                           int mode = 2);
 // the kernel's threads x window amplitudes cover the column (L=12 in both precisions)
 bool win_sweep_fits(int L, int fp32);
 int win_sweep_window_bits(int fp32);

@@ -234,6 +234,9 @@ def test_window_register_kernel_matches_oracle(mg, layout, depth, prec):
             # one state (64 KB) or two (128 KB) per re-layout round
             assert (info["lds_bytes"] < 100 * 1024) == (P > 79), (P, info["lds_bytes"])
         r1 = mg.evaluate_columns(t, p, V, None, mg.EVAL_F_GRAD, precision, 7, 71)
+        if prec == "fp64":
+            # without the D^2 f[Z] half of the accumulator two states fit at every depth
+            assert mg.last_sweep_info()["lds_bytes"] > 128 * 1024, (P, mg.last_sweep_info())
         r0 = mg.evaluate_columns(t, p, V, None, mg.EVAL_F, precision, 7, 71)
     finally:
         mg.set_window_kernel(-1)
@@ -241,6 +244,24 @@ def test_window_register_kernel_matches_oracle(mg, layout, depth, prec):
     tol = 1e-11 if prec == "fp64" else 2e-5
     assert abs(r[0] - fo) / abs(fo) < tol and rel(r[1], dgo) < tol and rel(r[2], dho) < tol
     assert abs(r1[0] - fo) / abs(fo) < tol and rel(r1[1], dgo) < tol and abs(r0[0] - fo) / abs(fo) < tol
+
+
+@pytest.mark.parametrize("depth", [5, 15])
+def test_lds_kernel_f_and_gradient_two_workgroups_per_cu(mg, depth):
+    """FP32 L=12 f and f + gradient on the LDS-resident kernel: 256-thread workgroups whose
+    states (2 x 32 KB) and gradient-only accumulator fit twice per CU, up to depth 15 (83
+    parameter gates), against the oracle (2e-5)."""
+    L = 12
+    say(f"case: LDS kernel FP32 L={L} open depth={depth}, f and f+grad")
+    V, Z, param, wires, P, U = build_case(L, "open", depth, seed=depth * 3 + 2)
+    t = mg.Target.dense(U)
+    p = mg.Program.create(L, P, param, wires)
+    fo, dgo, _ = orc.target_gradient_hvp(U, V, Z, param, wires, L, 5, 69)
+    r1 = mg.evaluate_columns(t, p, V, None, mg.EVAL_F_GRAD, mg.MG_FP32, 5, 69)
+    info = mg.last_sweep_info()
+    assert info["kernel"] == "sweep_kernel" and info["lds_states"] and info["lds_bytes"] <= 80 * 1024, info
+    r0 = mg.evaluate_columns(t, p, V, None, mg.EVAL_F, mg.MG_FP32, 5, 69)
+    assert abs(r1[0] - fo) / abs(fo) < 2e-5 and rel(r1[1], dgo) < 2e-5 and abs(r0[0] - fo) / abs(fo) < 2e-5
 
 
 @pytest.mark.parametrize("L,layout,depth,prec", [(13, "open", 11, "fp64"), (13, "tied", 4, "fp32"), (14, "open", 7, "fp64"),
