@@ -96,7 +96,7 @@ def copy_calibration(mib):
     print(json.dumps({"copy_bytes": n, "copies": 5}))
 
 
-def served_bench(mg, njobs, seed=1):
+def served_bench(mg, njobs, seed=1, Ls=(8, 10, 12)):
     """The served stream: Poisson arrivals (200 jobs/s, seed 1) of trust-region jobs with
     L in {8, 10, 12}, odd depth 3..15, budget 5..50 iterations, circuit targets of 3x the
     depth; FP32 sweeps with the last 3 iterations in FP64, continuous batching, targets
@@ -106,7 +106,7 @@ def served_bench(mg, njobs, seed=1):
     tr_reuse)."""
     from rqcopt_mg.workloads import served_stream
 
-    stream = served_stream(njobs, seed=seed, rate=200.0, Ls=(8, 10, 12))
+    stream = served_stream(njobs, seed=seed, rate=200.0, Ls=Ls)
     jobs = [{"program": mg.Program.create(j["L"], j["G"], j["param"], j["wires"]), "gates": j["gates"],
              "niter": j["niter"], "target_gates": j["target_gates"], "target_wires": j["target_wires"],
              "arrival": j["arrival"]} for j in stream]
@@ -121,15 +121,17 @@ def served_bench(mg, njobs, seed=1):
     log(f"autotuned (threads, window bits, workgroups/CU, ms): {tune}")
     mg.set_kernel_timing(True)
     log(f"serving {njobs} jobs with continuous batching ...")
-    res, st = mg.serve(jobs, batched=True, precision=mg.MG_FP32, precision_switch=3, densify_qubits=12,
-                       realtime=True, tr_reuse=True)
+    dq = max(12, max(Ls))
+    res, st = mg.serve(jobs, batched=True, precision=mg.MG_FP32, precision_switch=3, densify_qubits=dq,
+                       realtime=True, tr_reuse=True, progress_s=30.0)
     log(f"  {njobs / st['wall_s']:.2f} jobs/s, {st['rounds']} launches for {st['requests']} evaluations")
     log("the same stream one job at a time (FP64, one launch per evaluation, targets materialised) ...")
-    base, stb = mg.serve(jobs, batched=False, precision=mg.MG_FP64, realtime=True, densify_qubits=12)
+    base, stb = mg.serve(jobs, batched=False, precision=mg.MG_FP64, realtime=True, densify_qubits=dq, progress_s=30.0)
     log("the same stream one job at a time (FP64, one launch per evaluation, circuit targets as given) ...")
-    plain, stp = mg.serve(jobs, batched=False, precision=mg.MG_FP64, realtime=True, densify_qubits=0)
+    plain, stp = mg.serve(jobs, batched=False, precision=mg.MG_FP64, realtime=True, densify_qubits=0, progress_s=30.0)
     log("the same stream one job at a time (FP64, targets materialised, trust region with tr_reuse) ...")
-    reuse1, stu = mg.serve(jobs, batched=False, precision=mg.MG_FP64, realtime=True, densify_qubits=12, tr_reuse=True)
+    reuse1, stu = mg.serve(jobs, batched=False, precision=mg.MG_FP64, realtime=True, densify_qubits=dq, tr_reuse=True,
+                           progress_s=30.0)
     lat = np.array([r["latency"] for r in res]) * 1e3
     latu = np.array([r["latency"] for r in reuse1]) * 1e3
     latb = np.array([r["latency"] for r in base]) * 1e3
@@ -168,7 +170,7 @@ def served_bench(mg, njobs, seed=1):
             "speedup_vs_plain": jps / jpsp, "speedup_vs_dense_targets": jps / jpsb,
             "speedup_vs_dense_targets_tr_reuse": jps / jpsu,
             "rel_err_vs_fp64": err, "autotune": tune,
-            "workload": "Poisson 200 jobs/s seed 1, L in {8,10,12}, odd depth 3..15, 5..50 iterations, "
+            "workload": f"Poisson 200 jobs/s seed 1, L in {{{','.join(map(str, Ls))}}}, odd depth 3..15, 5..50 iterations, "
                         "circuit targets of 3x depth; FP32 sweeps, last 3 iterations FP64; trust region with tr_reuse"}
 
 
@@ -279,9 +281,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--rtr-iters", type=int, default=5, help="full trust-region iterations timed for the extra field")
+    ap.add_argument("--rtr-iters", type=int, default=50, help="full trust-region iterations timed for the extra field")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-sweep variant")
     ap.add_argument("--serve-jobs", type=int, default=32, help="jobs of the served stream (0 skips it)")
+    ap.add_argument("--serve-Ls", default="8,10,12",
+                    help="qubit counts of the served stream's jobs (with 14: minutes for the one-job baselines)")
     ap.add_argument("--large", action="store_true", help="also run the L=16 instance (about 20 s of GPU time)")
     ap.add_argument("--batched-iters", type=int, default=2,
                     help="trust-region iterations per instance of the 64-instance batched workload (20 is the "
@@ -337,16 +341,29 @@ def main():
             "launch": info32}
         log(f"FP32 sweeps: {1e3 * dt32 / args.steps:.3f} ms/step, kernel {k32:.3f} ms")
 
-    # full trust-region iterations (f+grad, tCG HVPs, f at the candidate)
+    # full trust-region iterations (f+grad, tCG HVPs, f at the candidate): FP64 with the
+    # reference control flow, and FP32 sweeps with the last 3 iterations in FP64 and the served
+    # trust region (tr_reuse), one job, one launch per evaluation
     if args.rtr_iters > 0:
         t0 = time.perf_counter()
         f_iter, _, _, nh = mg.optimize(t, p, V, args.rtr_iters, hvp=True, precision=prec)
         rt = time.perf_counter() - t0
         extra["rtr"] = {"iters": args.rtr_iters, "iters_per_s": args.rtr_iters / rt, "hvp_sweeps": nh,
-                        "f_first": float(f_iter[0]), "f_last": float(f_iter[-1])}
+                        "f_first": float(f_iter[0]), "f_last": float(f_iter[-1]),
+                        "precision": "fp64 sweeps, reference control flow"}
+        job = {"program": p, "gates": V, "niter": args.rtr_iters, "U": W, "arrival": 0.0}
+        r, st = mg.serve([job], batched=False, precision=mg.MG_FP32, precision_switch=3, realtime=False,
+                         tr_reuse=True)
+        extra["rtr_fp32_sweeps"] = {
+            "iters": args.rtr_iters, "iters_per_s": args.rtr_iters / st["wall_s"], "hvp_sweeps": r[0]["nhvp"],
+            "evaluations": st["requests"], "f_first": float(r[0]["f_iter"][0]), "f_last": float(r[0]["f_iter"][-1]),
+            "f_last_rel_diff_vs_fp64": float(abs(r[0]["f_iter"][-1] - f_iter[-1]) / abs(f_iter[-1])),
+            "precision": "fp32 sweeps with fp64 accumulation, last 3 iterations fp64; tr_reuse"}
+        log(f"trust region: {extra['rtr']['iters_per_s']:.1f} it/s FP64, "
+            f"{extra['rtr_fp32_sweeps']['iters_per_s']:.1f} it/s FP32 sweeps + tr_reuse")
 
     if args.serve_jobs > 0:
-        extra["served"] = served_bench(mg, args.serve_jobs)
+        extra["served"] = served_bench(mg, args.serve_jobs, Ls=tuple(int(x) for x in args.serve_Ls.split(",")))
     if args.batched_iters > 0:
         extra["batched"] = batched_bench(mg, args.batched_iters)
     if args.large:
