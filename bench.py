@@ -96,24 +96,25 @@ def copy_calibration(mib):
     print(json.dumps({"copy_bytes": n, "copies": 5}))
 
 
-def served_bench(mg, njobs, seed=1, Ls=(8, 10, 12)):
+def served_bench(mg, njobs, seed=1, Ls=(8, 10, 12), baselines=("dense", "plain", "reuse")):
     """The served stream: Poisson arrivals (200 jobs/s, seed 1) of trust-region jobs with
-    L in {8, 10, 12}, odd depth 3..15, budget 5..50 iterations, circuit targets of 3x the
-    depth; FP32 sweeps with the last 3 iterations in FP64, continuous batching, targets
-    materialised at admission, each job's trust region reusing its own products (tr_reuse);
-    against the same stream run one job at a time in FP64 with one launch per evaluation and the
-    reference trust-region control flow (and, to separate the gains, one job at a time with
-    tr_reuse)."""
+    L in Ls, odd depth 3..15, budget 5..50 iterations, circuit targets of 3x the depth; FP32
+    sweeps with the last 3 iterations in FP64, continuous batching, targets materialised at
+    admission, each job's trust region reusing its own products (tr_reuse); against the same
+    stream run one job at a time in FP64 with one launch per evaluation and the reference
+    trust-region control flow ("plain": circuit targets as given; "dense": targets materialised)
+    and, to separate the gains, one job at a time with tr_reuse ("reuse")."""
     from rqcopt_mg.workloads import served_stream
 
     stream = served_stream(njobs, seed=seed, rate=200.0, Ls=Ls)
     jobs = [{"program": mg.Program.create(j["L"], j["G"], j["param"], j["wires"]), "gates": j["gates"],
              "niter": j["niter"], "target_gates": j["target_gates"], "target_wires": j["target_wires"],
              "arrival": j["arrival"]} for j in stream]
-    # autotune the batched kernel on one HVP per job of the first 16 (their initial gates)
+    # autotune the batched kernel on one HVP per job of the first 16 that it runs (L < 12: the
+    # dense larger ones go to the single-request kernels), at their initial gates
     rng = np.random.default_rng(seed)
     calib = []
-    for j in jobs[:16]:
+    for j in [j for j in jobs if j["program"].L < 12][:16]:
         t = mg.Target.circuit(j["target_gates"], j["target_wires"], j["program"].L).densify()
         calib.append({"target": t, "program": j["program"], "gates": j["gates"],
                       "v": rng.standard_normal(8 * j["program"].P), "what": mg.EVAL_F_GRAD_HVP})
@@ -124,22 +125,38 @@ def served_bench(mg, njobs, seed=1, Ls=(8, 10, 12)):
     dq = max(12, max(Ls))
     res, st = mg.serve(jobs, batched=True, precision=mg.MG_FP32, precision_switch=3, densify_qubits=dq,
                        realtime=True, tr_reuse=True, progress_s=30.0)
-    log(f"  {njobs / st['wall_s']:.2f} jobs/s, {st['rounds']} launches for {st['requests']} evaluations")
-    log("the same stream one job at a time (FP64, one launch per evaluation, targets materialised) ...")
-    base, stb = mg.serve(jobs, batched=False, precision=mg.MG_FP64, realtime=True, densify_qubits=dq, progress_s=30.0)
-    log("the same stream one job at a time (FP64, one launch per evaluation, circuit targets as given) ...")
-    plain, stp = mg.serve(jobs, batched=False, precision=mg.MG_FP64, realtime=True, densify_qubits=0, progress_s=30.0)
-    log("the same stream one job at a time (FP64, targets materialised, trust region with tr_reuse) ...")
-    reuse1, stu = mg.serve(jobs, batched=False, precision=mg.MG_FP64, realtime=True, densify_qubits=dq, tr_reuse=True,
-                           progress_s=30.0)
+    jps = njobs / st["wall_s"]
+    log(f"  {jps:.2f} jobs/s, {st['rounds']} launches for {st['requests']} evaluations")
     lat = np.array([r["latency"] for r in res]) * 1e3
-    latu = np.array([r["latency"] for r in reuse1]) * 1e3
-    latb = np.array([r["latency"] for r in base]) * 1e3
-    latp = np.array([r["latency"] for r in plain]) * 1e3
+    out = {"jobs": njobs, "jobs_per_s": jps, "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+           "launches": st["rounds"], "evaluations": st["requests"], "peak_admitted": st["max_active"],
+           "kernel_s": st["kernel_ms"] * 1e-3, "wall_s": st["wall_s"], "graph_replays": st["graph_replays"],
+           "hvp_sweeps": int(sum(r["nhvp"] for r in res))}
+    runs = {
+        "dense": ("baseline_fp64_one_job_dense_targets", "speedup_vs_dense_targets", dict(densify_qubits=dq),
+                  "targets materialised at admission, best single-job kernels, reference control flow"),
+        "plain": ("baseline_fp64_one_job_plain", "speedup_vs_plain", dict(densify_qubits=0),
+                  "circuit targets applied per column, one launch per evaluation, reference control flow"),
+        "reuse": ("baseline_fp64_one_job_dense_targets_tr_reuse", "speedup_vs_dense_targets_tr_reuse",
+                  dict(densify_qubits=dq, tr_reuse=True),
+                  "targets materialised, the served trust region (H eta from the truncated CG, f+grad kept)"),
+    }
+    base = None
+    for name in baselines:
+        key, skey, kw, note = runs[name]
+        log(f"the same stream one job at a time (FP64, one launch per evaluation): {note} ...")
+        r1, s1 = mg.serve(jobs, batched=False, precision=mg.MG_FP64, realtime=True, progress_s=30.0, **kw)
+        l1 = np.array([r["latency"] for r in r1]) * 1e3
+        out[key] = {"jobs_per_s": njobs / s1["wall_s"], "p50_ms": float(np.percentile(l1, 50)),
+                    "p99_ms": float(np.percentile(l1, 99)), "evaluations": s1["requests"],
+                    "hvp_sweeps": int(sum(r["nhvp"] for r in r1)), "note": note}
+        out[skey] = jps / out[key]["jobs_per_s"]
+        if name == "dense":
+            base = r1
     # error of each job's final evaluation (FP64 sweeps in the last iterations) and of an FP32
     # sweep at the same gates, against the FP64 single-request path (pinned to the oracle in tests)
-    err = {"grad_final": 0.0, "hvp_final": 0.0, "grad_fp32_sweep": 0.0, "hvp_fp32_sweep": 0.0, "f_vs_fp64_path": 0.0}
-    for j, r, b in zip(jobs, res, base):
+    err = {"grad_final": 0.0, "hvp_final": 0.0, "grad_fp32_sweep": 0.0, "hvp_fp32_sweep": 0.0}
+    for k, (j, r) in enumerate(zip(jobs, res)):
         t = mg.Target.circuit(j["target_gates"], j["target_wires"], j["program"].L)
         v = rng.standard_normal(8 * j["program"].P)
         f0, g0, h0 = mg.riemannian_gradient_hvp(t, j["program"], r["gates"], v, mg.MG_FP64)
@@ -150,28 +167,15 @@ def served_bench(mg, njobs, seed=1, Ls=(8, 10, 12)):
         err["hvp_final"] = max(err["hvp_final"], float(np.max(np.abs(o64[2] - h0)) / hn))
         err["grad_fp32_sweep"] = max(err["grad_fp32_sweep"], float(np.max(np.abs(o32[1] - g0)) / gn))
         err["hvp_fp32_sweep"] = max(err["hvp_fp32_sweep"], float(np.max(np.abs(o32[2] - h0)) / hn))
-        err["f_vs_fp64_path"] = max(err["f_vs_fp64_path"], float(abs(r["f_iter"][-1] - b["f_iter"][-1]) / abs(b["f_iter"][-1])))
-    jps, jpsb, jpsp, jpsu = njobs / st["wall_s"], njobs / stb["wall_s"], njobs / stp["wall_s"], njobs / stu["wall_s"]
-    return {"jobs": njobs, "jobs_per_s": jps, "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
-            "launches": st["rounds"], "evaluations": st["requests"], "peak_admitted": st["max_active"],
-            "kernel_s": st["kernel_ms"] * 1e-3, "wall_s": st["wall_s"], "graph_replays": st["graph_replays"],
-            "baseline_fp64_one_job_plain": {"jobs_per_s": jpsp, "p50_ms": float(np.percentile(latp, 50)),
-                                            "p99_ms": float(np.percentile(latp, 99)),
-                                            "note": "circuit targets applied per column, one launch per evaluation"},
-            "baseline_fp64_one_job_dense_targets": {"jobs_per_s": jpsb, "p50_ms": float(np.percentile(latb, 50)),
-                                                    "p99_ms": float(np.percentile(latb, 99)),
-                                                    "note": "targets materialised at admission, best single-job kernels"},
-            "baseline_fp64_one_job_dense_targets_tr_reuse": {
-                "jobs_per_s": jpsu, "p50_ms": float(np.percentile(latu, 50)), "p99_ms": float(np.percentile(latu, 99)),
-                "evaluations": stu["requests"],
-                "note": "as above with the served trust region (H eta from the truncated CG, f+grad kept on rejection)"},
-            "evaluations_reference_control_flow": stb["requests"],
-            "hvp_sweeps": int(sum(r["nhvp"] for r in res)), "hvp_sweeps_reference_control_flow": int(sum(b["nhvp"] for b in base)),
-            "speedup_vs_plain": jps / jpsp, "speedup_vs_dense_targets": jps / jpsb,
-            "speedup_vs_dense_targets_tr_reuse": jps / jpsu,
-            "rel_err_vs_fp64": err, "autotune": tune,
-            "workload": f"Poisson 200 jobs/s seed 1, L in {{{','.join(map(str, Ls))}}}, odd depth 3..15, 5..50 iterations, "
-                        "circuit targets of 3x depth; FP32 sweeps, last 3 iterations FP64; trust region with tr_reuse"}
+        if base is not None:
+            b = base[k]
+            err["f_vs_fp64_path"] = max(err.get("f_vs_fp64_path", 0.0),
+                                        float(abs(r["f_iter"][-1] - b["f_iter"][-1]) / abs(b["f_iter"][-1])))
+    out.update({"rel_err_vs_fp64": err, "autotune": tune,
+                "workload": f"Poisson 200 jobs/s seed 1, L in {{{','.join(map(str, Ls))}}}, odd depth 3..15, 5..50 "
+                            "iterations, circuit targets of 3x depth; FP32 sweeps, last 3 iterations FP64; trust region "
+                            "with tr_reuse"})
+    return out
 
 
 def batched_bench(mg, niter, seed=0):
@@ -286,6 +290,8 @@ def main():
     ap.add_argument("--serve-jobs", type=int, default=32, help="jobs of the served stream (0 skips it)")
     ap.add_argument("--serve-Ls", default="8,10,12",
                     help="qubit counts of the served stream's jobs (with 14: minutes for the one-job baselines)")
+    ap.add_argument("--serve-baselines", default="dense,plain,reuse",
+                    help="one-job-at-a-time FP64 runs of the same stream to compare with (subset of dense,plain,reuse)")
     ap.add_argument("--large", action="store_true", help="also run the L=16 instance (about 20 s of GPU time)")
     ap.add_argument("--batched-iters", type=int, default=2,
                     help="trust-region iterations per instance of the 64-instance batched workload (20 is the "
@@ -363,7 +369,8 @@ def main():
             f"{extra['rtr_fp32_sweeps']['iters_per_s']:.1f} it/s FP32 sweeps + tr_reuse")
 
     if args.serve_jobs > 0:
-        extra["served"] = served_bench(mg, args.serve_jobs, Ls=tuple(int(x) for x in args.serve_Ls.split(",")))
+        extra["served"] = served_bench(mg, args.serve_jobs, Ls=tuple(int(x) for x in args.serve_Ls.split(",")),
+                                       baselines=tuple(x for x in args.serve_baselines.split(",") if x))
     if args.batched_iters > 0:
         extra["batched"] = batched_bench(mg, args.batched_iters)
     if args.large:
