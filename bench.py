@@ -118,8 +118,10 @@ def served_bench(mg, njobs, seed=1, Ls=(8, 10, 12), baselines=("dense", "plain",
         t = mg.Target.circuit(j["target_gates"], j["target_wires"], j["program"].L).densify()
         calib.append({"target": t, "program": j["program"], "gates": j["gates"],
                       "v": rng.standard_normal(8 * j["program"].P), "what": mg.EVAL_F_GRAD_HVP})
-    tune = {"fp32": mg.batch_autotune(calib, mg.MG_FP32), "fp64": mg.batch_autotune(calib, mg.MG_FP64)}
-    log(f"autotuned (threads, window bits, workgroups/CU, ms): {tune}")
+    tune = None
+    if calib:
+        tune = {"fp32": mg.batch_autotune(calib, mg.MG_FP32), "fp64": mg.batch_autotune(calib, mg.MG_FP64)}
+        log(f"autotuned (threads, window bits, workgroups/CU, ms): {tune}")
     mg.set_kernel_timing(True)
     log(f"serving {njobs} jobs with continuous batching ...")
     dq = max(12, max(Ls))
