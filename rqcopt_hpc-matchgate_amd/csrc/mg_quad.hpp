@@ -211,6 +211,32 @@ __device__ inline const cpx<R>& sel(const cpx<R>* s, uint32_t a)
     else return s[a];
 }
 
+// element index of amplitude a: swizzled in LDS, plain in memory
+template <typename R, bool SWZ>
+__device__ inline uint32_t swz_idx(uint32_t a)
+{
+    if constexpr (SWZ) return lds_swz<R>(a);
+    else return a;
+}
+
+// the quadruple whose first element sits at index sb = swz_idx(b): lds_swz is linear over GF(2)
+// and b has both wire bits clear, so the other three sit at sb ^ olo, sb ^ ohi and sb ^ olo ^ ohi
+// with olo, ohi the (wave-uniform) indices of the wire bits themselves
+template <typename R>
+__device__ inline Quad<R> load_quad_o(const cpx<R>* s, uint32_t sb, uint32_t olo, uint32_t ohi)
+{
+    return {s[sb], s[sb ^ olo], s[sb ^ ohi], s[sb ^ olo ^ ohi]};
+}
+
+template <typename R>
+__device__ inline void store_quad_o(cpx<R>* s, uint32_t sb, uint32_t olo, uint32_t ohi, const Quad<R>& q)
+{
+    s[sb] = q.x;
+    s[sb ^ olo] = q.y;
+    s[sb ^ ohi] = q.z;
+    s[sb ^ olo ^ ohi] = q.w;
+}
+
 template <typename R, bool SWZ = false>
 __device__ inline Quad<R> load_quad(const cpx<R>* s, uint32_t b, int lo, int hi)
 {

@@ -71,9 +71,10 @@ __global__ __launch_bounds__(T) void sweep_kernel(SweepArgs<R> a)
                 const Inst in = a.tinst[g];
                 cpx<R> V[8];
                 load_coef<R, 8>(a.tcoef + (size_t)g * kCoefPerInst, V);
+                const uint32_t olo = swz_idx<R, LDS>(1u << in.lo), ohi = swz_idx<R, LDS>(1u << in.hi);
                 for (uint32_t q = tid; q < Q; q += T) {
-                    const uint32_t base = quad_base(q, in.lo, in.hi);
-                    store_quad<R, LDS>(lam, base, in.lo, in.hi, gapply(V, load_quad<R, LDS>(lam, base, in.lo, in.hi)));
+                    const uint32_t sb = swz_idx<R, LDS>(quad_base(q, in.lo, in.hi));
+                    store_quad_o(lam, sb, olo, ohi, gapply(V, load_quad_o(lam, sb, olo, ohi)));
                 }
                 __syncthreads();
             }
@@ -99,14 +100,16 @@ __global__ __launch_bounds__(T) void sweep_kernel(SweepArgs<R> a)
             cpx<R>* pout = CKPT ? ck + (size_t)2 * (g + 1) * S : psi;
             const cpx<R>* din = CKPT ? ck + (size_t)(2 * g + 1) * S : dpsi;
             cpx<R>* dout = CKPT ? ck + (size_t)(2 * g + 3) * S : dpsi;
+            // one swizzle per quadruple: the wire bits' images are uniform
+            const uint32_t olo = swz_idx<R, LDS && !CKPT>(1u << in.lo), ohi = swz_idx<R, LDS && !CKPT>(1u << in.hi);
             for (uint32_t q = tid; q < Q; q += T) {
-                const uint32_t base = quad_base(q, in.lo, in.hi);
-                const Quad<R> p = load_quad<R, LDS && !CKPT>(pin, base, in.lo, in.hi);
+                const uint32_t sb = swz_idx<R, LDS && !CKPT>(quad_base(q, in.lo, in.hi));
+                const Quad<R> p = load_quad_o(pin, sb, olo, ohi);
                 if constexpr (MODE == 2) {
-                    const Quad<R> d = load_quad<R, LDS && !CKPT>(din, base, in.lo, in.hi);
-                    store_quad<R, LDS && !CKPT>(dout, base, in.lo, in.hi, gapply_add(cf, d, gapply(cf + 16, p)));
+                    const Quad<R> d = load_quad_o(din, sb, olo, ohi);
+                    store_quad_o(dout, sb, olo, ohi, gapply_add(cf, d, gapply(cf + 16, p)));
                 }
-                store_quad<R, LDS && !CKPT>(pout, base, in.lo, in.hi, gapply(cf, p));
+                store_quad_o(pout, sb, olo, ohi, gapply(cf, p));
             }
             __syncthreads();
         }
@@ -152,30 +155,32 @@ __global__ __launch_bounds__(T) void sweep_kernel(SweepArgs<R> a)
                     ga[k] = {R(0), R(0)};
                     ha[k] = {R(0), R(0)};
                 }
+                const uint32_t olo = swz_idx<R, LDS>(1u << in.lo), ohi = swz_idx<R, LDS>(1u << in.hi);
                 for (uint32_t q = tid; q < Q; q += T) {
                     const uint32_t base = quad_base(q, in.lo, in.hi);
-                    const Quad<R> l = load_quad<R, LDS>(lam, base, in.lo, in.hi);
+                    const uint32_t sb = swz_idx<R, LDS>(base);
+                    const Quad<R> l = load_quad_o(lam, sb, olo, ohi);
                     Quad<R> p, dp, dl;
                     if constexpr (CKPT) {
                         p = load_quad(ck + (size_t)2 * g * S, base, in.lo, in.hi);
                         if constexpr (MODE == 2) dp = load_quad(ck + (size_t)(2 * g + 1) * S, base, in.lo, in.hi);
                     } else {
-                        p = gapply(cf + 8, load_quad<R, LDS>(psi, base, in.lo, in.hi));
-                        store_quad<R, LDS>(psi, base, in.lo, in.hi, p);
+                        p = gapply(cf + 8, load_quad_o(psi, sb, olo, ohi));
+                        store_quad_o(psi, sb, olo, ohi, p);
                         if constexpr (MODE == 2) {
-                            const Quad<R> dn = load_quad<R, LDS>(dpsi, base, in.lo, in.hi);
+                            const Quad<R> dn = load_quad_o(dpsi, sb, olo, ohi);
                             dp = gapply(cf + 8, qsub(dn, gapply(cf + 16, p)));
-                            store_quad<R, LDS>(dpsi, base, in.lo, in.hi, dp);
+                            store_quad_o(dpsi, sb, olo, ohi, dp);
                         }
                     }
                     outer_acc(ga, l, p);
                     if constexpr (MODE == 2) {
-                        dl = load_quad<R, LDS>(dlam, base, in.lo, in.hi);
+                        dl = load_quad_o(dlam, sb, olo, ohi);
                         outer_acc(ha, dl, p);
                         outer_acc(ha, l, dp);
-                        store_quad<R, LDS>(dlam, base, in.lo, in.hi, gapplyT_add(cf, dl, gapplyT(cf + 16, l)));
+                        store_quad_o(dlam, sb, olo, ohi, gapplyT_add(cf, dl, gapplyT(cf + 16, l)));
                     }
-                    store_quad<R, LDS>(lam, base, in.lo, in.hi, gapplyT(cf, l));
+                    store_quad_o(lam, sb, olo, ohi, gapplyT(cf, l));
                 }
                 // lane sums in the sweep precision, then FP64 across waves, columns and workgroups
                 R v[NV];
