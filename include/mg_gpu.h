@@ -94,6 +94,17 @@ int mg_optimize(const mg_target* t, const mg_program* p, const struct matchgate 
                 const struct rtr_params* params, int niter, int hvp, int precision, int precision_switch,
                 double* f_iter, numeric* f_citer, struct matchgate gates_opt[], int* nhvp);
 
+/* ---- host trust region with reuse (no device work) ----
+ * riemannian_trust_region_optimize_hvp (include/trust_region.h, same arguments); with
+ * heta_from_cg != 0, H eta of each ratio test is the truncated CG's own sum of its steps times
+ * the products H d it already evaluated: one f_hvp call per iteration fewer, equal to a separate
+ * product up to rounding (the served jobs' trust region, mg_serve_options.tr_reuse). */
+void mg_trust_region_optimize_hvp(const target_func f, const target_gradient_func f_deriv,
+                                  const hessian_vector_product_func f_hvp, void* fdata, retract_func retract,
+                                  void* rdata, const int n, const double* x_init, const int m,
+                                  const struct rtr_params* params, const int niter, double* f_iter, double* x_final,
+                                  int heta_from_cg);
+
 /* ---- batched evaluation ----
  * Requests of any targets, programs, qubit counts (<= 16) and modes, merged into one
  * persistent launch per precision.  Outputs as mg_riemannian_gradient_hvp.  A request's

@@ -197,4 +197,14 @@ void riemannian_trust_region_optimize_hvp(const target_func f, const target_grad
     mg::rtr_optimize_hvp(f, f_deriv, f_hvp, fdata, retract, rdata, n, x_init, m, params, niter, f_iter, x_final, false);
 }
 
+void mg_trust_region_optimize_hvp(const target_func f, const target_gradient_func f_deriv,
+                                  const hessian_vector_product_func f_hvp, void* fdata, retract_func retract,
+                                  void* rdata, const int n, const double* x_init, const int m,
+                                  const struct rtr_params* params, const int niter, double* f_iter, double* x_final,
+                                  int heta_from_cg)
+{
+    mg::rtr_optimize_hvp(f, f_deriv, f_hvp, fdata, retract, rdata, n, x_init, m, params, niter, f_iter, x_final,
+                         heta_from_cg != 0);
+}
+
 }  // extern "C"
