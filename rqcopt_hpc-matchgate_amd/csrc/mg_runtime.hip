@@ -395,6 +395,14 @@ static void make_windowed(const std::vector<Inst>& inst, const std::vector<cpx<d
 // register-window kernel: instances in the final order (lo = window-local pair index),
 // coefficients, tile blocks, window blocks with LDS (swizzled tile-local) and global
 // (amplitude-index) offsets
+// gates of the largest window block: the kernels' staging slots
+static int max_block_gates(const std::vector<WBlock>& blocks)
+{
+    int m = 1;
+    for (const WBlock& bk : blocks) m = std::max(m, bk.g1 - bk.g0);
+    return m;
+}
+
 struct WindowedTiles {
     std::vector<Inst> inst;
     std::vector<cpx<double>> coef;
@@ -528,12 +536,16 @@ static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, in
     }
     // register-resident gate-window kernel (FP64, 512 threads x 8 amplitudes, dense targets)
     size_t win_acc = 0, win_buf = 0, win_lds = 0;
-    int win_rs = 0;
+    int win_rs = 0, win_slots = kWinRegGates;
     bool use_win = false;
+    Windowed w_win;
     constexpr int fp32 = std::is_same<R, float>::value ? 1 : 0;
     if (!ckpt && b.defect <= 1e-12 && t.kind == 0 && win_sweep_fits(L, fp32) && kn.force_win != 0 && kn.force_reg != 0 &&
         kn.force_lds != 0 && (!fp32 || kn.force_win == 1)) {
-        win_lds = win_sweep_lds_bytes(L, p.P, fp32, c->lds_limit, &win_acc, &win_buf, &win_rs, mode);
+        // the window form first: its largest block sizes the staging area
+        make_windowed<R>(b.inst, b.coef, L, win_sweep_window_bits(fp32), &w_win, kWinRegGates);
+        win_slots = max_block_gates(w_win.blk_lds);
+        win_lds = win_sweep_lds_bytes(L, p.P, fp32, c->lds_limit, &win_acc, &win_buf, &win_rs, mode, win_slots);
         use_win = win_lds <= c->lds_limit;
     }
     if (use_win) reg_T = 0;
@@ -551,9 +563,12 @@ static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, in
     // tiled register-window kernel (FP64, unitary gates): preferred over the LDS tiles
     bool use_wtile = false;
     size_t wt_acc = 0, wt_buf = 0, wt_lds = 0;
-    int wt_rs = 0;
-    if (use_tile && !fp32 && b.defect <= 1e-12 && kn.force_win != 0 && L > 12 && L - 12 <= 20) {
-        wt_lds = win_sweep_lds_bytes(12, p.P, 0, c->lds_limit, &wt_acc, &wt_buf, &wt_rs, mode);
+    int wt_rs = 0, wt_slots = kWinRegGates;
+    WindowedTiles w_wt;
+    if (use_tile && !fp32 && b.defect <= 1e-12 && kn.force_win != 0 && L > 12 && L - 12 <= 20 &&
+        make_windowed_tiles(b.inst, b.coef, L, &w_wt)) {
+        wt_slots = max_block_gates(w_wt.blk_lds);
+        wt_lds = win_sweep_lds_bytes(12, p.P, 0, c->lds_limit, &wt_acc, &wt_buf, &wt_rs, mode, wt_slots);
         use_wtile = wt_lds <= c->lds_limit;
     }
     const int nw_reg = reg_T / 64;
@@ -613,8 +628,7 @@ static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, in
 
     WinSweepArgs wa{};
     if (use_win) {
-        Windowed w;
-        make_windowed<R>(b.inst, b.coef, L, win_sweep_window_bits(fp32), &w, kWinRegGates);
+        const Windowed& w = w_win;
         std::vector<cpx<R>> wc(w.coef.size());
         for (size_t k = 0; k < wc.size(); k++) wc[k] = {(R)w.coef[k].re, (R)w.coef[k].im};
         MG_HIP(c->w_inst.ensure(w.inst.size() * sizeof(Inst)));
@@ -640,16 +654,13 @@ static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, in
         wa.acc_off = (int)win_acc;
         wa.buf_off = (int)win_buf;
         wa.rs = win_rs;
+        wa.slots = win_slots;
         // the copies above read pageable host vectors: complete them before w goes away
         MG_HIP(hipStreamSynchronize(s));
     }
     WinTileArgs wta{};
     if (use_wtile) {
-        WindowedTiles w;
-        if (!make_windowed_tiles(b.inst, b.coef, L, &w)) {
-            set_error("tiled window sweep: gate blocking failed");
-            return -1;
-        }
+        const WindowedTiles& w = w_wt;
         MG_HIP(c->w_inst.ensure(w.inst.size() * sizeof(Inst)));
         MG_HIP(c->w_coef.ensure(w.coef.size() * sizeof(cpx<double>)));
         MG_HIP(c->w_blk.ensure(2 * w.blk_lds.size() * sizeof(WBlock) + w.tblocks.size() * sizeof(WTBlock)));
@@ -679,6 +690,7 @@ static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, in
         wta.acc_off = (int)wt_acc;
         wta.buf_off = (int)wt_buf;
         wta.rs = wt_rs;
+        wta.slots = wt_slots;
         MG_HIP(hipStreamSynchronize(s));
     }
     TileArgs<R> ta{};
@@ -922,7 +934,8 @@ int target_densify(Target& t)
         MG_HIP(hipMemcpy(blk + 2 * nb, wt.tblocks.data(), wt.tblocks.size() * sizeof(WTBlock), hipMemcpyHostToDevice));
         size_t acc_off = 0, buf_off = 0;
         int rs = 0;
-        win_sweep_lds_bytes(12, 0, 0, c->lds_limit, &acc_off, &buf_off, &rs);
+        const int slots = max_block_gates(wt.blk_lds);
+        win_sweep_lds_bytes(12, 0, 0, c->lds_limit, &acc_off, &buf_off, &rs, 2, slots);
         const size_t lds = buf_off + ((size_t)1 << 12) * sizeof(cpx<double>);
         WinTileArgs a{};
         a.L = L;
@@ -940,6 +953,7 @@ int target_densify(Target& t)
         a.acc_off = (int)acc_off;
         a.buf_off = (int)buf_off;
         a.rs = 1;
+        a.slots = slots;
         a.rows = t.seed64.p;
         const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(2, c->lds_limit / lds));
         const int grid = (int)std::min<long>((long)S, (long)c->ncu * per_cu);

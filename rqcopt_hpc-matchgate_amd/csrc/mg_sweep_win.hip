@@ -108,8 +108,8 @@ __global__ __launch_bounds__(WinShape<R>::T) void sweep_win_kernel(WinSweepArgs 
     const uint32_t S = 1u << a.L;
     const int tid0 = threadIdx.x, lane = tid0 & 63, wv = tid0 >> 6;
 
-    double* stage = reinterpret_cast<double*>(smem);  // [kWinRegGates][kNW][32]
-    double* fst = stage + kWinRegGates * kNW * 32;    // [kNW][2]
+    double* stage = reinterpret_cast<double*>(smem);  // [a.slots][kNW][32]
+    double* fst = stage + a.slots * kNW * 32;         // [kNW][2]
     double* acc = reinterpret_cast<double*>(smem + a.acc_off);
     cpx<R>* buf = reinterpret_cast<cpx<R>*>(smem + a.buf_off);  // [2 or 4][S] re-layout buffer
 
@@ -341,8 +341,8 @@ __global__ __launch_bounds__(512) void sweep_win_tile_kernel(WinTileArgs a)
     const uint32_t S = 1u << L, ntiles = 1u << (L - TB);
     const int tid0 = threadIdx.x, lane = tid0 & 63, wv = tid0 >> 6;
 
-    double* stage = reinterpret_cast<double*>(smem);  // [kWinRegGates][kNW][32]
-    double* fst = stage + kWinRegGates * kNW * 32;    // [kNW][2]
+    double* stage = reinterpret_cast<double*>(smem);  // [a.slots][kNW][32]
+    double* fst = stage + a.slots * kNW * 32;         // [kNW][2]
     double* acc = reinterpret_cast<double*>(smem + a.acc_off);
     cpx<R>* buf = reinterpret_cast<cpx<R>*>(smem + a.buf_off);  // [RS][TSZ] re-layout buffer
     // the column's states in global memory: psi, dpsi (MODE 2), lam (MODE >= 1), dlam (MODE 2)
@@ -602,7 +602,8 @@ __global__ __launch_bounds__(512) void sweep_win_tile_kernel(WinTileArgs a)
 
 hipError_t launch_materialize_win_tile(const WinTileArgs& a, int grid, size_t lds, hipStream_t s)
 {
-    if (a.L <= 12 || a.L - 12 > 20 || a.ntb < 1 || a.rs != 1 || !a.rows) return hipErrorInvalidValue;
+    if (a.L <= 12 || a.L - 12 > 20 || a.ntb < 1 || a.rs != 1 || !a.rows || a.slots < 1 || a.slots > kWinRegGates)
+        return hipErrorInvalidValue;
     const void* k = (const void*)sweep_win_tile_kernel<3, 1>;
     if (lds > 65536) {
         hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -630,16 +631,18 @@ static hipError_t launch_wtile_rs(const WinTileArgs& a, int mode, int grid, size
 hipError_t launch_sweep_win_tile(const WinTileArgs& a, int mode, int grid, size_t lds, hipStream_t s)
 {
     // 12-qubit tiles of a larger column, the other bits listed in WTBlock::other
-    if (a.L <= 12 || a.L - 12 > 20 || a.ntb < 1) return hipErrorInvalidValue;
+    if (a.L <= 12 || a.L - 12 > 20 || a.ntb < 1 || a.slots < 1 || a.slots > kWinRegGates) return hipErrorInvalidValue;
     if (a.rs == 2) return launch_wtile_rs<2>(a, mode, grid, lds, s);
     if (a.rs == 1) return launch_wtile_rs<1>(a, mode, grid, lds, s);
     return hipErrorInvalidValue;
 }
 
-size_t win_sweep_lds_bytes(int L, int P, int fp32, size_t limit, size_t* acc_off, size_t* buf_off, int* rs, int mode)
+size_t win_sweep_lds_bytes(int L, int P, int fp32, size_t limit, size_t* acc_off, size_t* buf_off, int* rs, int mode,
+                           int slots)
 {
     const int nw = (fp32 ? WinShape<float>::T : WinShape<double>::T) / 64;
-    const size_t fixed = sizeof(double) * ((size_t)kWinRegGates * nw * 32 + 2 * nw);
+    // staging: one slot per gate of the largest window block
+    const size_t fixed = sizeof(double) * ((size_t)slots * nw * 32 + 2 * nw);
     const size_t a0 = (fixed + 255) & ~(size_t)255;
     // per parameter gate 16 complex sums (gradient, D^2 f[Z]); 8 below MODE 2
     const size_t b0 = (a0 + sizeof(double) * (mode == 2 ? 32 : 16) * (size_t)P + 255) & ~(size_t)255;
@@ -679,6 +682,7 @@ static hipError_t launch_win_r(const WinSweepArgs& a, int mode, int grid, size_t
 
 hipError_t launch_sweep_win(const WinSweepArgs& a, int mode, int fp32, int grid, size_t lds, hipStream_t s)
 {
+    if (a.slots < 1 || a.slots > kWinRegGates) return hipErrorInvalidValue;
     if (fp32) return a.rs == 4 ? launch_win_r<float, 4>(a, mode, grid, lds, s) : hipErrorInvalidValue;
     if (a.rs == 2) return launch_win_r<double, 2>(a, mode, grid, lds, s);
     if (a.rs == 1) return launch_win_r<double, 1>(a, mode, grid, lds, s);

@@ -18,6 +18,7 @@ struct WinSweepArgs {
     double* partial;           // [gridDim.x][2 + 32 P]
     int acc_off, buf_off;      // byte offsets in dynamic LDS
     int rs;                    // states per re-layout round (FP32: 4; FP64: 2, or 1 for many parameters)
+    int slots;                 // staging slots: gates of the largest window block (<= kWinRegGates)
 };
 
 // Tiled variant (FP64, L > 12): the program is cut into tile blocks acting inside 12-qubit
@@ -43,6 +44,7 @@ struct WinTileArgs {
     void* scratch;             // cpx<double> [gridDim.x][nstate][S]
     int nstate;                // 1 (f), 2 (+ gradient), 4 (+ HVP)
     int acc_off, buf_off, rs;  // as WinSweepArgs (12-qubit tiles)
+    int slots;                 // as WinSweepArgs
     void* rows;                // materialisation: cpx<double> [S][S] output rows (column b's state)
 };
 hipError_t launch_sweep_win_tile(const WinTileArgs& a, int mode, int grid, size_t lds, hipStream_t s);
@@ -54,7 +56,7 @@ hipError_t launch_materialize_win_tile(const WinTileArgs& a, int grid, size_t ld
 // states per re-layout round that fit: FP64 re-lays two states out per round, or one when the
 // accumulator of many parameter gates leaves no room for two)
 size_t win_sweep_lds_bytes(int L, int P, int fp32, size_t limit, size_t* acc_off, size_t* buf_off, int* rs,
-                           int mode = 2);
+                           int mode = 2, int slots = kWinRegGates);
 // the kernel's threads x window amplitudes cover the column (L=12 in both precisions)
 bool win_sweep_fits(int L, int fp32);
 int win_sweep_window_bits(int fp32);

@@ -216,9 +216,9 @@ def test_register_kernel_matches_oracle(mg, L, layout, depth, threads, prec):
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
 def test_window_register_kernel_matches_oracle(mg, layout, depth, prec):
     """The register-resident gate-window kernel (L=12): tied brickwalls with random
-    permutations (windows of scattered qubits) and open brickwalls, all three modes; at depth 15
-    (83 parameter gates) the FP64 kernel re-lays one state out per round (its accumulator leaves
-    no LDS room for two)."""
+    permutations (windows of scattered qubits) and open brickwalls, all three modes; the FP64
+    kernel re-lays two states out per round, or one when the accumulator of many parameter gates
+    and the staging slots of the largest window block leave no LDS room for two."""
     L = 12
     say(f"case: register window kernel L={L} {layout} depth={depth} {prec}")
     V, Z, param, wires, P, U = build_case(L, layout, depth, seed=depth * 5 + 1)
@@ -231,8 +231,12 @@ def test_window_register_kernel_matches_oracle(mg, layout, depth, prec):
         info = mg.last_sweep_info()
         assert info["kernel"] == "sweep_win_kernel"
         if prec == "fp64":
-            # one state (64 KB) or two (128 KB) per re-layout round
-            assert (info["lds_bytes"] < 100 * 1024) == (P > 79), (P, info["lds_bytes"])
+            # one state (64 KB) or two (128 KB) per re-layout round: two whenever the accumulator
+            # (32 P doubles) and at most 6 staging slots (12 KB) leave room
+            print(f"P={P}, LDS bytes {info['lds_bytes']}", flush=True)
+            assert info["lds_bytes"] <= 160 * 1024
+            if P <= 79:
+                assert info["lds_bytes"] > 128 * 1024, (P, info["lds_bytes"])
         r1 = mg.evaluate_columns(t, p, V, None, mg.EVAL_F_GRAD, precision, 7, 71)
         if prec == "fp64":
             # without the D^2 f[Z] half of the accumulator two states fit at every depth
@@ -270,7 +274,7 @@ def test_lds_kernel_f_and_gradient_two_workgroups_per_cu(mg, depth):
 def test_tile_kernels_match_oracle(mg, L, layout, depth, prec):
     """The tiled kernels for dense targets whose states exceed the LDS, all three modes on a
     column subset against the oracle: FP64 unitary gates run the tiled register-window kernel
-    (12-qubit tiles swept in 3-qubit register windows; one state per re-layout round at depth 15)
+    (12-qubit tiles swept in 3-qubit register windows; one or two states per re-layout round)
     and, with it switched off, the LDS-tile kernel; FP32 and the FP64 gates that are only
     invertible (L=12, unitarity defect 2e-11) run the LDS-tile kernel (2^12 / 2^11 amplitudes)."""
     say(f"case: tiled kernels L={L} {layout} depth={depth} {prec}")
