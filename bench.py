@@ -180,12 +180,13 @@ def served_bench(mg, njobs, seed=1, Ls=(8, 10, 12), baselines=("dense", "plain",
     return out
 
 
-def batched_bench(mg, niter, seed=0):
+def batched_bench(mg, niter, seed=0, tr_reuse=True):
     """The batched workload: 64 independent instances, L uniform in {8, 10, 12, 14}, odd depth
     3..11, targets seeded random brickwalls of 3x the depth (materialised in HBM at admission),
-    FP64; every instance runs `niter` trust-region iterations and all of them step together, each
-    round of evaluations one persistent launch per precision over all instances and basis
-    columns, repeated launch shapes replayed from captured HIP graphs."""
+    FP64; every instance runs `niter` trust-region iterations (with tr_reuse: H eta from the
+    truncated CG, f with its gradient at trial points) and all of them step together, each round
+    of evaluations one persistent launch per precision over all instances and basis columns,
+    repeated launch shapes replayed from captured HIP graphs."""
     from rqcopt_mg.workloads import batched_instances
 
     inst = batched_instances(64, seed=seed, niter=niter)
@@ -195,7 +196,7 @@ def batched_bench(mg, niter, seed=0):
     mg.set_kernel_timing(True)
     log(f"batched workload: 64 instances, {niter} trust-region iterations each (FP64) ...")
     res, st = mg.serve(jobs, batched=True, precision=mg.MG_FP64, precision_switch=0, densify_qubits=14,
-                       realtime=False, progress_s=30.0)
+                       realtime=False, progress_s=30.0, tr_reuse=tr_reuse)
     iters = sum(len(r["f_iter"]) - 1 for r in res)
     log(f"  {iters / st['wall_s']:.1f} trust-region iterations/s, {st['rounds']} launches, "
         f"{st['graph_replays']} from graphs")
@@ -213,16 +214,19 @@ def batched_bench(mg, niter, seed=0):
             "evaluations_per_s": st["requests"] / st["wall_s"], "launches": st["rounds"],
             "evaluations": st["requests"], "graph_replays": st["graph_replays"], "kernel_s": st["kernel_ms"] * 1e-3,
             "wall_s": st["wall_s"], "grad_rel_err_vs_fp64_single_path": err,
+            "hvp_sweeps": int(sum(r["nhvp"] for r in res)),
             "workload": "64 instances, L uniform in {8,10,12,14}, odd depth 3..11, targets random brickwalls of 3x "
                         "depth (materialised at admission), FP64, seed 0" +
-                        ("" if niter == 20 else f"; {niter} of the 20 iterations per instance")}
+                        ("" if niter == 20 else f"; {niter} of the 20 iterations per instance") +
+                        ("; trust region with tr_reuse" if tr_reuse else "; reference trust-region control flow")}
 
 
 def large_bench(mg):
     """The large single instance: L=16, open brickwall depth 11 (83 gates), target = 50
-    second-order Trotter steps of the XXZ chain with random fields (758 matrix-free gates);
-    one f + gradient + HVP over all 65536 columns with FP32 sweeps and with FP64 sweeps
-    (batched kernel, states in global scratch, gate windows)."""
+    second-order Trotter steps of the XXZ chain with random fields (758 gates); one f + gradient
+    + HVP over all 65536 columns with FP32 sweeps and with FP64 sweeps, first with the target
+    applied per column (batched kernel, states in global scratch, gate windows), then with the
+    target materialised once in HBM and read as dense rows (tiled kernels)."""
     from rqcopt_mg.workloads import large_instance, sweep_flops
 
     w = large_instance()
@@ -295,6 +299,8 @@ def main():
     ap.add_argument("--serve-baselines", default="dense,plain,reuse",
                     help="one-job-at-a-time FP64 runs of the same stream to compare with (subset of dense,plain,reuse)")
     ap.add_argument("--large", action="store_true", help="also run the L=16 instance (about 20 s of GPU time)")
+    ap.add_argument("--batched-reference-flow", action="store_true",
+                    help="batched workload with the reference trust-region control flow instead of tr_reuse")
     ap.add_argument("--batched-iters", type=int, default=2,
                     help="trust-region iterations per instance of the 64-instance batched workload (20 is the "
                          "full workload, about 2.5 min of GPU time; 0 skips it)")
@@ -374,7 +380,7 @@ def main():
         extra["served"] = served_bench(mg, args.serve_jobs, Ls=tuple(int(x) for x in args.serve_Ls.split(",")),
                                        baselines=tuple(x for x in args.serve_baselines.split(",") if x))
     if args.batched_iters > 0:
-        extra["batched"] = batched_bench(mg, args.batched_iters)
+        extra["batched"] = batched_bench(mg, args.batched_iters, tr_reuse=not args.batched_reference_flow)
     if args.large:
         mg.set_kernel_timing(True)
         extra["large_L16"] = large_bench(mg)
