@@ -3,7 +3,7 @@
  * gradient, Hessian and Hessian-vector product.
  *
  * This is a from-scratch FP64 restatement of the reference's algorithms, used
- * by tests/ (as the checker), by __graft_entry__.smoke() and by bench.py's
+ * by tests/ (to check results), by __graft_entry__.smoke() and by bench.py's
  * cpu_baseline leg.  The product library never links or calls it.
  *
  * Conventions (reference src/matchgate.h:5-14, src/gate.c:581-652):

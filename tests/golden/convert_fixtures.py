@@ -1,9 +1,10 @@
 """Convert the reference's HDF5 golden fixtures into HDF5-free ``.npz`` files.
 
 Run once in a container that holds the reference checkout (``/root/reference``)
-and the HDF5 command-line tools (``/opt/conda/bin/h5dump``).  The GPU box has
-neither, so the converted fixtures are committed next to this script and the
-tests read only the ``.npz`` files (``numpy.load`` with ``allow_pickle=False``).
+and the HDF5 command-line tools (``/opt/conda/bin/h5dump``).  Machines that run
+the tests need neither: the converted fixtures are committed next to this script
+and the tests read only the ``.npz`` files (``numpy.load`` with
+``allow_pickle=False``).
 
 Every dataset of each listed file is exported with ``h5dump -b LE`` (raw
 little-endian bytes) and stored under its HDF5 path with ``/`` replaced by
