@@ -325,4 +325,60 @@ __device__ inline void outer_acc(cpx<R>* acc, const Quad<R>& d, const Quad<R>& p
     acc[7] = cfma(d.w, p.w, acc[7]);
 }
 
+// One gate on N of a thread's quadruples (q0, q0 + stride, ...), which are disjoint: all loads
+// first, then the arithmetic and the stores, so that the quadruples' latencies overlap.  The
+// accumulators see the quadruples in increasing q, as a loop over them one at a time would.
+template <typename R, int MODE, bool SW, int N>
+__device__ inline void fwd_quads(uint32_t q0, uint32_t stride, int lo, int hi, uint32_t olo, uint32_t ohi,
+                                 const cpx<R>* cf, const cpx<R>* pin, cpx<R>* pout, const cpx<R>* din, cpx<R>* dout)
+{
+    uint32_t sb[N];
+    Quad<R> p[N], d[N];
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        sb[i] = swz_idx<R, SW>(quad_base(q0 + i * stride, lo, hi));
+        p[i] = load_quad_o(pin, sb[i], olo, ohi);
+        if constexpr (MODE == 2) d[i] = load_quad_o(din, sb[i], olo, ohi);
+    }
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        if constexpr (MODE == 2) store_quad_o(dout, sb[i], olo, ohi, gapply_add(cf, d[i], gapply(cf + 16, p[i])));
+        store_quad_o(pout, sb[i], olo, ohi, gapply(cf, p[i]));
+    }
+}
+
+// the reverse step (states resident, psi recovered with V^{-1}) on N disjoint quadruples
+template <typename R, int MODE, bool SW, int N>
+__device__ inline void rev_quads(uint32_t q0, uint32_t stride, int lo, int hi, uint32_t olo, uint32_t ohi,
+                                 const cpx<R>* cf, cpx<R>* lam, cpx<R>* psi, cpx<R>* dlam, cpx<R>* dpsi, cpx<R>* ga,
+                                 cpx<R>* ha)
+{
+    uint32_t sb[N];
+    Quad<R> l[N], pn[N], dn[N], dl[N];
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        sb[i] = swz_idx<R, SW>(quad_base(q0 + i * stride, lo, hi));
+        l[i] = load_quad_o(lam, sb[i], olo, ohi);
+        pn[i] = load_quad_o(psi, sb[i], olo, ohi);
+        if constexpr (MODE == 2) {
+            dn[i] = load_quad_o(dpsi, sb[i], olo, ohi);
+            dl[i] = load_quad_o(dlam, sb[i], olo, ohi);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        const Quad<R> p = gapply(cf + 8, pn[i]);
+        store_quad_o(psi, sb[i], olo, ohi, p);
+        outer_acc(ga, l[i], p);
+        if constexpr (MODE == 2) {
+            const Quad<R> dp = gapply(cf + 8, qsub(dn[i], gapply(cf + 16, p)));
+            store_quad_o(dpsi, sb[i], olo, ohi, dp);
+            outer_acc(ha, dl[i], p);
+            outer_acc(ha, l[i], dp);
+            store_quad_o(dlam, sb[i], olo, ohi, gapplyT_add(cf, dl[i], gapplyT(cf + 16, l[i])));
+        }
+        store_quad_o(lam, sb[i], olo, ohi, gapplyT(cf, l[i]));
+    }
+}
+
 }  // namespace mg

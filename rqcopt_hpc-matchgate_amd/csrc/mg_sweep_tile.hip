@@ -127,15 +127,13 @@ __global__ __launch_bounds__(1 << tile_lt<R>()) void sweep_tile_kernel(TileArgs<
                     cpx<R> cf[24];
                     load_coef<R, 8>(a.coef + (size_t)g * kCoefPerInst, cf);
                     if constexpr (MODE == 2) load_coef<R, 8>(a.coef + (size_t)g * kCoefPerInst + 16, cf + 16);
-                    for (uint32_t q = tid; q < Qt; q += T) {
-                        const uint32_t base = quad_base(q, in.lo, in.hi);
-                        const Quad<R> p = load_quad<R, true>(t_psi, base, in.lo, in.hi);
-                        if constexpr (MODE == 2) {
-                            const Quad<R> d = load_quad<R, true>(t_dpsi, base, in.lo, in.hi);
-                            store_quad<R, true>(t_dpsi, base, in.lo, in.hi, gapply_add(cf, d, gapply(cf + 16, p)));
-                        }
-                        store_quad<R, true>(t_psi, base, in.lo, in.hi, gapply(cf, p));
-                    }
+                    const uint32_t olo = swz_idx<R, true>(1u << in.lo), ohi = swz_idx<R, true>(1u << in.hi);
+                    uint32_t q = tid;
+                    if constexpr (MODE == 2)
+                        for (; q + T < Qt; q += 2 * T)
+                            fwd_quads<R, MODE, true, 2>(q, T, in.lo, in.hi, olo, ohi, cf, t_psi, t_psi, t_dpsi, t_dpsi);
+                    for (; q < Qt; q += T)
+                        fwd_quads<R, MODE, true, 1>(q, T, in.lo, in.hi, olo, ohi, cf, t_psi, t_psi, t_dpsi, t_dpsi);
                     __syncthreads();
                 }
                 // store (each thread its own positions, so the next tile's load needs no barrier);
@@ -204,23 +202,12 @@ __global__ __launch_bounds__(1 << tile_lt<R>()) void sweep_tile_kernel(TileArgs<
                             ga[e] = {R(0), R(0)};
                             ha[e] = {R(0), R(0)};
                         }
-                        for (uint32_t q = tid; q < Qt; q += T) {
-                            const uint32_t base = quad_base(q, in.lo, in.hi);
-                            const Quad<R> l = load_quad<R, true>(t_lam, base, in.lo, in.hi);
-                            const Quad<R> p = gapply(cf + 8, load_quad<R, true>(t_psi, base, in.lo, in.hi));
-                            store_quad<R, true>(t_psi, base, in.lo, in.hi, p);
-                            outer_acc(ga, l, p);
-                            if constexpr (MODE == 2) {
-                                const Quad<R> dn = load_quad<R, true>(t_dpsi, base, in.lo, in.hi);
-                                const Quad<R> dp = gapply(cf + 8, qsub(dn, gapply(cf + 16, p)));
-                                store_quad<R, true>(t_dpsi, base, in.lo, in.hi, dp);
-                                const Quad<R> dl = load_quad<R, true>(t_dlam, base, in.lo, in.hi);
-                                outer_acc(ha, dl, p);
-                                outer_acc(ha, l, dp);
-                                store_quad<R, true>(t_dlam, base, in.lo, in.hi, gapplyT_add(cf, dl, gapplyT(cf + 16, l)));
-                            }
-                            store_quad<R, true>(t_lam, base, in.lo, in.hi, gapplyT(cf, l));
-                        }
+                        const uint32_t olo = swz_idx<R, true>(1u << in.lo), ohi = swz_idx<R, true>(1u << in.hi);
+                        uint32_t q = tid;
+                        for (; q + T < Qt; q += 2 * T)
+                            rev_quads<R, MODE, true, 2>(q, T, in.lo, in.hi, olo, ohi, cf, t_lam, t_psi, t_dlam, t_dpsi, ga, ha);
+                        if (q < Qt)
+                            rev_quads<R, MODE, true, 1>(q, T, in.lo, in.hi, olo, ohi, cf, t_lam, t_psi, t_dlam, t_dpsi, ga, ha);
                         R v[NV];
 #pragma unroll
                         for (int e = 0; e < 8; e++) {
