@@ -4,13 +4,16 @@
  * :54-75 is dropped: one GPU, no rank split).
  *
  * U is given, as in the reference, by a linear_func callback.  Because the GPU
- * needs U on the device, the library materialises the callback once as a dense
- * matrix (2^n host calls, from several host threads, so the callback must be
- * reentrant exactly as under the reference's OpenMP loop) and caches it keyed by
- * (ufunc, udata, nqubits).  Before a cached matrix is reused the callback is run on
- * four probe columns whose results must match the cached ones bitwise, else U is
- * materialised again; mg_target_cache_clear() drops every cached matrix.  Callers that know their target should use the
- * descriptor API of mg_gpu.h instead (dense or brickwall targets on the device).
+ * needs U on the device, every call materialises the callback as dense rows on the
+ * device (2^n host calls, from several host threads, so the callback must be
+ * reentrant exactly as under the reference's OpenMP loop): like the reference, which
+ * calls ufunc for every column on every evaluation (src/matchgate_target.c:795), a
+ * changed udata takes effect at the next call.  A caller whose map does not change
+ * may call mg_target_cache_reuse(1): materialised targets are then kept per
+ * (ufunc, udata, nqubits) and reused until mg_target_cache_clear() (at most 8
+ * targets and 64 GiB of device rows; the least recently used one is dropped first).
+ * Callers that know their target should use the descriptor API of mg_gpu.h instead
+ * (dense or brickwall targets on the device).
  *
  * Return codes as in the reference: 0 ok, -1 allocation/device error, -2 ufunc
  * failed (src/matchgate_target.c:400-487).
@@ -58,6 +61,11 @@ int parallel_matchgate_brickwall_unitary_target_gradient_hessian(linear_func ufu
 
 /* drop every cached materialised linear_func target (no reference counterpart) */
 void mg_target_cache_clear(void);
+/* on != 0: keep materialised linear_func targets per (ufunc, udata, nqubits) and reuse them
+ * until mg_target_cache_clear(); the caller promises that the map does not change meanwhile.
+ * 0 (default): materialise on every call, as the reference evaluates ufunc on every call
+ * (no reference counterpart) */
+void mg_target_cache_reuse(int on);
 
 #ifdef __cplusplus
 }

@@ -3,8 +3,22 @@
  * reference's names, arguments and return values (0 on success, negative on failure, a message
  * on stderr). */
 #pragma once
+#include <math.h>
 #include "config.h"
 #include "mg_hdf5.h"
+
+/* relative distance |x - y| / max(|x| + |y|, eps) of two numbers (src/util.h:13-16) */
+#ifdef __cplusplus
+static inline double reldist(numeric x, numeric y, double eps)
+{
+    return std::abs(x - y) / fmax(std::abs(x) + std::abs(y), eps);
+}
+#else
+static inline double reldist(numeric x, numeric y, double eps)
+{
+    return cabs(x - y) / fmax(cabs(x) + cabs(y), eps);
+}
+#endif
 
 #ifdef __cplusplus
 extern "C" {

@@ -53,6 +53,7 @@ _sig("orc_apply_program", [_P, _I, _P, _P, _I, _P, _P])
 _sig("orc_program_forward", [_P, _I, _P, _P, _I, _P, _P, _P])
 _sig("orc_program_backward", [_P, _I, _P, _P, _I, _P, _P, _P, _P])
 _sig("orc_program_backward_hessian", [_P, _I, _I, _P, _P, _I, _P, _P, _P, _P, _P])
+_sig("orc_ti_brickwall_backward", [_P, _I, _P, _I, _P, _P, _P, _P])
 _sig("orc_ti_brickwall_backward_hessian", [_P, _I, _P, _I, _P, _P, _P, _P, _P])
 _sig("orc_target", [_P, _P, _I, _P, _P, _I, _I, _P])
 _sig("orc_target_gradient", [_P, _P, _I, _I, _P, _P, _I, _I, _P, _P])
@@ -184,6 +185,16 @@ def program_backward_hessian(gates, param, wires, L, cache, dout, P=None):
     _lib.orc_program_backward_hessian(_p(gates), P, len(param), _p(param), _p(wires), L, _p(cache), _p(dout),
                                       _p(dpsi), _p(dgrad), _p(hess))
     return dpsi, dgrad.reshape(P, 8), hess
+
+
+def ti_brickwall_backward(vlist, perms, L, cache, dout):
+    """TI gradient sweep of one column (reference src/matchgate_brickwall.c:266-306)"""
+    vlist, perms, cache, dout = _c(vlist), _i(perms), _c(cache), _c(dout)
+    nl = perms.shape[0]
+    dpsi = np.empty(1 << L, np.complex128)
+    dgrad = np.empty(8 * nl, np.complex128)
+    _lib.orc_ti_brickwall_backward(_p(vlist), nl, _p(perms), L, _p(cache), _p(dout), _p(dpsi), _p(dgrad))
+    return dpsi, dgrad.reshape(nl, 8)
 
 
 def ti_brickwall_backward_hessian(vlist, perms, L, cache, dout):

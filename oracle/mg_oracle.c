@@ -261,6 +261,32 @@ void orc_program_backward_hessian(const ocplx* gates, int P, int G, const int* p
     free(A); free(t0); free(t1); free(gc);
 }
 
+/* reference src/matchgate_brickwall.c:266-306 (matchgate_ti_brickwall_unitary_backward): the
+ * gradient of each layer from its last pair only (cache state k = (i+1) L/2 - 1, wires
+ * perms[i][L-2], perms[i][L-1]), scaled by L/2; the backpropagated state passes through those
+ * pairs only.  The reference alternates two buffers by the parity of k, which chains the
+ * layers in this order when L/2 is odd; for even L/2 it reads a buffer not written by the
+ * previous layer, and this restatement keeps the chain. */
+void orc_ti_brickwall_backward(const ocplx* vlist, int nlayers, const int* perms, int L, const ocplx* cache,
+                               const ocplx* dout, ocplx* dpsi, ocplx* dgrad)
+{
+    const long S = 1L << L;
+    const int per = L / 2;
+    ocplx* cur = malloc(S * sizeof(ocplx));
+    ocplx* nxt = malloc(S * sizeof(ocplx));
+    memcpy(cur, dout, S * sizeof(ocplx));
+    for (int i = nlayers - 1; i >= 0; i--) {
+        const int k = (i + 1) * per - 1;
+        ocplx dg[8];
+        orc_apply_gate_backward(vlist + 8 * i, perms[(size_t)i * L + L - 2], perms[(size_t)i * L + L - 1], L,
+                                cache + (size_t)k * S, cur, nxt, dg);
+        for (int e = 0; e < 8; e++) dgrad[8 * i + e] = per * dg[e];
+        ocplx* t = cur; cur = nxt; nxt = t;
+    }
+    memcpy(dpsi, cur, S * sizeof(ocplx));
+    free(cur); free(nxt);
+}
+
 /* reference src/matchgate_brickwall.c:308-435 (matchgate_ti_brickwall_unitary_backward_hessian) */
 void orc_ti_brickwall_backward_hessian(const ocplx* vlist, int nlayers, const int* perms, int L,
                                        const ocplx* cache, const ocplx* dout, ocplx* dpsi, ocplx* dgrad, ocplx* hess)

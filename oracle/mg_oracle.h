@@ -46,6 +46,8 @@ void orc_program_backward(const ocplx* gates, int G, const int* param, const int
 void orc_program_backward_hessian(const ocplx* gates, int P, int G, const int* param, const int* wires, int L,
                                   const ocplx* cache, const ocplx* dout, ocplx* dpsi, ocplx* dgrad, ocplx* hess);
 /* translation-invariant shortcut (reference src/matchgate_brickwall.c:308-435), tied brickwall only */
+void orc_ti_brickwall_backward(const ocplx* vlist, int nlayers, const int* perms, int L, const ocplx* cache,
+                               const ocplx* dout, ocplx* dpsi, ocplx* dgrad);
 void orc_ti_brickwall_backward_hessian(const ocplx* vlist, int nlayers, const int* perms, int L,
                                        const ocplx* cache, const ocplx* dout, ocplx* dpsi, ocplx* dgrad, ocplx* hess);
 

@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <list>
 #include <map>
 #include <memory>
 #include <tuple>
@@ -64,53 +65,34 @@ zd* Zp(struct matchgate* g) { return reinterpret_cast<zd*>(g); }
 
 mg::EvalStats g_last_stats;
 
-// ---- materialised linear_func targets, cached per (ufunc, udata, nqubits) ----
-// A cache entry also keeps a few probe columns of U.  Before an entry is reused the
-// callback is called again on those basis vectors and the results must match
-// bitwise; otherwise (a function pointer reused for another callback, or udata whose
-// contents changed) the target is materialised again.
-struct CachedTarget {
-    std::unique_ptr<mg::Target> t;
-    std::vector<size_t> probe_cols;
-    std::vector<numeric> probe_vals;
+// ---- targets of linear_func callbacks ----
+// The reference calls ufunc for every basis column on every evaluation
+// (src/matchgate_target.c:795), so a map whose udata changed between two calls takes
+// effect at once.  The entry points do the same by default: every call materialises the
+// callback's map again (2^n host calls) into a device target that lives until the next
+// call.  With mg_target_cache_reuse(1) a caller that promises an unchanged map keeps the
+// materialised targets per (ufunc, udata, nqubits) instead, until mg_target_cache_clear()
+// or until the least recently used entry is evicted (at most kCacheEntries targets and
+// kCacheBytes of device rows).
+constexpr size_t kCacheEntries = 8;
+constexpr double kCacheBytes = 64.0 * (1ull << 30);
+
+using CacheKey = std::tuple<void*, void*, int>;
+
+struct TargetCache {
+    bool reuse = false;
+    std::list<std::pair<CacheKey, std::unique_ptr<mg::Target>>> lru;  // most recently used first
+    std::unique_ptr<mg::Target> last;                                  // the per-call target (reuse off)
 };
 
-std::map<std::tuple<void*, void*, int>, CachedTarget>& target_cache()
+TargetCache& target_cache()
 {
-    static std::map<std::tuple<void*, void*, int>, CachedTarget> c;
+    static TargetCache c;
     return c;
 }
 
-std::vector<size_t> probe_columns(int L)
-{
-    const size_t S = (size_t)1 << L;
-    std::vector<size_t> cols = {0, S - 1, (S / 3) | 1, (2 * S / 3) & ~(size_t)1};
-    std::sort(cols.begin(), cols.end());
-    cols.erase(std::unique(cols.begin(), cols.end()), cols.end());
-    return cols;
-}
-
-// U e_b for the probe columns; false if ufunc fails
-bool probe(linear_func ufunc, void* udata, int L, const std::vector<size_t>& cols, std::vector<numeric>* vals)
-{
-    const size_t S = (size_t)1 << L;
-    statevector psi{}, out{};
-    if (allocate_statevector(L, &psi) < 0 || allocate_statevector(L, &out) < 0) return false;
-    vals->clear();
-    bool ok = true;
-    for (size_t b : cols) {
-        std::memset((void*)psi.data, 0, S * sizeof(numeric));
-        psi.data[b] = 1.0;
-        if (ufunc(&psi, udata, &out) < 0) {
-            ok = false;
-            break;
-        }
-        vals->insert(vals->end(), out.data, out.data + S);
-    }
-    free_statevector(&psi);
-    free_statevector(&out);
-    return ok;
-}
+// device bytes of a materialised target: FP64 rows and the FP32 copy made on first FP32 use
+double target_bytes(int L) { return 24.0 * (double)((size_t)1 << L) * (double)((size_t)1 << L); }
 
 // Calls ufunc on every basis vector (several host threads, as the reference's
 // OpenMP column loop does, src/matchgate_target.c:788-805) and stores the seed rows
@@ -158,29 +140,28 @@ mg::Target* materialise(linear_func ufunc, void* udata, int L, int* err)
 mg::Target* cached_target(linear_func ufunc, void* udata, int L, int* err)
 {
     *err = 0;
-    auto key = std::make_tuple((void*)ufunc, udata, L);
-    auto& c = target_cache();
-    const std::vector<size_t> cols = probe_columns(L);
-    std::vector<numeric> vals;
-    if (!probe(ufunc, udata, L, cols, &vals)) {
-        std::fprintf(stderr, "call of 'ufunc' failed\n");
-        *err = -2;
-        return nullptr;
+    TargetCache& c = target_cache();
+    if (!c.reuse) {
+        c.last.reset();
+        mg::Target* t = materialise(ufunc, udata, L, err);
+        c.last.reset(t);
+        return t;
     }
-    auto it = c.find(key);
-    if (it != c.end()) {
-        const CachedTarget& e = it->second;
-        if (e.probe_vals.size() == vals.size() &&
-            std::memcmp(e.probe_vals.data(), vals.data(), vals.size() * sizeof(numeric)) == 0)
-            return e.t.get();
-        c.erase(it);
-    }
+    const CacheKey key = std::make_tuple((void*)ufunc, udata, L);
+    for (auto it = c.lru.begin(); it != c.lru.end(); ++it)
+        if (it->first == key) {
+            c.lru.splice(c.lru.begin(), c.lru, it);
+            return c.lru.front().second.get();
+        }
     mg::Target* t = materialise(ufunc, udata, L, err);
     if (!t) return nullptr;
-    CachedTarget& e = c[key];
-    e.t.reset(t);
-    e.probe_cols = cols;
-    e.probe_vals = std::move(vals);
+    c.lru.emplace_front(key, std::unique_ptr<mg::Target>(t));
+    double bytes = 0;
+    for (const auto& e : c.lru) bytes += target_bytes(std::get<2>(e.first));
+    while (c.lru.size() > 1 && (c.lru.size() > kCacheEntries || bytes > kCacheBytes)) {
+        bytes -= target_bytes(std::get<2>(c.lru.back().first));
+        c.lru.pop_back();
+    }
     return t;
 }
 
@@ -262,7 +243,17 @@ extern "C" {
 
 // =================== reference target functions (include/matchgate_target.h) ===================
 
-void mg_target_cache_clear(void) { target_cache().clear(); }
+void mg_target_cache_clear(void)
+{
+    target_cache().lru.clear();
+    target_cache().last.reset();
+}
+
+void mg_target_cache_reuse(int on)
+{
+    target_cache().reuse = on != 0;
+    if (!on) target_cache().lru.clear();
+}
 
 int matchgate_circuit_unitary_target(linear_func ufunc, void* udata, const struct matchgate gates[], const int ngates,
                                      const int wires[], const int nqubits, numeric* fval)
@@ -787,16 +778,31 @@ static void brickwall_opt(linear_func ufunc, void* udata, const struct matchgate
     params->g_func = NULL;
     params->g_data = NULL;
     params->g_iter = NULL;
-    if (!check_brickwall(nlayers, nqubits, perms)) return;
+    // on any failure the caller gets NaN objective values and the start gates, never
+    // uninitialised output, and the reason on stderr (the reference reports its failures there)
+    auto fail = [&](const char* what) {
+        std::fprintf(stderr, "%s: %s\n", what, mg::last_error());
+        for (int k = 0; k <= niter; k++) {
+            if (f_iter) f_iter[k] = NAN;
+            if (f_citer) f_citer[k] = numeric(NAN, NAN);
+        }
+        if (vlist_opt && vlist_start && nlayers > 0) std::memcpy(vlist_opt, vlist_start, sizeof(struct matchgate) * nlayers);
+    };
+    const char* name = hvp ? "optimize_matchgate_brickwall_circuit_hvp" : "optimize_matchgate_brickwall_circuit_hmat";
+    if (!check_brickwall(nlayers, nqubits, perms)) return fail(name);
     int err;
     mg::Target* t = cached_target(ufunc, udata, nqubits, &err);
-    if (!t) return;
+    if (!t) {
+        if (err == -2) mg::set_error("call of 'ufunc' failed");
+        return fail(name);
+    }
     mg_program prog;
     prog.p = mg::brickwall_program(nqubits, nlayers, perms);
     mg_target th;
     th.t.reset(t);
-    mg_optimize(&th, &prog, vlist_start, params, niter, hvp, MG_FP64, 0, f_iter, f_citer, vlist_opt, nullptr);
-    (void)th.t.release();  // owned by the cache
+    const int ret = mg_optimize(&th, &prog, vlist_start, params, niter, hvp, MG_FP64, 0, f_iter, f_citer, vlist_opt, nullptr);
+    (void)th.t.release();  // owned by the target cache
+    if (ret < 0) fail(name);
 }
 
 void optimize_matchgate_brickwall_circuit_hmat(linear_func ufunc, void* udata, const struct matchgate vlist_start[],

@@ -39,13 +39,6 @@ struct Bits<float> {
 
 // One halving step at offset O for the value pair (a, b) = (v[k], v[k+H]):
 // lanes with bit O clear return a + partner's a, lanes with it set b + partner's b.
-#ifndef MG_PERMLANE_BUTTERFLY
-#define MG_PERMLANE_BUTTERFLY 1
-#endif
-#ifndef MG_SGPR_COEF
-#define MG_SGPR_COEF 1
-#endif
-
 // With the permlane swaps (offsets 32 and 16) no lane select is needed: the swap leaves
 // (a_lo, b_lo) in one register and (a_hi, b_hi) in the other (halves of 32 lanes, or
 // alternating rows of 16), so their sum is a + partner's a where bit O is clear and
@@ -53,9 +46,7 @@ struct Bits<float> {
 template <int O>
 __device__ inline float halve_pair(float a, float b, int lane)
 {
-    const bool up = (lane & O) != 0;
-    if constexpr (MG_PERMLANE_BUTTERFLY && (O == 32 || O == 16)) {
-        (void)up;
+    if constexpr (O == 32 || O == 16) {
         const uint32_t ua = __builtin_bit_cast(uint32_t, a), ub = __builtin_bit_cast(uint32_t, b);
         uint32_t na, nb;
         if constexpr (O == 32) {
@@ -69,6 +60,7 @@ __device__ inline float halve_pair(float a, float b, int lane)
         }
         return __builtin_bit_cast(float, na) + __builtin_bit_cast(float, nb);
     } else {
+        const bool up = (lane & O) != 0;
         const float send = up ? a : b;
         const float keep = up ? b : a;
         return keep + xlane<O>(send);
@@ -78,8 +70,7 @@ __device__ inline float halve_pair(float a, float b, int lane)
 template <int O>
 __device__ inline double halve_pair(double a, double b, int lane)
 {
-    const bool up = (lane & O) != 0;
-    if constexpr (MG_PERMLANE_BUTTERFLY && (O == 32 || O == 16)) {
+    if constexpr (O == 32 || O == 16) {
         const uint64_t ua = __builtin_bit_cast(uint64_t, a), ub = __builtin_bit_cast(uint64_t, b);
         const uint32_t al = (uint32_t)ua, ah = (uint32_t)(ua >> 32), bl = (uint32_t)ub, bh = (uint32_t)(ub >> 32);
         uint32_t nal, nbl, nah, nbh;
@@ -92,11 +83,11 @@ __device__ inline double halve_pair(double a, double b, int lane)
             auto r1 = __builtin_amdgcn_permlane16_swap(ah, bh, false, false);
             nal = r0[0]; nbl = r0[1]; nah = r1[0]; nbh = r1[1];
         }
-        (void)up;
         const double na = __builtin_bit_cast(double, (uint64_t)nal | ((uint64_t)nah << 32));
         const double nb = __builtin_bit_cast(double, (uint64_t)nbl | ((uint64_t)nbh << 32));
         return na + nb;
     } else {
+        const bool up = (lane & O) != 0;
         const double send = up ? a : b;
         const double keep = up ? b : a;
         return keep + xlane<O>(send);
@@ -142,7 +133,7 @@ __device__ inline void load_coef(const cpx<R>* src, cpx<R>* dst)
         // slow the FP32 kernels, which have VGPRs to spare.  (Scalar loads through the constant
         // address space were slower still in the LDS-resident kernels: their waits share the
         // lgkm counter with the LDS traffic.)
-        if constexpr (MG_SGPR_COEF && sizeof(R) == 8) dst[k] = {uniform(src[k].re), uniform(src[k].im)};
+        if constexpr (sizeof(R) == 8) dst[k] = {uniform(src[k].re), uniform(src[k].im)};
         else dst[k] = src[k];
     }
 }
@@ -168,7 +159,7 @@ struct Quad {
 };
 
 // LDS position of amplitude a.  A wave's LDS access is served in groups of lanes
-// that together cover all banks once (MG_LDS_BANK_BYTES per cycle): 8-byte (FP32
+// that together cover all banks once (kLdsBankBytes per cycle): 8-byte (FP32
 // complex) elements fill a group with BANK/8 lanes, 16-byte (FP64 complex) ones with
 // BANK/16.  A quadruple access with a low wire bit keeps one of the group's index
 // bits constant across the lanes and would collide.  XOR-ing the two index bits just
@@ -176,9 +167,7 @@ struct Quad {
 // nearest-neighbour pair (k, k+1) conflict-free: consecutive rows of the 2-column
 // mask matrix are linearly independent and row B-1 has its second bit set.  Only
 // the low B bits change, so the map permutes every aligned 2^(B+2)-element block.
-#ifndef MG_LDS_BANK_BYTES
-#define MG_LDS_BANK_BYTES 128
-#endif
+constexpr int kLdsBankBytes = 128;
 template <int B>
 struct SwzMasks;
 template <>
@@ -191,7 +180,7 @@ struct SwzMasks<5> { static constexpr uint32_t m0 = 21u, m1 = 26u; };
 template <typename R>
 __device__ inline uint32_t lds_swz(uint32_t a)
 {
-    constexpr int lanes = MG_LDS_BANK_BYTES / (int)sizeof(cpx<R>);
+    constexpr int lanes = kLdsBankBytes / (int)sizeof(cpx<R>);
     constexpr int B = lanes == 8 ? 3 : (lanes == 16 ? 4 : 5);
     return a ^ (((a >> B) & 1u) * SwzMasks<B>::m0) ^ (((a >> (B + 1)) & 1u) * SwzMasks<B>::m1);
 }

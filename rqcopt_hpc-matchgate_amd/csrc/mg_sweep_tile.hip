@@ -23,9 +23,6 @@
 #include "mg_quad.hpp"
 #include "mg_sweep_tile.hpp"
 
-#ifndef MG_TILE_LT64
-#define MG_TILE_LT64 9
-#endif
 
 namespace mg {
 
@@ -35,9 +32,9 @@ namespace {
 template <typename R>
 constexpr int tile_kw() { return sizeof(R) == 8 ? 11 : 12; }
 
-// threads per workgroup (log2): 512 (256 measured the same for FP64)
+// threads per workgroup (log2): 512 in both precisions (256 measured the same for FP64)
 template <typename R>
-constexpr int tile_lt() { return sizeof(R) == 8 ? MG_TILE_LT64 : 9; }
+constexpr int tile_lt() { return 9; }
 
 __device__ inline uint32_t scatter_bits(uint32_t x, const int* pos, int n)
 {

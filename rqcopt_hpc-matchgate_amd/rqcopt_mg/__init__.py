@@ -127,6 +127,7 @@ _sig("mg_set_tile_kernel", [_I], None)
 _sig("mg_last_kernel_ms", [], ctypes.c_float)
 _sig("mg_last_sweep_info", [_P, _P, _P, _P], None)
 _sig("mg_target_cache_clear", [], None)
+_sig("mg_target_cache_reuse", [_I], None)
 _sig("mg_evaluate_batch", [ctypes.POINTER(Request), _I])
 _sig("mg_batch_set_kernel", [_I, _I, _I])
 _sig("mg_batch_get_kernel", [_I, _P, _P], None)

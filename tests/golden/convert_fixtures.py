@@ -44,6 +44,11 @@ TEST_FILES = [
     "test_project_tangent_matchgate_cplx",
     "test_polar_matchgate_factor_cplx",
     "test_real_to_tangent_matchgate_cplx",
+    # translation-invariant Hessian shortcut (test/test_matchgate_target.c:506-772)
+    "test_matchgate_ti_brickwall_unitary_target_gradient_hessian_cplx",
+    "test_matchgate_ti_brickwall_unitary_target_gradient_hessian2_cplx",
+    "test_matchgate_ti_brickwall_unitary_target_gradient_hessian3_cplx",
+    "test_matchgate_ti_brickwall_unitary_target_gradient_hessian4_cplx",
 ]
 
 # trust-region inputs and outputs of the L=8 Fermi-Hubbard example
@@ -56,6 +61,12 @@ EXAMPLE_FILES = [
     (f"{HUB}/opt_out/q8/hubbard1d_suzuki2_n5_q8_u0_t0.25s_g1.50_iter10_inv0_opt.hdf5", "hubbard1d_suzuki2_n5_q8_iter10_opt"),
     (f"{HUB}/opt_out/q8/hubbard1d_suzuki2_n5_q8_u0_t0.25s_g1.50_iter30_inv0_opt.hdf5", "hubbard1d_suzuki2_n5_q8_iter30_opt"),
     (f"{HUB}/opt_out/q8/hubbard1d_suzuki2_n9_q8_u0_t0.25s_g1.50_iter10_inv0_opt.hdf5", "hubbard1d_suzuki2_n9_q8_iter10_opt"),
+    (f"{HUB}/opt_in/q8/hubbard1d_suzuki2_n13_q8_u29_t0.25s_g1.50_init.hdf5", "hubbard1d_suzuki2_n13_q8_init"),
+    (f"{HUB}/opt_out/q8/hubbard1d_suzuki2_n13_q8_u0_t0.25s_g1.50_iter10_inv0_opt.hdf5", "hubbard1d_suzuki2_n13_q8_iter10_opt"),
+    (f"{HUB}/opt_in/q8/hubbard1d_suzuki2_n21_q8_u29_t0.25s_g1.50_init.hdf5", "hubbard1d_suzuki2_n21_q8_init"),
+    (f"{HUB}/opt_out/q8/hubbard1d_suzuki2_n21_q8_u0_t0.25s_g1.50_iter10_inv0_opt.hdf5", "hubbard1d_suzuki2_n21_q8_iter10_opt"),
+    (f"{HUB}/opt_in/q8/hubbard1d_suzuki4_n21_q8_u29_t0.25s_g1.50_init.hdf5", "hubbard1d_suzuki4_n21_q8_init"),
+    (f"{HUB}/opt_out/q8/hubbard1d_suzuki4_n21_q8_u0_t0.25s_g1.50_iter10_inv0_opt.hdf5", "hubbard1d_suzuki4_n21_q8_iter10_opt"),
 ]
 
 _DTYPES = {

@@ -110,3 +110,25 @@ def test_ctypes_structs_match_the_c_header(tmp_path):
         assert got[(cname, "size")] == ctypes.sizeof(py), cname
         for fname, _ in py._fields_:
             assert got[(cname, fname)] == getattr(py, fname).offset, (cname, fname)
+
+
+def test_reldist_in_the_header(tmp_path):
+    """reldist (src/util.h:13-16) is usable from C and C++ callers of include/util.h."""
+    import subprocess
+
+    inc = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
+    src = """#include <stdio.h>
+#include "util.h"
+int main(void) {
+    numeric x = 3.0, y = 4.0;
+    printf("%.17g %.17g\\n", reldist(x, y, 1e-14), reldist(0.0, 0.0, 2.0));
+    return 0;
+}
+"""
+    for cc, ext in (("gcc", "c"), ("g++", "cpp")):
+        f = tmp_path / f"r.{ext}"
+        f.write_text(src)
+        exe = tmp_path / f"r_{ext}"
+        subprocess.run([cc, "-I", inc, str(f), "-o", str(exe), "-lm"], check=True)
+        a, b = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()
+        assert abs(float(a) - 1.0 / 7.0) < 1e-16 and float(b) == 0.0
