@@ -60,6 +60,19 @@ int mg_program_ninst(const mg_program* p);
 int mg_evaluate(const mg_target* t, const mg_program* p, const struct matchgate gates[], const struct matchgate Z[],
                 int what, int precision, numeric* f, struct matchgate dgrad[], struct matchgate dhz[]);
 
+/* mg_evaluate without device allocation (no reference counterpart: for callers that cannot
+ * allocate device memory, e.g. code under HIP graph capture or tight serving loops).  Every
+ * device buffer of the sweep (gate descriptors, per-workgroup partial sums, the reduced
+ * result) comes from a static 32 MiB array that each device holds for the life of the
+ * process.  Sweeps that need more -- states that fit neither the LDS nor registers (more
+ * than 12 qubits), checkpointed sweeps of gates that are not unitary, or partial sums beyond
+ * the array -- fail with an error, as does an FP32 evaluation of a target whose FP32 copy
+ * one FP32 mg_evaluate has not yet made.  Results equal mg_evaluate's.  Calls share the
+ * array: calls that overlap on one device (several host threads) race and must not be made. */
+int mg_evaluate_noalloc(const mg_target* t, const mg_program* p, const struct matchgate gates[],
+                        const struct matchgate Z[], int what, int precision, numeric* f, struct matchgate dgrad[],
+                        struct matchgate dhz[]);
+
 /* the same over the basis columns [b0, b1) only: partial sums that add up to the full
  * trace (one GPU's share when the columns are split across devices) */
 int mg_evaluate_columns(const mg_target* t, const mg_program* p, const struct matchgate gates[],

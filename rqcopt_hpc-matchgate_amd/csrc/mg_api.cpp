@@ -532,6 +532,23 @@ int mg_evaluate(const mg_target* t, const mg_program* p, const struct matchgate 
     return 0;
 }
 
+int mg_evaluate_noalloc(const mg_target* t, const mg_program* p, const struct matchgate gates[],
+                        const struct matchgate Z[], int what, int precision, numeric* f, struct matchgate dgrad[],
+                        struct matchgate dhz[])
+{
+    if (!t || !p || !gates) {
+        mg::set_error("mg_evaluate_noalloc: null argument");
+        return -1;
+    }
+    mg::EvalOut out;
+    if (mg::evaluate(*t->t, p->p, Zp(gates), Z ? Zp(Z) : nullptr, what, precision, &out, 0, -1, &g_last_stats, true) < 0)
+        return -1;
+    if (f) *f = out.f;
+    if (dgrad && what >= 1) std::copy(out.grad.begin(), out.grad.end(), Zp(dgrad));
+    if (dhz && what >= 2) std::copy(out.dhz.begin(), out.dhz.end(), Zp(dhz));
+    return 0;
+}
+
 int mg_evaluate_columns(const mg_target* t, const mg_program* p, const struct matchgate gates[],
                         const struct matchgate Z[], int what, int precision, long b0, long b1, numeric* f,
                         struct matchgate dgrad[], struct matchgate dhz[])

@@ -56,9 +56,17 @@ void DevBuf::release()
     n = 0;
 }
 
+// ---- static device scratch of the allocation-free evaluation ----------------------------
+// mg_evaluate_noalloc draws every device buffer of one single-request sweep (descriptors,
+// workgroup partial sums, the reduced result) from this array instead of allocating.  Each
+// device holds its own copy for the life of the process; host code reaches it only through
+// the address hipGetSymbolAddress returns with that device current (Context::noalloc).
+static __device__ __align__(256) unsigned char scratch[kNoallocScratchBytes];
+
 // ---- context (one per process, for the device current at first use) ------------------
 struct Context {
     int device = -1;
+    unsigned char* noalloc = nullptr;  // device address of `scratch` on this device
     hipStream_t s = nullptr;
     int ncu = 0;
     size_t lds_limit = 0;
@@ -134,6 +142,13 @@ static Context* ctx()
         n->lds_limit = prop.sharedMemPerBlockOptin ? prop.sharedMemPerBlockOptin : prop.sharedMemPerBlock;
         if (hipStreamCreateWithFlags(&n->s, hipStreamNonBlocking) != hipSuccess) return nullptr;
         if (hipEventCreate(&n->ev0) != hipSuccess || hipEventCreate(&n->ev1) != hipSuccess) return nullptr;
+        void* sp = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) {
+            set_error(std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+            return nullptr;
+        }
+        n->noalloc = (unsigned char*)sp;
         c = std::move(n);
     }
     return c.get();
@@ -473,9 +488,38 @@ static bool make_windowed_tiles(const std::vector<Inst>& inst, const std::vector
 }
 
 // ---- evaluation -------------------------------------------------------------------------------
+// Device buffers of one evaluation: the context's grow-only buffers, or (allocation-free
+// evaluation) consecutive 256-byte aligned pieces of the static scratch array.
+struct SweepMem {
+    unsigned char* base = nullptr;  // null: the context's buffers
+    size_t cap = 0, used = 0;
+    int get(DevBuf& b, size_t bytes, void** p)
+    {
+        if (!base) {
+            const hipError_t e = b.ensure(bytes);
+            if (e != hipSuccess) {
+                set_error(std::string("device allocation failed: ") + hipGetErrorString(e));
+                return -1;
+            }
+            *p = b.p;
+            return 0;
+        }
+        const size_t off = (used + 255) & ~(size_t)255;
+        if (off + bytes > cap) {
+            set_error("allocation-free evaluation needs more than the " + std::to_string(cap >> 20) +
+                      " MiB static scratch (the states of this sweep fit neither the LDS nor registers, the gates need "
+                      "checkpoints, or the partial sums are too large)");
+            return -1;
+        }
+        used = off + bytes;
+        *p = base + off;
+        return 0;
+    }
+};
+
 template <typename R>
 static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, int mode, long b0, long b1, EvalOut* out,
-                     EvalStats* st)
+                     EvalStats* st, SweepMem& ar)
 {
     hipStream_t s = c->s;
     const int L = p.L;
@@ -498,10 +542,11 @@ static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, in
     // coefficients in precision R
     std::vector<cpx<R>> coefR(b.coef.size());
     for (size_t k = 0; k < coefR.size(); k++) coefR[k] = {(R)b.coef[k].re, (R)b.coef[k].im};
-    MG_HIP(c->inst.ensure(b.inst.size() * sizeof(Inst)));
-    MG_HIP(c->coef.ensure(coefR.size() * sizeof(cpx<R>)));
-    MG_HIP(hipMemcpyAsync(c->inst.p, b.inst.data(), b.inst.size() * sizeof(Inst), hipMemcpyHostToDevice, s));
-    MG_HIP(hipMemcpyAsync(c->coef.p, coefR.data(), coefR.size() * sizeof(cpx<R>), hipMemcpyHostToDevice, s));
+    void *d_inst = nullptr, *d_coef = nullptr;
+    if (ar.get(c->inst, b.inst.size() * sizeof(Inst), &d_inst) < 0 || ar.get(c->coef, coefR.size() * sizeof(cpx<R>), &d_coef) < 0)
+        return -1;
+    MG_HIP(hipMemcpyAsync(d_inst, b.inst.data(), b.inst.size() * sizeof(Inst), hipMemcpyHostToDevice, s));
+    MG_HIP(hipMemcpyAsync(d_coef, coefR.data(), coefR.size() * sizeof(cpx<R>), hipMemcpyHostToDevice, s));
 
     // checkpointing: needed when psi_g cannot be recovered by V^{-1} (non-unitary gates)
     bool ckpt = false;
@@ -598,18 +643,23 @@ static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, in
     grid = std::max<long>(grid, 1);
 
     const int stride = 2 + 32 * p.P;
-    MG_HIP(c->partial.ensure((size_t)grid * stride * sizeof(double)));
-    MG_HIP(c->out.ensure((size_t)stride * sizeof(double)));
-    if (use_tile) MG_HIP(c->scratch.ensure((size_t)grid * tile_state_bytes));
-    else if (!lds_states && !reg_T && !use_win) MG_HIP(c->scratch.ensure((size_t)grid * state_bytes));
-    if (ckpt) MG_HIP(c->ckpt.ensure((size_t)grid * ck_per));
+    void *d_partial = nullptr, *d_out = nullptr, *d_scratch = nullptr, *d_ckpt = nullptr;
+    if (ar.get(c->partial, (size_t)grid * stride * sizeof(double), &d_partial) < 0 ||
+        ar.get(c->out, (size_t)stride * sizeof(double), &d_out) < 0)
+        return -1;
+    if (use_tile) {
+        if (ar.get(c->scratch, (size_t)grid * tile_state_bytes, &d_scratch) < 0) return -1;
+    } else if (!lds_states && !reg_T && !use_win) {
+        if (ar.get(c->scratch, (size_t)grid * state_bytes, &d_scratch) < 0) return -1;
+    }
+    if (ckpt && ar.get(c->ckpt, (size_t)grid * ck_per, &d_ckpt) < 0) return -1;
 
     SweepArgs<R> a{};
     a.L = L;
     a.G = p.G;
     a.P = p.P;
-    a.inst = (const Inst*)c->inst.p;
-    a.coef = (const cpx<R>*)c->coef.p;
+    a.inst = (const Inst*)d_inst;
+    a.coef = (const cpx<R>*)d_coef;
     a.target_kind = t.kind;
     if (t.kind == 0) {
         a.seed = std::is_same<R, double>::value ? (const cpx<R>*)t.seed64.p : (const cpx<R>*)t.seed32.p;
@@ -620,9 +670,9 @@ static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, in
     }
     a.b0 = b0;
     a.b1 = b1;
-    a.partial = (double*)c->partial.p;
-    a.scratch = (cpx<R>*)c->scratch.p;
-    a.ckpt = (cpx<R>*)c->ckpt.p;
+    a.partial = (double*)d_partial;
+    a.scratch = (cpx<R>*)d_scratch;
+    a.ckpt = (cpx<R>*)d_ckpt;
     a.nstate = reg_T ? reg_nst : nstate;
     a.lds_state_offset = (int)(reg_T ? head_reg : head);
 
@@ -631,26 +681,27 @@ static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, in
         const Windowed& w = w_win;
         std::vector<cpx<R>> wc(w.coef.size());
         for (size_t k = 0; k < wc.size(); k++) wc[k] = {(R)w.coef[k].re, (R)w.coef[k].im};
-        MG_HIP(c->w_inst.ensure(w.inst.size() * sizeof(Inst)));
-        MG_HIP(c->w_coef.ensure(wc.size() * sizeof(cpx<R>)));
-        MG_HIP(c->w_blk.ensure(2 * w.blk_lds.size() * sizeof(WBlock)));
-        MG_HIP(hipMemcpyAsync(c->w_inst.p, w.inst.data(), w.inst.size() * sizeof(Inst), hipMemcpyHostToDevice, s));
-        MG_HIP(hipMemcpyAsync(c->w_coef.p, wc.data(), wc.size() * sizeof(cpx<R>), hipMemcpyHostToDevice, s));
-        MG_HIP(hipMemcpyAsync(c->w_blk.p, w.blk_lds.data(), w.blk_lds.size() * sizeof(WBlock), hipMemcpyHostToDevice, s));
-        MG_HIP(hipMemcpyAsync((WBlock*)c->w_blk.p + w.blk_lds.size(), w.blk_glob.data(), w.blk_glob.size() * sizeof(WBlock),
+        void *d_wi = nullptr, *d_wc = nullptr, *d_wb = nullptr;
+        if (ar.get(c->w_inst, w.inst.size() * sizeof(Inst), &d_wi) < 0 || ar.get(c->w_coef, wc.size() * sizeof(cpx<R>), &d_wc) < 0 ||
+            ar.get(c->w_blk, 2 * w.blk_lds.size() * sizeof(WBlock), &d_wb) < 0)
+            return -1;
+        MG_HIP(hipMemcpyAsync(d_wi, w.inst.data(), w.inst.size() * sizeof(Inst), hipMemcpyHostToDevice, s));
+        MG_HIP(hipMemcpyAsync(d_wc, wc.data(), wc.size() * sizeof(cpx<R>), hipMemcpyHostToDevice, s));
+        MG_HIP(hipMemcpyAsync(d_wb, w.blk_lds.data(), w.blk_lds.size() * sizeof(WBlock), hipMemcpyHostToDevice, s));
+        MG_HIP(hipMemcpyAsync((WBlock*)d_wb + w.blk_lds.size(), w.blk_glob.data(), w.blk_glob.size() * sizeof(WBlock),
                               hipMemcpyHostToDevice, s));
         wa.L = L;
         wa.G = p.G;
         wa.P = p.P;
         wa.nblocks = (int)w.blk_lds.size();
-        wa.inst = (const Inst*)c->w_inst.p;
-        wa.coef = c->w_coef.p;
-        wa.blk_lds = (const WBlock*)c->w_blk.p;
-        wa.blk_glob = (const WBlock*)c->w_blk.p + w.blk_lds.size();
+        wa.inst = (const Inst*)d_wi;
+        wa.coef = d_wc;
+        wa.blk_lds = (const WBlock*)d_wb;
+        wa.blk_glob = (const WBlock*)d_wb + w.blk_lds.size();
         wa.seed = fp32 ? t.seed32.p : t.seed64.p;
         wa.b0 = b0;
         wa.b1 = b1;
-        wa.partial = (double*)c->partial.p;
+        wa.partial = (double*)d_partial;
         wa.acc_off = (int)win_acc;
         wa.buf_off = (int)win_buf;
         wa.rs = win_rs;
@@ -661,12 +712,14 @@ static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, in
     WinTileArgs wta{};
     if (use_wtile) {
         const WindowedTiles& w = w_wt;
-        MG_HIP(c->w_inst.ensure(w.inst.size() * sizeof(Inst)));
-        MG_HIP(c->w_coef.ensure(w.coef.size() * sizeof(cpx<double>)));
-        MG_HIP(c->w_blk.ensure(2 * w.blk_lds.size() * sizeof(WBlock) + w.tblocks.size() * sizeof(WTBlock)));
-        unsigned char* blk = (unsigned char*)c->w_blk.p;
-        MG_HIP(hipMemcpyAsync(c->w_inst.p, w.inst.data(), w.inst.size() * sizeof(Inst), hipMemcpyHostToDevice, s));
-        MG_HIP(hipMemcpyAsync(c->w_coef.p, w.coef.data(), w.coef.size() * sizeof(cpx<double>), hipMemcpyHostToDevice, s));
+        void *d_wi = nullptr, *d_wc = nullptr, *d_wb = nullptr;
+        if (ar.get(c->w_inst, w.inst.size() * sizeof(Inst), &d_wi) < 0 ||
+            ar.get(c->w_coef, w.coef.size() * sizeof(cpx<double>), &d_wc) < 0 ||
+            ar.get(c->w_blk, 2 * w.blk_lds.size() * sizeof(WBlock) + w.tblocks.size() * sizeof(WTBlock), &d_wb) < 0)
+            return -1;
+        unsigned char* blk = (unsigned char*)d_wb;
+        MG_HIP(hipMemcpyAsync(d_wi, w.inst.data(), w.inst.size() * sizeof(Inst), hipMemcpyHostToDevice, s));
+        MG_HIP(hipMemcpyAsync(d_wc, w.coef.data(), w.coef.size() * sizeof(cpx<double>), hipMemcpyHostToDevice, s));
         MG_HIP(hipMemcpyAsync(blk, w.blk_lds.data(), w.blk_lds.size() * sizeof(WBlock), hipMemcpyHostToDevice, s));
         MG_HIP(hipMemcpyAsync(blk + w.blk_lds.size() * sizeof(WBlock), w.blk_glob.data(), w.blk_glob.size() * sizeof(WBlock),
                               hipMemcpyHostToDevice, s));
@@ -676,16 +729,16 @@ static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, in
         wta.G = p.G;
         wta.P = p.P;
         wta.ntb = (int)w.tblocks.size();
-        wta.inst = (const Inst*)c->w_inst.p;
-        wta.coef = c->w_coef.p;
+        wta.inst = (const Inst*)d_wi;
+        wta.coef = d_wc;
         wta.tblocks = (const WTBlock*)(blk + 2 * w.blk_lds.size() * sizeof(WBlock));
         wta.blk_lds = (const WBlock*)blk;
         wta.blk_glob = (const WBlock*)(blk + w.blk_lds.size() * sizeof(WBlock));
         wta.seed = t.seed64.p;
         wta.b0 = b0;
         wta.b1 = b1;
-        wta.partial = (double*)c->partial.p;
-        wta.scratch = c->scratch.p;
+        wta.partial = (double*)d_partial;
+        wta.scratch = d_scratch;
         wta.nstate = nstate_tile(mode);
         wta.acc_off = (int)wt_acc;
         wta.buf_off = (int)wt_buf;
@@ -722,25 +775,26 @@ static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, in
                     tc[(size_t)k * kCoefPerInst + e] = {(R)v.re, (R)v.im};
                 }
             }
-        MG_HIP(c->w_inst.ensure(ti.size() * sizeof(Inst)));
-        MG_HIP(c->w_coef.ensure(tc.size() * sizeof(cpx<R>)));
-        MG_HIP(c->w_blk.ensure(blocks.size() * sizeof(TBlock)));
-        MG_HIP(hipMemcpyAsync(c->w_inst.p, ti.data(), ti.size() * sizeof(Inst), hipMemcpyHostToDevice, s));
-        MG_HIP(hipMemcpyAsync(c->w_coef.p, tc.data(), tc.size() * sizeof(cpx<R>), hipMemcpyHostToDevice, s));
-        MG_HIP(hipMemcpyAsync(c->w_blk.p, blocks.data(), blocks.size() * sizeof(TBlock), hipMemcpyHostToDevice, s));
+        void *d_wi = nullptr, *d_wc = nullptr, *d_wb = nullptr;
+        if (ar.get(c->w_inst, ti.size() * sizeof(Inst), &d_wi) < 0 || ar.get(c->w_coef, tc.size() * sizeof(cpx<R>), &d_wc) < 0 ||
+            ar.get(c->w_blk, blocks.size() * sizeof(TBlock), &d_wb) < 0)
+            return -1;
+        MG_HIP(hipMemcpyAsync(d_wi, ti.data(), ti.size() * sizeof(Inst), hipMemcpyHostToDevice, s));
+        MG_HIP(hipMemcpyAsync(d_wc, tc.data(), tc.size() * sizeof(cpx<R>), hipMemcpyHostToDevice, s));
+        MG_HIP(hipMemcpyAsync(d_wb, blocks.data(), blocks.size() * sizeof(TBlock), hipMemcpyHostToDevice, s));
         ta.L = L;
         ta.G = p.G;
         ta.P = p.P;
         ta.kw = tile_kw;
         ta.nblocks = (int)blocks.size();
-        ta.inst = (const Inst*)c->w_inst.p;
-        ta.coef = (const cpx<R>*)c->w_coef.p;
-        ta.blocks = (const TBlock*)c->w_blk.p;
+        ta.inst = (const Inst*)d_wi;
+        ta.coef = (const cpx<R>*)d_wc;
+        ta.blocks = (const TBlock*)d_wb;
         ta.seed = fp32 ? (const cpx<R>*)t.seed32.p : (const cpx<R>*)t.seed64.p;
         ta.b0 = b0;
         ta.b1 = b1;
-        ta.partial = (double*)c->partial.p;
-        ta.scratch = (cpx<R>*)c->scratch.p;
+        ta.partial = (double*)d_partial;
+        ta.scratch = (cpx<R>*)d_scratch;
         ta.nstate = nstate_tile(mode);
         ta.tile_off = (int)tile_off;
         // the copies above read pageable host vectors: complete them before they go away
@@ -758,9 +812,9 @@ static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, in
     else
         MG_HIP(launch_sweep<R>(a, mode, lds_states, ckpt, lds_T, (int)grid, lds, s));
     if (kn.time_kernels) MG_HIP(hipEventRecord(c->ev1, s));
-    MG_HIP(launch_reduce((const double*)c->partial.p, (int)grid, stride, (double*)c->out.p, s));
+    MG_HIP(launch_reduce((const double*)d_partial, (int)grid, stride, (double*)d_out, s));
     c->out_h.resize(stride);
-    MG_HIP(hipMemcpyAsync(c->out_h.data(), c->out.p, stride * sizeof(double), hipMemcpyDeviceToHost, s));
+    MG_HIP(hipMemcpyAsync(c->out_h.data(), d_out, stride * sizeof(double), hipMemcpyDeviceToHost, s));
     MG_HIP(hipStreamSynchronize(s));
 
     const double* o = c->out_h.data();
@@ -791,10 +845,15 @@ static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, in
 }
 
 int evaluate(const Target& tc, const Program& p, const zd* gates, const zd* Z, int mode, int precision, EvalOut* out,
-             long b0, long b1, EvalStats* stats)
+             long b0, long b1, EvalStats* stats, bool noalloc)
 {
     Context* c = ctx();
     if (!c) { set_error("no HIP device available"); return -1; }
+    SweepMem ar;
+    if (noalloc) {
+        ar.base = c->noalloc;
+        ar.cap = kNoallocScratchBytes;
+    }
     Target& t = const_cast<Target&>(tc);
     std::string why;
     if (!p.validate(&why)) { set_error("program: " + why); return -1; }
@@ -811,10 +870,14 @@ int evaluate(const Target& tc, const Program& p, const zd* gates, const zd* Z, i
         return -1;
     }
     if (precision == kFP32) {
+        if (noalloc && ((t.kind == 0 && !t.seed32_ready) || (t.kind == 1 && !t.tcoef32_ready))) {
+            set_error("allocation-free FP32 evaluation: the target has no FP32 copy yet (one FP32 mg_evaluate makes it)");
+            return -1;
+        }
         if (ensure_fp32_target(t, c->s) < 0) return -1;
-        return run_sweep<float>(c, t, p, b, mode, b0, b1, out, stats);
+        return run_sweep<float>(c, t, p, b, mode, b0, b1, out, stats, ar);
     }
-    return run_sweep<double>(c, t, p, b, mode, b0, b1, out, stats);
+    return run_sweep<double>(c, t, p, b, mode, b0, b1, out, stats, ar);
 }
 
 }  // namespace mg

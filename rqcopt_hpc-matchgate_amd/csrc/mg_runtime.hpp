@@ -86,9 +86,14 @@ struct EvalStats {
     const char* kernel = "";  // name of the sweep kernel
 };
 
+// bytes of the static device scratch that allocation-free evaluations draw from
+constexpr size_t kNoallocScratchBytes = (size_t)32 << 20;
+
 // gates, Z: P*8 complex each (Z may be null when mode < 2).  Column range [b0, b1).
+// noalloc: every device buffer of the sweep comes from the static scratch (no hipMalloc);
+// fails with an error when it does not fit there
 int evaluate(const Target& t, const Program& p, const zd* gates, const zd* Z, int mode, int precision,
-             EvalOut* out, long b0 = 0, long b1 = -1, EvalStats* stats = nullptr);
+             EvalOut* out, long b0 = 0, long b1 = -1, EvalStats* stats = nullptr, bool noalloc = false);
 
 // configuration knobs (mostly for tests and tuning)
 struct Knobs {

@@ -109,6 +109,7 @@ _sig("mg_program_nparams", [_P])
 _sig("mg_program_ninst", [_P])
 _sig("mg_evaluate", [_P, _P, _P, _P, _I, _I, _P, _P, _P])
 _sig("mg_evaluate_columns", [_P, _P, _P, _P, _I, _I, _L, _L, _P, _P, _P])
+_sig("mg_evaluate_noalloc", [_P, _P, _P, _P, _I, _I, _P, _P, _P])
 _sig("mg_riemannian_gradient_hvp", [_P, _P, _P, _P, _I, _P, _P, _P])
 _sig("mg_riemannian_project", [_P, _I, _P, _P, _P, _P, _P])
 _sig("mg_hessian", [_P, _P, _P, _I, _P, _P, _P])
