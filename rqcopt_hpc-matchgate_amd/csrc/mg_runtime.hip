@@ -63,7 +63,9 @@ void DevBuf::release()
 // the address hipGetSymbolAddress returns with that device current (Context::noalloc).
 static __device__ __align__(256) unsigned char scratch[kNoallocScratchBytes];
 
-// ---- context (one per process, for the device current at first use) ------------------
+// ---- context (one per host thread, for the device current at its first use): a stream, the
+// grow-only buffers and the captured graphs of that thread's evaluations, so that two threads
+// (the server's lanes) can evaluate at the same time ------------------------------------------
 struct Context {
     int device = -1;
     unsigned char* noalloc = nullptr;  // device address of `scratch` on this device
@@ -132,7 +134,7 @@ struct Context {
 
 static Context* ctx()
 {
-    static std::unique_ptr<Context> c;
+    static thread_local std::unique_ptr<Context> c;
     if (!c) {
         auto n = std::make_unique<Context>();
         if (hipGetDevice(&n->device) != hipSuccess) return nullptr;
@@ -654,6 +656,16 @@ static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, in
     }
     if (ckpt && ar.get(c->ckpt, (size_t)grid * ck_per, &d_ckpt) < 0) return -1;
 
+    // the FP32 copy of dense rows: made for the kernels that read it (the tiled kernel rounds
+    // the FP64 rows on load, so large targets keep one copy)
+    if (fp32 && t.kind == 0 && !use_tile && !t.seed32_ready) {
+        if (ar.base) {
+            set_error("allocation-free FP32 evaluation: the target has no FP32 copy yet (one FP32 mg_evaluate makes it)");
+            return -1;
+        }
+        if (ensure_fp32_target(t, s) < 0) return -1;
+    }
+
     SweepArgs<R> a{};
     a.L = L;
     a.G = p.G;
@@ -790,7 +802,8 @@ static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, in
         ta.inst = (const Inst*)d_wi;
         ta.coef = (const cpx<R>*)d_wc;
         ta.blocks = (const TBlock*)d_wb;
-        ta.seed = fp32 ? (const cpx<R>*)t.seed32.p : (const cpx<R>*)t.seed64.p;
+        ta.seed = fp32 ? nullptr : (const cpx<R>*)t.seed64.p;
+        ta.seed64 = fp32 ? (const cpx<double>*)t.seed64.p : nullptr;
         ta.b0 = b0;
         ta.b1 = b1;
         ta.partial = (double*)d_partial;
@@ -870,11 +883,14 @@ int evaluate(const Target& tc, const Program& p, const zd* gates, const zd* Z, i
         return -1;
     }
     if (precision == kFP32) {
-        if (noalloc && ((t.kind == 0 && !t.seed32_ready) || (t.kind == 1 && !t.tcoef32_ready))) {
-            set_error("allocation-free FP32 evaluation: the target has no FP32 copy yet (one FP32 mg_evaluate makes it)");
-            return -1;
+        // FP32 coefficients of a target program (dense rows: see run_sweep)
+        if (t.kind == 1 && !t.tcoef32_ready) {
+            if (noalloc) {
+                set_error("allocation-free FP32 evaluation: the target has no FP32 copy yet (one FP32 mg_evaluate makes it)");
+                return -1;
+            }
+            if (ensure_fp32_target(t, c->s) < 0) return -1;
         }
-        if (ensure_fp32_target(t, c->s) < 0) return -1;
         return run_sweep<float>(c, t, p, b, mode, b0, b1, out, stats, ar);
     }
     return run_sweep<double>(c, t, p, b, mode, b0, b1, out, stats, ar);
@@ -891,9 +907,15 @@ BatchTuning& batch_tuning()
     return t;
 }
 
+bool batch_single_request(const Target& t, const Program& p)
+{
+    const BatchTuning& bt = batch_tuning();
+    return t.kind == 0 && p.L >= bt.single_min && p.L <= bt.single_max;
+}
+
 BatchStats& last_batch_stats()
 {
-    static BatchStats s;
+    static thread_local BatchStats s;
     return s;
 }
 
@@ -1323,8 +1345,7 @@ int evaluate_batch(BatchReq* reqs, int n, BatchStats* st)
             set_error("batched evaluation: invalid column range");
             return -1;
         }
-        const BatchTuning& bt = batch_tuning();
-        if (r.t->kind == 0 && r.p->L >= bt.single_min && r.p->L <= bt.single_max) {
+        if (batch_single_request(*r.t, *r.p)) {
             single.push_back(i);
             continue;
         }

@@ -158,6 +158,9 @@ void batch_shape(int L, int precision, int* cgw, int* cpi);
 bool batch_variant_exists(int precision_fp32, int threads, int wbits);
 
 int evaluate_batch(BatchReq* reqs, int n, BatchStats* st);
+// true if evaluate_batch runs a request of this target and program on the single-request
+// kernels (dense target, batch_tuning().single_min <= L <= single_max): it fills the GPU alone
+bool batch_single_request(const Target& t, const Program& p);
 BatchStats& last_batch_stats();
 
 }  // namespace mg

@@ -11,6 +11,10 @@
 // needs is posted to the server, which launches one batched evaluation as soon as
 // every admitted job has posted one (continuous batching: jobs that arrive while a
 // launch is on the GPU are admitted before the next one, finished jobs leave at once).
+// Evaluations that fill the GPU on their own (dense targets of 12 or more qubits, which run
+// on the single-request kernels) do not wait for these rounds: a second lane, a worker thread
+// with its own stream and buffers, runs them as they are posted, so the large jobs' sweeps
+// and the small jobs' batched rounds overlap on the device and neither waits for the other.
 // Targets are created on the device at admission and freed at completion; admission
 // is limited by a job count and by the device memory the admitted jobs hold.
 // Without batching the jobs run one after another, each evaluation its own launch
@@ -75,13 +79,16 @@ struct JobRt {
     int kept_next = 0;
     std::thread th;
     bool finished = false;
+    bool big = false;  // its evaluations fill the GPU alone: posted to the second lane
 };
 
 struct Server {
     std::mutex mu;
-    std::condition_variable cv_server, cv_jobs;
-    std::vector<Posted*> posted;
-    int running = 0;
+    std::condition_variable cv_server, cv_jobs, cv_big;
+    std::vector<Posted*> posted;      // batched rounds
+    std::vector<Posted*> big_posted;  // the second lane
+    bool stop_big = false;
+    int running = 0, running_small = 0;
     bool batched = true;
     long rounds = 0, requests = 0;
     double kernel_ms = 0.0, eval_s = 0.0, prep_ms = 0.0;
@@ -117,8 +124,13 @@ int job_eval(JobRt* j, const zd* gates, const zd* Z, int mode, mg::EvalOut* out)
     p.req.precision = job_precision(j);
     p.req.out = &p.out;
     std::unique_lock<std::mutex> lk(j->srv->mu);
-    j->srv->posted.push_back(&p);
-    j->srv->cv_server.notify_one();
+    if (j->big) {
+        j->srv->big_posted.push_back(&p);
+        j->srv->cv_big.notify_one();
+    } else {
+        j->srv->posted.push_back(&p);
+        j->srv->cv_server.notify_one();
+    }
     j->srv->cv_jobs.wait(lk, [&] { return p.done; });
     if (p.status == 0) *out = std::move(p.out);
     return p.status;
@@ -263,7 +275,9 @@ size_t job_footprint(const mg_job& s, int densify_qubits)
     const size_t S = (size_t)1 << L;
     const mg::Program& p = s.program->p;
     size_t b = 0;
-    if (s.U || L <= densify_qubits) b += S * S * (sizeof(double) * 2 + sizeof(float) * 2);
+    // dense rows: FP64, plus an FP32 copy up to 12 qubits (the tiled FP32 kernel of larger
+    // targets rounds the FP64 rows on load)
+    if (s.U || L <= densify_qubits) b += S * S * (sizeof(double) * 2 + (L <= 12 ? sizeof(float) * 2 : 0));
     else b += (size_t)s.target_ngates * (mg::kCoefPerInst * 24 + sizeof(mg::Inst));
     b += (size_t)p.G * (mg::kCoefPerInst * 16 + sizeof(mg::Inst));
     int cgw, cpi;
@@ -536,6 +550,36 @@ int mg_serve(mg_job* jobs, int njobs, const mg_serve_options* opt, mg_serve_stat
             report(srv.rounds, srv.requests);
         }
     } else {
+        // the second lane: evaluations that fill the GPU alone, run as they are posted on this
+        // worker's own stream and buffers (they overlap the batched rounds on the device)
+        std::thread big_worker([&srv] {
+            std::unique_lock<std::mutex> g(srv.mu);
+            for (;;) {
+                srv.cv_big.wait(g, [&] { return srv.stop_big || !srv.big_posted.empty(); });
+                if (srv.big_posted.empty()) break;
+                std::vector<Posted*> batch;
+                batch.swap(srv.big_posted);
+                g.unlock();
+                std::vector<mg::BatchReq> reqs(batch.size());
+                for (size_t k = 0; k < batch.size(); k++) reqs[k] = batch[k]->req;
+                mg::BatchStats st;
+                const auto tb = clock::now();
+                const int r = mg::evaluate_batch(reqs.data(), (int)reqs.size(), &st);
+                const double dt = std::chrono::duration<double>(clock::now() - tb).count();
+                g.lock();
+                srv.rounds++;
+                srv.requests += (long)batch.size();
+                srv.kernel_ms += st.kernel_ms;
+                srv.eval_s += dt;
+                srv.prep_ms += st.prep_ms;
+                srv.graph_replays += st.graph_replays;
+                for (Posted* p : batch) {
+                    p->status = r;
+                    p->done = true;
+                }
+                srv.cv_jobs.notify_all();
+            }
+        });
         std::unique_lock<std::mutex> lk(srv.mu);
         while (done < njobs) {
             // admit arrived jobs while slots and device memory allow
@@ -557,14 +601,17 @@ int mg_serve(mg_job* jobs, int njobs, const mg_serve_options* opt, mg_serve_stat
                 }
                 held += rt[i]->footprint;
                 peak_held = std::max(peak_held, held);
-                srv.running++;
-                peak_active = std::max(peak_active, srv.running);
                 JobRt* j = rt[i].get();
+                j->big = mg::batch_single_request(*j->target, jobs[i].program->p);
+                srv.running++;
+                if (!j->big) srv.running_small++;
+                peak_active = std::max(peak_active, srv.running);
                 j->th = std::thread([j, &srv] {
                     run_job(j);
                     std::lock_guard<std::mutex> g(srv.mu);
                     j->finished = true;
                     srv.running--;
+                    if (!j->big) srv.running_small--;
                     srv.cv_server.notify_one();
                 });
             }
@@ -581,8 +628,8 @@ int mg_serve(mg_job* jobs, int njobs, const mg_serve_options* opt, mg_serve_stat
                 }
             }
             if (done >= njobs) break;
-            // launch when every admitted job waits on an evaluation
-            if (srv.running > 0 && (int)srv.posted.size() == srv.running) {
+            // launch when every admitted job of this lane waits on an evaluation
+            if (srv.running_small > 0 && (int)srv.posted.size() == srv.running_small) {
                 std::vector<Posted*> batch;
                 batch.swap(srv.posted);
                 lk.unlock();
@@ -619,6 +666,10 @@ int mg_serve(mg_job* jobs, int njobs, const mg_serve_options* opt, mg_serve_stat
                 srv.cv_server.wait(lk);
             }
         }
+        srv.stop_big = true;
+        srv.cv_big.notify_all();
+        lk.unlock();
+        big_worker.join();
     }
     if (stats) {
         stats->wall_s = now_s();

@@ -97,7 +97,18 @@ __global__ __launch_bounds__(1 << tile_lt<R>()) void sweep_tile_kernel(TileArgs<
 
     const int nb = a.nblocks;
     for (long b = a.b0 + blockIdx.x; b < a.b1; b += gridDim.x) {
-        const cpx<R>* row = a.seed + (size_t)b * S;
+        const size_t rb = (size_t)b * S;
+        // seed row entry (FP32 sweeps may read the FP64 rows and round on load: the same values
+        // as an FP32 copy of the rows)
+        auto row = [&](uint32_t i) -> cpx<R> {
+            if constexpr (sizeof(R) == 4) {
+                if (a.seed64) {
+                    const cpx<double> v = a.seed64[rb + i];
+                    return {(float)v.re, (float)v.im};
+                }
+            }
+            return a.seed[rb + i];
+        };
         double fr = 0.0, fi = 0.0;
         // ---- forward: psi, dpsi ----------------------------------------------------------
         for (int k = 0; k < nb; k++) {
@@ -140,7 +151,7 @@ __global__ __launch_bounds__(1 << tile_lt<R>()) void sweep_tile_kernel(TileArgs<
                     const uint32_t j = tid + m * T, gi = ot | mp.win_tid | mp.wm[m];
                     const cpx<R> p = sel<R, true>(t_psi, j);
                     if (k == nb - 1) {
-                        const cpx<R> tt = cmul(row[gi], p);
+                        const cpx<R> tt = cmul(row(gi), p);
                         fr += (double)tt.re;
                         fi += (double)tt.im;
                         if constexpr (MODE == 0) continue;
@@ -182,7 +193,7 @@ __global__ __launch_bounds__(1 << tile_lt<R>()) void sweep_tile_kernel(TileArgs<
                     for (int m = 0; m < PER; m++) {
                         const uint32_t j = tid + m * T, gi = ot | mp.win_tid | mp.wm[m];
                         sel<R, true>(t_psi, j) = g_psi[gi];
-                        sel<R, true>(t_lam, j) = k == nb - 1 ? row[gi] : g_lam[gi];
+                        sel<R, true>(t_lam, j) = k == nb - 1 ? row(gi) : g_lam[gi];
                         if constexpr (MODE == 2) {
                             sel<R, true>(t_dpsi, j) = g_dpsi[gi];
                             sel<R, true>(t_dlam, j) = k == nb - 1 ? cpx<R>{R(0), R(0)} : g_dlam[gi];

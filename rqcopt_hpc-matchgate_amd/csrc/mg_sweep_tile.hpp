@@ -25,6 +25,8 @@ struct TileArgs {
     const cpx<R>* coef;     // [G][kCoefPerInst] in block order: V, V^{-1}, Z
     const TBlock* blocks;   // [nblocks]
     const cpx<R>* seed;     // [S][S]: row b = -conj(U e_b)
+    const cpx<double>* seed64;  // FP32 sweeps without an FP32 copy of the rows: the FP64 rows,
+                                // rounded to FP32 on load (seed unused); null otherwise
     long b0, b1;            // column range
     double* partial;        // [gridDim.x][2 + 32 P]
     cpx<R>* scratch;        // [gridDim.x][nstate][S]: psi, dpsi, lam, dlam (as the mode needs)

@@ -187,7 +187,7 @@ def _jobs(mg, seed, spec):
         j = brickwall_job(rng, L, d, it)
         out.append({"program": mg.Program.create(L, j["G"], j["param"], j["wires"]), "gates": j["gates"],
                     "niter": it, "target_gates": j["target_gates"], "target_wires": j["target_wires"],
-                    "arrival": 0.002 * k})
+                    "arrival": 0.002 * k, "program_wires": j["wires"]})
     return out
 
 
@@ -250,6 +250,50 @@ def test_serve_tr_reuse_matches_reference_control_flow(mg):
     alone, _ = mg.serve([jobs[1]], batched=True, precision=mg.MG_FP32, precision_switch=3, realtime=False,
                         tr_reuse=True)
     assert np.array_equal(alone[0]["f_iter"], mix[1]["f_iter"]) and np.array_equal(alone[0]["gates"], mix[1]["gates"])
+
+
+def test_serve_mixed_sizes_with_large_jobs(mg):
+    """A served stream with L=12 and L=14 jobs next to small ones, targets materialised at
+    admission, FP32 sweeps with the last 3 iterations FP64, tr_reuse (the served stream's
+    settings).  The L=12 and L=14 jobs fill the GPU alone and run on the single-request kernels in
+    the server's second lane.  Every job's trajectory is the same bits served alone; at each
+    job's final gates the served path's FP64 f, gradient and HVP (dense rows, the kernels the
+    server ran) match the oracle on a column subset (1e-11), and the last f of the trajectory
+    matches the oracle's full trace where that is cheap (L <= 12)."""
+    spec = [(12, 3, 4), (8, 5, 4), (14, 3, 3), (10, 3, 3), (14, 5, 2), (6, 5, 5)]
+    jobs = _jobs(mg, 24, spec)
+    kw = dict(batched=True, precision=mg.MG_FP32, precision_switch=3, densify_qubits=14, tr_reuse=True)
+    say("case: serve 6 jobs, L in {6,8,10,12,14,14}")
+    res, st = mg.serve(jobs, realtime=True, **kw)
+    assert all(r["status"] == 0 for r in res)
+    for k, j in enumerate(jobs):
+        say(f"case: serve job {k} (L={j['program'].L}) alone")
+        alone, _ = mg.serve([dict(j, arrival=0.0)], realtime=False, **kw)
+        assert np.array_equal(alone[0]["f_iter"], res[k]["f_iter"]), k
+        assert np.array_equal(alone[0]["gates"], res[k]["gates"]), k
+    rng = np.random.default_rng(25)
+    for k, (j, r) in enumerate(zip(jobs, res)):
+        L, prog = j["program"].L, j["program"]
+        say(f"case: job {k} (L={L}) final gates against the oracle")
+        V = r["gates"]
+        v = rng.standard_normal(8 * prog.P)
+        t = mg.Target.circuit(j["target_gates"], j["target_wires"], L).densify()
+        tp = np.arange(len(j["target_gates"]), dtype=np.int32)
+        param, wires = np.arange(prog.G, dtype=np.int32), j["program_wires"]
+        S = 1 << L
+        b0, b1 = (0, S) if L <= 8 else (S // 3, S // 3 + 16)
+        f, g, h = mg.evaluate_batch([{"target": t, "program": prog, "gates": V, "v": v, "what": mg.EVAL_F_GRAD_HVP,
+                                      "precision": mg.MG_FP64, "b0": b0, "b1": b1}])[0]
+        Z = orc.real_to_directions(V, v)
+        fo, dg, dh = orc.target_gradient_hvp_program(j["target_gates"], j["target_wires"], V, Z, param, wires, L,
+                                                     b0, b1, nthreads=8)
+        go, ho = orc.riemannian_gradient(V, dg), orc.riemannian_hvp(V, dg, Z, dh)
+        assert abs(f - fo) <= 1e-11 * abs(fo) and rel(g, go) < 1e-11 and rel(h, ho) < 1e-11, k
+        if L <= 12:
+            ff, _, _ = orc.target_gradient_hvp_program(j["target_gates"], j["target_wires"], V, 0 * Z, param, wires, L,
+                                                       0, S, nthreads=8)
+            assert abs(r["f_iter"][-1] - ff.real) <= 1e-11 * abs(ff), k
+        del t
 
 
 def test_serve_admission_limits(mg):
