@@ -9,8 +9,13 @@ Hessian-vector product over all 4096 basis columns (one trust-region iteration
 unit).  The default precision is FP64 sweeps; --precision fp32 runs FP32 sweeps
 with FP64 accumulation.
 
-Also measured: the served stream of trust-region jobs (continuous batching, p50/p99
-latency, against the one-job-at-a-time FP64 path).
+Also measured: 50 trust-region iterations on that instance; the served stream of
+trust-region jobs at its stated qubit counts (L in {8,...,16}; continuous batching, p50/p99
+latency, against the one-job-at-a-time FP64 path, which is stopped at a time limit and
+reported as unfinished when it does not finish) and a served stream of L <= 12 jobs against
+three one-job baselines run to the end; the 64-instance batched workload (20 iterations);
+the L=16 instance; and the CPU baselines (the reference's own code on a bounded column sample,
+the oracle's HVP, the L=8 configuration).
 
 Prints one JSON line.  `python bench.py --gpus 1 --steps K --warmup W`.
 """
@@ -44,32 +49,132 @@ def build_workload(seed=0):
     return L, W, param, wires, G, V, v
 
 
-def cpu_baseline(W, V, v, param, wires, L, budget_s=12.0):
-    """The oracle's OpenMP f + gradient + HVP on a bounded sample of basis columns,
-    scaled to whole evaluations per second."""
+def host_info():
+    """The host's CPU as this process sees it: cores online, cores it may run on, the OpenMP
+    thread setting and the model name."""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    aff = len(os.sched_getaffinity(0))
+    return {"nproc": os.cpu_count(), "affinity": aff, "omp_num_threads": os.environ.get("OMP_NUM_THREADS"),
+            "model": model}
+
+
+def cpu_threads():
+    """Threads for the CPU baselines: OMP_NUM_THREADS where it is set (the CPU share a GPU box
+    gives one GPU), else every core this process may run on."""
+    aff = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS")
+    return max(1, min(aff, int(env))) if env and env.isdigit() else aff
+
+
+def _grow(run, ncols0, budget_s, limit, step):
+    """run(ncols) -> seconds; grow the column sample to about budget_s seconds."""
+    dt = run(ncols0)
+    n = int(min(limit, max(ncols0, ncols0 * budget_s / max(dt, 1e-6))))
+    n = max(step, (n // step) * step)
+    if n == ncols0:
+        return n, dt
+    return n, run(n)
+
+
+def cpu_reference_algorithm(W, L, nthreads, budget_s=10.0, seed=0):
+    """The reference's own code (oracle/_ref: its src/*.c compiled unchanged) on the L=12
+    Fermi-Hubbard target: the per-column work of its OpenMP target loop with the O(G^2)
+    placeholder Hessian (src/matchgate_target.c:746-864) on a sample of basis columns, scaled
+    to the 4096-column trace; the f-only loop likewise; and the truncated CG of the trust region
+    (the oracle's restatement of src/trust_region.c:88-158, the reference's needs CBLAS).  The
+    reference's brickwall ties one gate to each layer and applies it to all L/2 pairs, so the
+    closest form of the open 50-gate brickwall is 9 tied layers with periodic pairs (54 gate
+    instances, 72 parameters)."""
+    sys.path.insert(0, ROOT)
+    import oracle as orc
+    from oracle import ref
+    from rqcopt_mg.workloads import haar_matchgates
+
+    if not ref.available():
+        return None
+    S = 1 << L
+    nl = 9
+    pm = np.stack([np.arange(L) if k % 2 == 0 else np.roll(np.arange(L), -1) for k in range(nl)]).astype(np.int32)
+    V = haar_matchgates(np.random.default_rng(seed), nl)
+    nh, th = _grow(lambda n: ref.time_target_columns(W, V, pm, n, nthreads, True)[0], nthreads, budget_s, S, nthreads)
+    nf, tf = _grow(lambda n: ref.time_target_columns(W, V, pm, n, nthreads, False)[0], 4 * nthreads, 0.2 * budget_s, S,
+                   nthreads)
+    m = 8 * nl
+    rng = np.random.default_rng(seed)
+    A = rng.standard_normal((m, m))
+    Hm = A @ A.T / m + np.eye(m)
+    g = rng.standard_normal(m)
+    t0 = time.perf_counter()
+    orc.truncated_cg_hmat(g, Hm, 0.01, 2 * 16 * nl)
+    ttcg = time.perf_counter() - t0
+    t_eval = th * S / nh
+    t_iter = t_eval + tf * S / nf + ttcg
+    return {
+        "value": 1.0 / t_iter,
+        "unit": "trust-region iterations/s",
+        "cores": nthreads,
+        "kind": "reference",
+        "sample": f"{nh} of {S} basis columns of the f+gradient+Hessian loop and {nf} of the f loop, timed and scaled "
+                  "to the full trace, plus one truncated CG",
+        "eval_s": t_eval,
+        "workload": "L=12 Fermi-Hubbard exp(-iHt) dense target (the same matrix), 9 tied brickwall layers with periodic "
+                    "pairs (54 gate instances, 72 parameters); one iteration = f + gradient + full Riemannian Hessian "
+                    "(O(G^2) placeholder sweeps) + f at the trial point + truncated CG",
+        "gpu_counterpart": "rtr.iters_per_s",
+    }
+
+
+def cpu_hvp_port(W, V, v, param, wires, L, nthreads, budget_s=8.0):
+    """The oracle's OpenMP f + gradient + HVP (the GPU's unit) on a bounded sample of basis
+    columns, scaled to whole evaluations per second."""
     sys.path.insert(0, ROOT)
     import oracle as orc
 
-    nthreads = min(os.cpu_count() or 1, 16)
     S = 1 << L
     Z = orc.real_to_directions(V, v)
-    ncols = 16
+
+    def run(n):
+        t0 = time.perf_counter()
+        orc.target_gradient_hvp(W, V, Z, param, wires, L, 0, n, nthreads)
+        return time.perf_counter() - t0
+
+    n, dt = _grow(run, nthreads, budget_s, S, nthreads)
+    return {"value": (n / S) / dt, "unit": "f+grad+HVP evaluations/s", "cores": nthreads, "kind": "port",
+            "sample": f"{n} of {S} basis columns (FP64, OpenMP over columns), scaled to the full trace"}
+
+
+def cpu_small_config(nthreads):
+    """The L=8 CPU configuration: open brickwall of 5 layers (4/3/4/3/4 = 18 independent gates),
+    seeded Haar gates (seed 0), target a seeded 21-layer brickwall (74 gates) materialised as a
+    dense 256 x 256 matrix, 10 trust-region iterations with the explicit Riemannian Hessian (the
+    reference's algorithm), on the oracle (the reference's brickwall API ties gates to layers)."""
+    sys.path.insert(0, ROOT)
+    import oracle as orc
+    from rqcopt_mg.workloads import haar_matchgates, open_brickwall
+
+    L = 8
+    rng = np.random.default_rng(0)
+    param, wires, G = open_brickwall(L, 5)
+    V = haar_matchgates(rng, G)
+    tp, tw, Gt = open_brickwall(L, 21)
+    Ug = haar_matchgates(rng, Gt)
+    S = 1 << L
+    U = np.stack([orc.apply_program(Ug, tp, tw, L, np.eye(1, S, b).ravel()) for b in range(S)], axis=1)
     t0 = time.perf_counter()
-    orc.target_gradient_hvp(W, V, Z, param, wires, L, 0, ncols, nthreads)
+    f_iter, _, _ = orc.rtr_optimize(U, V, param, wires, L, 10, mode="hmat", nthreads=nthreads)
     dt = time.perf_counter() - t0
-    # grow the sample to about budget_s seconds of CPU work
-    ncols = int(min(S, max(ncols, ncols * budget_s / max(dt, 1e-6) / 2)))
-    ncols = max(nthreads, (ncols // nthreads) * nthreads)
-    t0 = time.perf_counter()
-    orc.target_gradient_hvp(W, V, Z, param, wires, L, 0, ncols, nthreads)
-    dt = time.perf_counter() - t0
-    return {
-        "value": (ncols / S) / dt,
-        "unit": "f+grad+HVP evaluations/s",
-        "cores": nthreads,
-        "kind": "port",
-        "sample": f"{ncols} of {S} basis columns (FP64, OpenMP over columns), scaled to the full trace",
-    }
+    return {"s_for_10_iterations": dt, "iters_per_s": 10 / dt, "cores": nthreads, "kind": "port",
+            "f_first": float(f_iter[0]), "f_last": float(f_iter[-1]),
+            "workload": "L=8, open brickwall 5 layers (18 gates), target a 21-layer brickwall (74 gates, dense), "
+                        "10 trust-region iterations, explicit Hessian"}
 
 
 def measured_traffic(kernel, dtype):
@@ -96,20 +201,48 @@ def copy_calibration(mib):
     print(json.dumps({"copy_bytes": n, "copies": 5}))
 
 
-def served_bench(mg, njobs, seed=1, Ls=(8, 10, 12), baselines=("dense", "plain", "reuse")):
-    """The served stream: Poisson arrivals (200 jobs/s, seed 1) of trust-region jobs with
-    L in Ls, odd depth 3..15, budget 5..50 iterations, circuit targets of 3x the depth; FP32
-    sweeps with the last 3 iterations in FP64, continuous batching, targets materialised at
-    admission, each job's trust region reusing its own products (tr_reuse); against the same
-    stream run one job at a time in FP64 with one launch per evaluation and the reference
-    trust-region control flow ("plain": circuit targets as given; "dense": targets materialised)
-    and, to separate the gains, one job at a time with tr_reuse ("reuse")."""
+def device_free_bytes():
+    """Free device memory (hipMemGetInfo)."""
+    import ctypes
+
+    hip = ctypes.CDLL("libamdhip64.so")
+    free, total = ctypes.c_size_t(), ctypes.c_size_t()
+    assert hip.hipMemGetInfo(ctypes.byref(free), ctypes.byref(total)) == 0
+    return free.value, total.value
+
+
+BASELINE_RUNS = {
+    "plain": ("baseline_fp64_one_job_plain", "speedup_vs_plain", dict(densify_qubits=0),
+              "one job at a time, FP64, one launch per evaluation, circuit targets applied per column, the "
+              "reference trust-region control flow"),
+    "dense": ("baseline_fp64_one_job_dense_targets", "speedup_vs_dense_targets", dict(densify_qubits=16),
+              "one job at a time, FP64, one launch per evaluation, targets materialised at admission, best "
+              "single-job kernels, the reference control flow"),
+    "reuse": ("baseline_fp64_one_job_dense_targets_tr_reuse", "speedup_vs_dense_targets_tr_reuse",
+              dict(densify_qubits=16, tr_reuse=True),
+              "one job at a time, FP64, one launch per evaluation, targets materialised, the served trust region "
+              "(H eta from the truncated CG, f+grad kept)"),
+}
+
+
+def served_bench(mg, njobs, seed=1, Ls=(8, 10, 12, 14, 16), baselines=("plain",), limit_s=60.0):
+    """The served stream: the first njobs of the Poisson stream (200 jobs/s, seed 1) of
+    trust-region jobs with L uniform in Ls, odd depth 3..15, budget 5..50 iterations, circuit
+    targets of 3x the depth; FP32 sweeps with the last 3 iterations in FP64, continuous batching
+    (evaluations that fill the GPU alone in the server's second lane), targets materialised at
+    admission, each job's trust region reusing its own products (tr_reuse), admission by device
+    memory.  Against the same stream run one job at a time in FP64 (see BASELINE_RUNS), each
+    baseline stopped after limit_s seconds: a baseline that does not finish is reported as such
+    (jobs done, seconds), with the lower bound that its unfinished run puts on the speedup."""
     from rqcopt_mg.workloads import served_stream
 
     stream = served_stream(njobs, seed=seed, rate=200.0, Ls=Ls)
     jobs = [{"program": mg.Program.create(j["L"], j["G"], j["param"], j["wires"]), "gates": j["gates"],
              "niter": j["niter"], "target_gates": j["target_gates"], "target_wires": j["target_wires"],
              "arrival": j["arrival"]} for j in stream]
+    mix = {}
+    for j in stream:
+        mix[j["L"]] = mix.get(j["L"], 0) + 1
     # autotune the batched kernel on one HVP per job of the first 16 that it runs (L < 12: the
     # dense larger ones go to the single-request kernels), at their initial gates
     rng = np.random.default_rng(seed)
@@ -122,61 +255,71 @@ def served_bench(mg, njobs, seed=1, Ls=(8, 10, 12), baselines=("dense", "plain",
     if calib:
         tune = {"fp32": mg.batch_autotune(calib, mg.MG_FP32), "fp64": mg.batch_autotune(calib, mg.MG_FP64)}
         log(f"autotuned (threads, window bits, workgroups/CU, ms): {tune}")
+    del calib
+    free, total = device_free_bytes()
+    budget = max(0.0, free - (24 << 30))
     mg.set_kernel_timing(True)
-    log(f"serving {njobs} jobs with continuous batching ...")
-    dq = max(12, max(Ls))
+    dq = max(Ls)
+    log(f"serving {njobs} jobs (L: {dict(sorted(mix.items()))}) with continuous batching ...")
     res, st = mg.serve(jobs, batched=True, precision=mg.MG_FP32, precision_switch=3, densify_qubits=dq,
-                       realtime=True, tr_reuse=True, progress_s=30.0)
+                       realtime=True, tr_reuse=True, progress_s=20.0, hbm_budget=budget)
+    assert all(r["status"] == 0 for r in res), [r["status"] for r in res]
     jps = njobs / st["wall_s"]
-    log(f"  {jps:.2f} jobs/s, {st['rounds']} launches for {st['requests']} evaluations")
+    log(f"  {jps:.3f} jobs/s ({st['wall_s']:.1f} s), {st['rounds']} launches for {st['requests']} evaluations")
     lat = np.array([r["latency"] for r in res]) * 1e3
-    out = {"jobs": njobs, "jobs_per_s": jps, "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+    out = {"jobs": njobs, "qubits": {str(k): v for k, v in sorted(mix.items())}, "jobs_per_s": jps,
+           "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+           "latency_ms_per_job": [round(float(x), 1) for x in lat],
            "launches": st["rounds"], "evaluations": st["requests"], "peak_admitted": st["max_active"],
+           "peak_device_bytes": st["peak_footprint"], "hbm_budget": budget,
            "kernel_s": st["kernel_ms"] * 1e-3, "wall_s": st["wall_s"], "graph_replays": st["graph_replays"],
            "hvp_sweeps": int(sum(r["nhvp"] for r in res))}
-    runs = {
-        "dense": ("baseline_fp64_one_job_dense_targets", "speedup_vs_dense_targets", dict(densify_qubits=dq),
-                  "targets materialised at admission, best single-job kernels, reference control flow"),
-        "plain": ("baseline_fp64_one_job_plain", "speedup_vs_plain", dict(densify_qubits=0),
-                  "circuit targets applied per column, one launch per evaluation, reference control flow"),
-        "reuse": ("baseline_fp64_one_job_dense_targets_tr_reuse", "speedup_vs_dense_targets_tr_reuse",
-                  dict(densify_qubits=dq, tr_reuse=True),
-                  "targets materialised, the served trust region (H eta from the truncated CG, f+grad kept)"),
-    }
-    base = None
     for name in baselines:
-        key, skey, kw, note = runs[name]
-        log(f"the same stream one job at a time (FP64, one launch per evaluation): {note} ...")
-        r1, s1 = mg.serve(jobs, batched=False, precision=mg.MG_FP64, realtime=True, progress_s=30.0, **kw)
-        l1 = np.array([r["latency"] for r in r1]) * 1e3
-        out[key] = {"jobs_per_s": njobs / s1["wall_s"], "p50_ms": float(np.percentile(l1, 50)),
-                    "p99_ms": float(np.percentile(l1, 99)), "evaluations": s1["requests"],
-                    "hvp_sweeps": int(sum(r["nhvp"] for r in r1)), "note": note}
-        out[skey] = jps / out[key]["jobs_per_s"]
-        if name == "dense":
-            base = r1
+        key, skey, kw, note = BASELINE_RUNS[name]
+        log(f"the same stream, {note} (stopped after {limit_s:.0f} s) ...")
+        r1, s1 = mg.serve(jobs, batched=False, precision=mg.MG_FP64, realtime=True, progress_s=20.0,
+                          time_limit_s=limit_s, **kw)
+        ok = [r for r in r1 if r["status"] == 0]
+        if len(ok) == njobs:
+            l1 = np.array([r["latency"] for r in r1]) * 1e3
+            out[key] = {"finished": True, "jobs_per_s": njobs / s1["wall_s"], "wall_s": s1["wall_s"],
+                        "p50_ms": float(np.percentile(l1, 50)), "p99_ms": float(np.percentile(l1, 99)),
+                        "evaluations": s1["requests"], "hvp_sweeps": int(sum(r["nhvp"] for r in r1)), "note": note}
+            out[skey] = jps / out[key]["jobs_per_s"]
+        else:
+            # it stopped after s1["wall_s"] seconds with jobs left: its wall time exceeds that, so
+            # its jobs/s is below njobs / wall_s and the speedup above jps * wall_s / njobs
+            out[key] = {"finished": False, "jobs_done": len(ok), "stopped_after_s": s1["wall_s"],
+                        "time_limit_s": limit_s, "note": note}
+            out[skey + "_lower_bound"] = jps * s1["wall_s"] / njobs
     # error of each job's final evaluation (FP64 sweeps in the last iterations) and of an FP32
-    # sweep at the same gates, against the FP64 single-request path (pinned to the oracle in tests)
+    # sweep at the same gates, against the FP64 single-request path (pinned to the oracle in
+    # tests) on a column subset of the larger jobs
     err = {"grad_final": 0.0, "hvp_final": 0.0, "grad_fp32_sweep": 0.0, "hvp_fp32_sweep": 0.0}
-    for k, (j, r) in enumerate(zip(jobs, res)):
-        t = mg.Target.circuit(j["target_gates"], j["target_wires"], j["program"].L)
+    for j, r in zip(jobs, res):
+        L = j["program"].L
+        S = 1 << L
+        b0, b1 = (0, S) if L <= 12 else (S // 2, S // 2 + 64)
+        t = mg.Target.circuit(j["target_gates"], j["target_wires"], L)
         v = rng.standard_normal(8 * j["program"].P)
-        f0, g0, h0 = mg.riemannian_gradient_hvp(t, j["program"], r["gates"], v, mg.MG_FP64)
-        req = {"target": t, "program": j["program"], "gates": r["gates"], "v": v, "what": mg.EVAL_F_GRAD_HVP}
+        req = {"target": t, "program": j["program"], "gates": r["gates"], "v": v, "what": mg.EVAL_F_GRAD_HVP,
+               "b0": b0, "b1": b1}
         o64, o32 = mg.evaluate_batch([dict(req, precision=mg.MG_FP64)])[0], mg.evaluate_batch([dict(req, precision=mg.MG_FP32)])[0]
-        gn, hn = max(np.max(np.abs(g0)), 1e-300), max(np.max(np.abs(h0)), 1e-300)
-        err["grad_final"] = max(err["grad_final"], float(np.max(np.abs(o64[1] - g0)) / gn))
-        err["hvp_final"] = max(err["hvp_final"], float(np.max(np.abs(o64[2] - h0)) / hn))
-        err["grad_fp32_sweep"] = max(err["grad_fp32_sweep"], float(np.max(np.abs(o32[1] - g0)) / gn))
-        err["hvp_fp32_sweep"] = max(err["hvp_fp32_sweep"], float(np.max(np.abs(o32[2] - h0)) / hn))
-        if base is not None:
-            b = base[k]
-            err["f_vs_fp64_path"] = max(err.get("f_vs_fp64_path", 0.0),
-                                        float(abs(r["f_iter"][-1] - b["f_iter"][-1]) / abs(b["f_iter"][-1])))
+        dq_t = mg.Target.circuit(j["target_gates"], j["target_wires"], L).densify()
+        d64 = mg.evaluate_batch([dict(req, target=dq_t, precision=mg.MG_FP64)])[0]
+        gn, hn = max(np.max(np.abs(o64[1])), 1e-300), max(np.max(np.abs(o64[2])), 1e-300)
+        err["grad_final"] = max(err["grad_final"], float(np.max(np.abs(d64[1] - o64[1])) / gn))
+        err["hvp_final"] = max(err["hvp_final"], float(np.max(np.abs(d64[2] - o64[2])) / hn))
+        err["grad_fp32_sweep"] = max(err["grad_fp32_sweep"], float(np.max(np.abs(o32[1] - o64[1])) / gn))
+        err["hvp_fp32_sweep"] = max(err["hvp_fp32_sweep"], float(np.max(np.abs(o32[2] - o64[2])) / hn))
+        del dq_t
     out.update({"rel_err_vs_fp64": err, "autotune": tune,
-                "workload": f"Poisson 200 jobs/s seed 1, L in {{{','.join(map(str, Ls))}}}, odd depth 3..15, 5..50 "
-                            "iterations, circuit targets of 3x depth; FP32 sweeps, last 3 iterations FP64; trust region "
-                            "with tr_reuse"})
+                "workload": f"the first {njobs} jobs of the Poisson stream (200 jobs/s, seed 1), L uniform in "
+                            f"{{{','.join(map(str, Ls))}}}, odd depth 3..15, 5..50 iterations, circuit targets of 3x "
+                            "depth materialised at admission; FP32 sweeps, last 3 iterations FP64; trust region with "
+                            "tr_reuse; admission by free device memory",
+                "rel_err_note": "final gates of each job: the served (dense-row) FP64 path and an FP32 sweep against "
+                                "the FP64 circuit-target path, all basis columns up to L=12, 64 columns beyond"})
     return out
 
 
@@ -293,17 +436,24 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--rtr-iters", type=int, default=50, help="full trust-region iterations timed for the extra field")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-sweep variant")
-    ap.add_argument("--serve-jobs", type=int, default=32, help="jobs of the served stream (0 skips it)")
-    ap.add_argument("--serve-Ls", default="8,10,12",
-                    help="qubit counts of the served stream's jobs (with 14: minutes for the one-job baselines)")
-    ap.add_argument("--serve-baselines", default="dense,plain,reuse",
-                    help="one-job-at-a-time FP64 runs of the same stream to compare with (subset of dense,plain,reuse)")
-    ap.add_argument("--large", action="store_true", help="also run the L=16 instance (about 20 s of GPU time)")
+    ap.add_argument("--serve-jobs", type=int, default=8,
+                    help="jobs of the served stream at its stated qubit counts (0 skips it)")
+    ap.add_argument("--serve-Ls", default="8,10,12,14,16", help="qubit counts of the served stream's jobs")
+    ap.add_argument("--serve-baselines", default="plain",
+                    help="one-job-at-a-time FP64 runs of the same stream to compare with (subset of plain,dense,reuse)")
+    ap.add_argument("--serve-baseline-limit-s", type=float, default=45.0,
+                    help="each one-job baseline of the served stream stops after this many seconds")
+    ap.add_argument("--serve-small-jobs", type=int, default=32,
+                    help="jobs of a second served stream with L in {8,10,12}, compared with all three one-job "
+                         "baselines run to the end (0 skips it)")
+    ap.add_argument("--no-large", action="store_true", help="skip the L=16 instance (about 30 s of GPU time)")
     ap.add_argument("--batched-reference-flow", action="store_true",
                     help="batched workload with the reference trust-region control flow instead of tr_reuse")
-    ap.add_argument("--batched-iters", type=int, default=2,
+    ap.add_argument("--batched-iters", type=int, default=20,
                     help="trust-region iterations per instance of the 64-instance batched workload (20 is the "
-                         "full workload, about 2.5 min of GPU time; 0 skips it)")
+                         "full workload; 0 skips it)")
+    ap.add_argument("--only-served", action="store_true",
+                    help="only the served stream (after the headline steps), e.g. with a long baseline limit")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 5 times (reference kernel for HBM byte counters)")
     args = ap.parse_args()
@@ -334,15 +484,17 @@ def main():
 
     flops = sweep_flops(L, G)
     achieved = flops / (kernel_ms * 1e-3) / 1e12
-    roofline = {"bound": "mfma", "achieved": achieved, "peak": PEAK_TFLOPS[dtype], "unit": "TFLOP/s",
+    roofline = {"bound": "valu_" + ("fp64" if dtype == "f64" else "fp32"), "achieved": achieved,
+                "peak": PEAK_TFLOPS[dtype], "unit": "TFLOP/s",
                 "frac": achieved / PEAK_TFLOPS[dtype], "traffic": measured_traffic(info["kernel"], dtype),
                 "kernel": info["kernel"],
                 "kernel_ms": kernel_ms, "flops_per_launch": flops,
-                "note": "VALU FP64/FP32 FMA bound (MI355X FP64 vector peak equals its FP64 matrix peak); traffic = "
-                        "HBM bytes per launch from rocprofv3 FETCH_SIZE/WRITE_SIZE, profiles/"}
+                "note": "the sweeps are vector-ALU FMA work (no MFMA instructions); peak = MI355X FP64 vector rate "
+                        "(equal to its FP64 matrix rate) or FP32 vector rate; traffic = HBM bytes per launch from "
+                        "rocprofv3 FETCH_SIZE/WRITE_SIZE, profiles/"}
 
     extra = {}
-    if prec == mg.MG_FP64 and not args.no_fp32:
+    if prec == mg.MG_FP64 and not args.no_fp32 and not args.only_served:
         log("FP32-sweep variant ...")
         (f32, g32, hv32), dt32, k32, info32 = time_steps(mg, t, p, V, v, mg.MG_FP32, args.steps, args.warmup)
         a32 = flops / (k32 * 1e-3) / 1e12
@@ -358,7 +510,7 @@ def main():
     # full trust-region iterations (f+grad, tCG HVPs, f at the candidate): FP64 with the
     # reference control flow, and FP32 sweeps with the last 3 iterations in FP64 and the served
     # trust region (tr_reuse), one job, one launch per evaluation
-    if args.rtr_iters > 0:
+    if args.rtr_iters > 0 and not args.only_served:
         t0 = time.perf_counter()
         f_iter, _, _, nh = mg.optimize(t, p, V, args.rtr_iters, hvp=True, precision=prec)
         rt = time.perf_counter() - t0
@@ -378,17 +530,28 @@ def main():
 
     if args.serve_jobs > 0:
         extra["served"] = served_bench(mg, args.serve_jobs, Ls=tuple(int(x) for x in args.serve_Ls.split(",")),
-                                       baselines=tuple(x for x in args.serve_baselines.split(",") if x))
-    if args.batched_iters > 0:
+                                       baselines=tuple(x for x in args.serve_baselines.split(",") if x),
+                                       limit_s=args.serve_baseline_limit_s)
+    if args.serve_small_jobs > 0 and not args.only_served:
+        extra["served_L8_to_12"] = served_bench(mg, args.serve_small_jobs, Ls=(8, 10, 12),
+                                                baselines=("plain", "dense", "reuse"), limit_s=120.0)
+    if args.batched_iters > 0 and not args.only_served:
         extra["batched"] = batched_bench(mg, args.batched_iters, tr_reuse=not args.batched_reference_flow)
-    if args.large:
+    if not args.no_large and not args.only_served:
         mg.set_kernel_timing(True)
         extra["large_L16"] = large_bench(mg)
 
     cpu = None
-    if not args.no_cpu_baseline:
-        log("timing the CPU baseline (oracle, bounded sample) ...")
-        cpu = cpu_baseline(W, V, v, param, wires, L)
+    if not args.no_cpu_baseline and not args.only_served:
+        nt = cpu_threads()
+        log(f"CPU baselines on {nt} threads: the reference's own code (bounded column samples), the oracle's HVP, "
+            "the L=8 configuration ...")
+        hvp_port = cpu_hvp_port(W, V, v, param, wires, L, nt)
+        cpu = cpu_reference_algorithm(W, L, nt) or hvp_port
+        if cpu is not hvp_port:
+            cpu["hvp_port"] = hvp_port
+        cpu["host"] = host_info()
+        extra["cpu_L8"] = cpu_small_config(nt)
 
     out = {
         "metric": "trust-region iters/s (f+grad+HVP) at L=12; served jobs/s + p50/p99 latency",

@@ -191,7 +191,7 @@ typedef struct mg_job {
     struct matchgate* gates_opt;         /* out: [nparams] */
     double t_admit, t_done;              /* out: seconds after the start */
     int nhvp;                            /* out: HVP evaluations */
-    int status;                          /* out: 0 ok */
+    int status;                          /* out: 0 ok, -1 failed, -2 stopped or not run (time_limit_s) */
 } mg_job;
 
 typedef struct mg_serve_options {
@@ -214,6 +214,9 @@ typedef struct mg_serve_options {
                              after a rejected step; same precision) takes f + gradient from there (one
                              evaluation per iteration fewer); 0: the control flow of
                              riemannian_trust_region_optimize_hvp */
+    double time_limit_s;  /* > 0: no evaluation starts later than this many seconds after the start;
+                             a job cut short ends with status -2 (its outputs are not valid), a job
+                             not yet started is not run (status -2); 0: no limit */
 } mg_serve_options;
 
 typedef struct mg_serve_stats {

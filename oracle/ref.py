@@ -256,3 +256,21 @@ def circuit_target_and_gradient(U, gates, wires):
                                                             _p(dg)) < 0:
         raise RuntimeError("matchgate_circuit_unitary_target_and_gradient failed")
     return f[0], dg
+
+
+def time_target_columns(U, vlist, perms, ncols, nthreads, hessian=True):
+    """Wall seconds of the reference's per-column target work (ufunc seed, forward sweep, trace
+    term and, with hessian, the backward sweep with the placeholder Hessian and accumulation;
+    src/matchgate_target.c:746-864) over the columns [0, ncols) on nthreads OpenMP threads,
+    and the trace over those columns."""
+    vlist = _c(vlist)
+    m, uf = _dense_map(U)
+    L = int(np.log2(m.n))
+    pm, arr = _perm_ptrs(perms)
+    f = np.zeros(1, np.complex128)
+    fn = _lib.ref_time_target_columns
+    fn.restype = ctypes.c_double
+    fn.argtypes = [_P, _P, _P, _I, _I, _P, ctypes.c_long, _I, _I, _P]
+    dt = fn(uf, ctypes.cast(ctypes.byref(m), _P), _p(vlist), pm.shape[0], L, ctypes.cast(arr, _P), int(ncols),
+            int(nthreads), int(bool(hessian)), _p(f))
+    return dt, f[0]

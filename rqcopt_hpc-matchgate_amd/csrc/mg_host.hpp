@@ -1,6 +1,7 @@
 // Host-side helpers shared by the C-ABI files: projection of the sweep outputs
 // onto the Riemannian gradient and Hessian-vector product.
 #pragma once
+#include <functional>
 #include <memory>
 
 #include "../../include/matchgate.h"
@@ -32,6 +33,6 @@ void weingarten_block(const struct matchgate* V, const zd* g, const struct match
 void rtr_optimize_hvp(const target_func f, const target_gradient_func f_deriv, const hessian_vector_product_func f_hvp,
                       void* fdata, retract_func retract, void* rdata, const int n, const double* x_init, const int m,
                       const struct rtr_params* params, const int niter, double* f_iter, double* x_final,
-                      bool heta_from_cg);
+                      bool heta_from_cg, const std::function<bool()>* stop = nullptr);
 
 }  // namespace mg

@@ -80,6 +80,7 @@ struct JobRt {
     std::thread th;
     bool finished = false;
     bool big = false;  // its evaluations fill the GPU alone: posted to the second lane
+    bool cut = false;  // stopped by the server's time limit
 };
 
 struct Server {
@@ -89,6 +90,13 @@ struct Server {
     std::vector<Posted*> big_posted;  // the second lane
     bool stop_big = false;
     int running = 0, running_small = 0;
+    std::chrono::steady_clock::time_point t0;
+    double time_limit_s = 0.0;  // > 0: evaluations that would start later fail (jobs end with status -2)
+    bool past_limit() const
+    {
+        return time_limit_s > 0.0 &&
+               std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() > time_limit_s;
+    }
     bool batched = true;
     long rounds = 0, requests = 0;
     double kernel_ms = 0.0, eval_s = 0.0, prep_ms = 0.0;
@@ -105,6 +113,10 @@ int job_precision(const JobRt* j)
 int job_eval(JobRt* j, const zd* gates, const zd* Z, int mode, mg::EvalOut* out)
 {
     const mg::Program& prog = j->spec->program->p;
+    if (j->srv->past_limit()) {
+        j->cut = true;
+        return -1;
+    }
     if (!j->srv->batched) {
         mg::EvalStats st;
         const auto t0 = std::chrono::steady_clock::now();
@@ -260,11 +272,13 @@ void run_job(JobRt* j)
     int P = s->program->p.P;
     struct rtr_params prm;
     set_rtr_default_params(16 * P, &prm);
+    // a failed (or time-limited) evaluation ends the job's trust region at once
+    const std::function<bool()> stop = [j] { return j->failed != 0; };
     mg::rtr_optimize_hvp(job_f, job_f_deriv, job_hvp, j, job_retract, &P, 8 * P,
                          reinterpret_cast<const double*>(s->gates_start), 16 * P, &prm, s->niter, s->f_iter,
-                         reinterpret_cast<double*>(s->gates_opt), j->reuse);
+                         reinterpret_cast<double*>(s->gates_opt), j->reuse, &stop);
     s->nhvp = j->nhvp;
-    s->status = j->failed ? -1 : 0;
+    s->status = j->cut ? -2 : j->failed ? -1 : 0;
 }
 
 // device memory a job holds while admitted: its target, its coefficients and its
@@ -499,6 +513,8 @@ int mg_serve(mg_job* jobs, int njobs, const mg_serve_options* opt, mg_serve_stat
 
     Server srv;
     srv.batched = opt->batched != 0;
+    srv.t0 = t0;
+    srv.time_limit_s = opt->time_limit_s;
     std::vector<std::unique_ptr<JobRt>> rt(njobs);
     int next = 0, done = 0, peak_active = 0, rc = 0;
     size_t held = 0, peak_held = 0;
@@ -539,13 +555,19 @@ int mg_serve(mg_job* jobs, int njobs, const mg_serve_options* opt, mg_serve_stat
             const int i = order[k];
             const double wait = arrival(k) - now_s();
             if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(wait));
+            if (srv.past_limit()) {
+                jobs[i].status = -2;
+                jobs[i].t_admit = jobs[i].t_done = now_s();
+                done++;
+                continue;
+            }
             if (!start_job(i)) { rc = -1; continue; }
             peak_active = 1;
             peak_held = std::max(peak_held, rt[i]->footprint);
             run_job(rt[i].get());
             rt[i]->target.reset();
             jobs[i].t_done = now_s();
-            if (jobs[i].status != 0) rc = -1;
+            if (jobs[i].status == -1) rc = -1;
             done++;
             report(srv.rounds, srv.requests);
         }
@@ -585,6 +607,13 @@ int mg_serve(mg_job* jobs, int njobs, const mg_serve_options* opt, mg_serve_stat
             // admit arrived jobs while slots and device memory allow
             while (next < njobs && arrival(next) <= now_s()) {
                 const int i = order[next];
+                if (srv.past_limit()) {
+                    jobs[i].status = -2;
+                    jobs[i].t_admit = jobs[i].t_done = now_s();
+                    next++;
+                    done++;
+                    continue;
+                }
                 const size_t fp = job_footprint(jobs[i], opt->densify_qubits);
                 const bool slot = opt->max_active <= 0 || srv.running < opt->max_active;
                 const bool mem = opt->hbm_budget <= 0 || srv.running == 0 || (double)(held + fp) <= opt->hbm_budget;
@@ -623,7 +652,7 @@ int mg_serve(mg_job* jobs, int njobs, const mg_serve_options* opt, mg_serve_stat
                     j->target.reset();
                     held -= j->footprint;
                     j->spec->t_done = now_s();
-                    if (j->spec->status != 0) rc = -1;
+                    if (j->spec->status == -1) rc = -1;
                     done++;
                 }
             }

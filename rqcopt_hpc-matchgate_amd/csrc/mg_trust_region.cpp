@@ -60,7 +60,8 @@ double boundary_step(const double* b, const double* d, int n, double radius)
 // the iterate was moved to the boundary.  Hz (nullable) receives H z, accumulated from the
 // products H d of the CG directions with the same steps as z.
 bool steihaug(const double* grad, const std::function<void(const double*, double*)>& hv, int n, double radius,
-              const truncated_cg_params* prm, double* z, double* Hz = nullptr)
+              const truncated_cg_params* prm, double* z, double* Hz = nullptr,
+              const std::function<bool()>* stop = nullptr)
 {
     std::vector<double> r(grad, grad + n), d(n), Hd(n);
     double rsq = dot(n, r.data(), r.data());
@@ -70,6 +71,7 @@ bool steihaug(const double* grad, const std::function<void(const double*, double
     for (int k = 0; k < n; k++) d[k] = -r[k];
     for (int it = 0; it < prm->maxiter; it++) {
         hv(d.data(), Hd.data());
+        if (stop && (*stop)()) return false;
         const double dHd = dot(n, d.data(), Hd.data());
         const double t = boundary_step(z, d.data(), n, radius);
         const double alpha = rsq / dHd;
@@ -159,22 +161,34 @@ namespace mg {
 void rtr_optimize_hvp(const target_func f, const target_gradient_func f_deriv, const hessian_vector_product_func f_hvp,
                       void* fdata, retract_func retract, void* rdata, const int n, const double* x_init, const int m,
                       const struct rtr_params* params, const int niter, double* f_iter, double* x_final,
-                      bool heta_from_cg)
+                      bool heta_from_cg, const std::function<bool()>* stop)
 {
     assert(0 <= params->rho_trust && params->rho_trust < 0.25);
     std::vector<double> x(x_init, x_init + m), xn(m), grad(n), eta(n), Heta(n);
     double radius = params->radius_init;
     numeric fc;
     if (params->g_func) params->g_iter[0] = params->g_func(x.data(), params->g_data, &fc);
+    // a caller's stop request (an evaluation failed): the remaining f_iter are NaN, x_final the
+    // current iterate
+    auto stopped = [&](int k) {
+        if (!stop || !(*stop)()) return false;
+        for (int i = k; i <= niter; i++) f_iter[i] = NAN;
+        std::memcpy(x_final, x.data(), m * sizeof(double));
+        return true;
+    };
     for (int k = 0; k < niter; k++) {
         const double fx = f_deriv(x.data(), fdata, grad.data());
+        if (stopped(k)) return;
         const bool bnd = steihaug(
             grad.data(), [&](const double* d, double* Hd) { f_hvp(x.data(), fdata, const_cast<double*>(d), Hd); }, n,
-            radius, &params->tcg_params, eta.data(), heta_from_cg ? Heta.data() : nullptr);
+            radius, &params->tcg_params, eta.data(), heta_from_cg ? Heta.data() : nullptr, stop);
+        if (stopped(k)) return;
         retract(x.data(), eta.data(), rdata, xn.data());
         f_iter[k] = fx;
         if (!heta_from_cg) f_hvp(x.data(), fdata, eta.data(), Heta.data());
-        const double rho = (f(xn.data(), fdata, &fc) - fx) / (dot(n, grad.data(), eta.data()) + 0.5 * dot(n, eta.data(), Heta.data()));
+        const double fxn = f(xn.data(), fdata, &fc);
+        if (stopped(k + 1)) return;
+        const double rho = (fxn - fx) / (dot(n, grad.data(), eta.data()) + 0.5 * dot(n, eta.data(), Heta.data()));
         bool accept;
         tr_update(rho, bnd, params, &radius, &accept);
         if (accept) x.swap(xn);

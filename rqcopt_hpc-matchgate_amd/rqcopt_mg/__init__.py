@@ -73,7 +73,8 @@ class ServeOptions(ctypes.Structure):
     """struct mg_serve_options (include/mg_gpu.h)"""
     _fields_ = [("batched", ctypes.c_int), ("precision", ctypes.c_int), ("precision_switch", ctypes.c_int),
                 ("max_active", ctypes.c_int), ("hbm_budget", _D), ("realtime", ctypes.c_int),
-                ("densify_qubits", ctypes.c_int), ("progress_s", _D), ("tr_reuse", ctypes.c_int)]
+                ("densify_qubits", ctypes.c_int), ("progress_s", _D), ("tr_reuse", ctypes.c_int),
+                ("time_limit_s", _D)]
 
 
 class ServeStats(ctypes.Structure):
@@ -468,7 +469,7 @@ def batch_last_graph_stats():
 
 
 def serve(jobs, batched=True, precision=MG_FP32, precision_switch=3, max_active=0, hbm_budget=0.0,
-          realtime=True, densify_qubits=0, progress_s=0.0, tr_reuse=False):
+          realtime=True, densify_qubits=0, progress_s=0.0, tr_reuse=False, time_limit_s=0.0):
     """Run a stream of trust-region optimisation jobs (mg_serve).
 
     jobs: list of dicts with keys program (Program), gates (P, 8), niter, arrival (s),
@@ -476,6 +477,7 @@ def serve(jobs, batched=True, precision=MG_FP32, precision_switch=3, max_active=
     tr_reuse: H eta of each ratio test from the truncated CG's own products and f + gradient
     kept across rejected steps (mg_serve_options.tr_reuse); False keeps the control flow of
     riemannian_trust_region_optimize_hvp.
+    time_limit_s > 0: no evaluation starts later; jobs cut short or not started end with status -2.
     Returns (results, stats): per job a dict with f_iter, gates, t_admit, t_done,
     latency (t_done - arrival), nhvp, status; stats a dict of mg_serve_stats.
     """
@@ -498,7 +500,8 @@ def serve(jobs, batched=True, precision=MG_FP32, precision_switch=3, max_active=
                      f_iter.ctypes.data, gout.ctypes.data, 0.0, 0.0, 0, 0)
         outs.append((f_iter, gout))
     opt = ServeOptions(int(bool(batched)), int(precision), int(precision_switch), int(max_active), float(hbm_budget),
-                       int(bool(realtime)), int(densify_qubits), float(progress_s), int(bool(tr_reuse)))
+                       int(bool(realtime)), int(densify_qubits), float(progress_s), int(bool(tr_reuse)),
+                       float(time_limit_s))
     st = ServeStats()
     _check(lib.mg_serve(arr, n, ctypes.byref(opt), ctypes.byref(st)), "mg_serve")
     res = []

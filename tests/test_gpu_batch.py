@@ -296,6 +296,34 @@ def test_serve_mixed_sizes_with_large_jobs(mg):
         del t
 
 
+def test_serve_time_limit(mg):
+    """time_limit_s: a job that would evaluate past the limit ends with status -2 (not an error of
+    the call); jobs before the limit are unaffected."""
+    jobs = _jobs(mg, 26, SERVE_SPEC[:3])
+    say("case: serve with a time limit")
+    for batched in (True, False):
+        res, _ = mg.serve(jobs, batched=batched, precision=mg.MG_FP64, realtime=False, time_limit_s=1e-9)
+        assert all(r["status"] == -2 for r in res)
+    full, _ = mg.serve(jobs, batched=False, precision=mg.MG_FP64, realtime=False)
+    lim, _ = mg.serve(jobs, batched=False, precision=mg.MG_FP64, realtime=False, time_limit_s=3600.0)
+    for a, b in zip(full, lim):
+        assert a["status"] == 0 and b["status"] == 0 and np.array_equal(a["f_iter"], b["f_iter"])
+    # a limit inside a job's run: that job stops at its next evaluation (status -2, the rest of
+    # f_iter NaN), its finished iterations equal the unlimited run's, the later jobs do not run
+    say("case: serve with a limit inside the first job")
+    long_job = _jobs(mg, 27, [(8, 5, 40)])
+    for batched in (False, True):
+        ref, _ = mg.serve(long_job, batched=batched, precision=mg.MG_FP64, realtime=False)
+        assert ref[0]["status"] == 0
+        cut, st = mg.serve(long_job + jobs, batched=batched, precision=mg.MG_FP64, realtime=False,
+                           max_active=1, time_limit_s=0.25 * ref[0]["latency"])
+        assert cut[0]["status"] == -2 and all(r["status"] == -2 for r in cut[1:])
+        fi = cut[0]["f_iter"]
+        k = int(np.argmax(np.isnan(fi)))
+        assert 0 < k < len(fi) and np.all(np.isnan(fi[k:]))
+        assert np.array_equal(fi[:k], ref[0]["f_iter"][:k])
+
+
 def test_serve_admission_limits(mg):
     jobs = _jobs(mg, 22, SERVE_SPEC[:4])
     say("case: serve with one slot and with a tight memory budget")
