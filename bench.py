@@ -364,24 +364,25 @@ def batched_bench(mg, niter, seed=0, tr_reuse=True):
                         ("; trust region with tr_reuse" if tr_reuse else "; reference trust-region control flow")}
 
 
-def large_bench(mg):
+def large_bench(mg, L=16):
     """The large single instance: L=16, open brickwall depth 11 (83 gates), target = 50
     second-order Trotter steps of the XXZ chain with random fields (758 gates); one f + gradient
     + HVP over all 65536 columns with FP32 sweeps and with FP64 sweeps, first with the target
     applied per column (batched kernel, states in global scratch, gate windows), then with the
-    target materialised once in HBM and read as dense rows (tiled kernels)."""
+    target materialised once in HBM and read as dense rows (tiled kernels).  L=14 runs the same
+    construction at 14 qubits (a profiling size of the same kernels)."""
     from rqcopt_mg.workloads import large_instance, sweep_flops
 
-    w = large_instance()
+    w = large_instance(L=L)
     prog = mg.Program.create(w["L"], w["G"], w["param"], w["wires"])
     tgt = mg.Target.circuit(w["target_gates"], w["target_wires"], w["L"])
     v = np.random.default_rng(0).standard_normal(8 * w["G"])
     fl = sweep_flops(w["L"], w["G"], w["Gt"])
-    out = {"workload": "L=16 XXZ Trotter-2 target (50 steps, t=1, 758 gates), open brickwall depth 11 (83 gates), "
-                       "65536 columns", "flops": fl}
+    out = {"workload": f"L={L} XXZ Trotter-2 target (50 steps, t=1, {w['Gt']} gates), open brickwall depth 11 "
+                       f"({w['G']} gates), {1 << L} columns", "flops": fl}
     res = {}
     for name, prec in (("fp32", mg.MG_FP32), ("fp64", mg.MG_FP64)):
-        log(f"L=16 f+grad+HVP, {name} sweeps ...")
+        log(f"L={L} f+grad+HVP, {name} sweeps ...")
         req = {"target": tgt, "program": prog, "gates": w["gates"], "v": v, "what": mg.EVAL_F_GRAD_HVP, "precision": prec}
         t0 = time.perf_counter()
         res[name] = mg.evaluate_batch([req])[0]
@@ -454,9 +455,13 @@ def main():
                          "full workload; 0 skips it)")
     ap.add_argument("--only-served", action="store_true",
                     help="only the served stream (after the headline steps), e.g. with a long baseline limit")
+    ap.add_argument("--only-large", type=int, default=0, metavar="L",
+                    help="only the large instance at L qubits (14 or 16) after the headline steps (profiling runs)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 5 times (reference kernel for HBM byte counters)")
     args = ap.parse_args()
+    if args.only_large:
+        args.only_served, args.serve_jobs = True, 0
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
         return
@@ -528,6 +533,9 @@ def main():
         log(f"trust region: {extra['rtr']['iters_per_s']:.1f} it/s FP64, "
             f"{extra['rtr_fp32_sweeps']['iters_per_s']:.1f} it/s FP32 sweeps + tr_reuse")
 
+    if args.only_large:
+        mg.set_kernel_timing(True)
+        extra["large_L%d" % args.only_large] = large_bench(mg, args.only_large)
     if args.serve_jobs > 0:
         extra["served"] = served_bench(mg, args.serve_jobs, Ls=tuple(int(x) for x in args.serve_Ls.split(",")),
                                        baselines=tuple(x for x in args.serve_baselines.split(",") if x),
