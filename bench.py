@@ -368,9 +368,9 @@ def large_bench(mg, L=16):
     """The large single instance: L=16, open brickwall depth 11 (83 gates), target = 50
     second-order Trotter steps of the XXZ chain with random fields (758 gates); one f + gradient
     + HVP over all 65536 columns with FP32 sweeps and with FP64 sweeps, first with the target
-    applied per column (batched kernel, states in global scratch, gate windows), then with the
-    target materialised once in HBM and read as dense rows (tiled kernels).  L=14 runs the same
-    construction at 14 qubits (a profiling size of the same kernels)."""
+    given as its gates (its rows materialised one 4 GiB column chunk at a time next to the tiled
+    sweeps), then with the target materialised once in HBM (69 GB of FP64 rows) and read as dense
+    rows by the same tiled kernels.  L=14 runs the same construction at 14 qubits."""
     from rqcopt_mg.workloads import large_instance, sweep_flops
 
     w = large_instance(L=L)
@@ -389,8 +389,8 @@ def large_bench(mg, L=16):
         dt = time.perf_counter() - t0
         km = mg.batch_last_kernel_ms()
         out[name] = {"s": dt, "kernel_ms": km, "tflops": fl / (km * 1e-3) / 1e12,
-                     "kernel": "batch_kernel<%s, %d, %d>" % ((("float",) if prec == mg.MG_FP32 else ("double",)) +
-                                                              mg.batch_get_kernel(prec))}
+                     "kernels": "sweep_win_tile_kernel<3,1> (target rows of a column chunk) + "
+                                + ("sweep_tile_kernel<float,2>" if prec == mg.MG_FP32 else "sweep_win_tile_kernel<2,2>")}
         log(f"  {dt:.2f} s, kernel {km:.1f} ms, {out[name]['tflops']:.2f} TFLOP/s")
     log("materialising the target in HBM (FP64 rows, then an FP32 copy) ...")
     t0 = time.perf_counter()

@@ -257,6 +257,10 @@ void mg_set_window_kernel(int mode);
 /* tiled kernel for dense targets whose states exceed the LDS (L >= 13; FP64 L=12 for gates not
  * unitary to 1e-12): mode -1 automatic (default), 0 never (global-scratch or register kernels) */
 void mg_set_tile_kernel(int mode);
+/* program (circuit or brickwall) targets beyond 12 qubits are evaluated by materialising their
+ * dense rows one column chunk at a time next to the tiled sweeps: bytes of rows per chunk (at least
+ * one grid of columns; 0 restores the default 4 GiB) */
+void mg_set_row_chunk_bytes(long bytes);
 /* record HIP events around every sweep kernel on the library's stream */
 void mg_set_kernel_timing(int on);
 /* duration of the last sweep kernel in ms (needs mg_set_kernel_timing(1)) */

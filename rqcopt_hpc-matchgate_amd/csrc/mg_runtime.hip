@@ -73,6 +73,7 @@ struct Context {
     int ncu = 0;
     size_t lds_limit = 0;
     DevBuf inst, coef, partial, out, scratch, ckpt;
+    DevBuf rows;  // materialised target rows of one column chunk (program targets beyond 12 qubits)
     DevBuf w_inst, w_coef, w_blk;  // gate-window form of the program (register window kernel)
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::vector<double> out_h;
@@ -489,6 +490,71 @@ static bool make_windowed_tiles(const std::vector<Inst>& inst, const std::vector
     return true;
 }
 
+// the target program's 12-qubit tile form (window blocks inside tile blocks), uploaded once
+static int prepare_tile_form(Context* c, Target& t)
+{
+    Target::TileForm& f = t.tf;
+    if (f.ready) return 0;
+    WindowedTiles wt;
+    if (!make_windowed_tiles(t.tinst_h, t.tcoef_h, t.L, &wt)) {
+        set_error("target materialisation: gate blocking failed");
+        return -1;
+    }
+    MG_HIP(f.inst.ensure(wt.inst.size() * sizeof(Inst)));
+    MG_HIP(f.coef.ensure(wt.coef.size() * sizeof(cpx<double>)));
+    const size_t nb = wt.blk_lds.size() * sizeof(WBlock);
+    MG_HIP(f.blk.ensure(2 * nb + wt.tblocks.size() * sizeof(WTBlock)));
+    unsigned char* blk = (unsigned char*)f.blk.p;
+    MG_HIP(hipMemcpy(f.inst.p, wt.inst.data(), wt.inst.size() * sizeof(Inst), hipMemcpyHostToDevice));
+    MG_HIP(hipMemcpy(f.coef.p, wt.coef.data(), wt.coef.size() * sizeof(cpx<double>), hipMemcpyHostToDevice));
+    MG_HIP(hipMemcpy(blk, wt.blk_lds.data(), nb, hipMemcpyHostToDevice));
+    MG_HIP(hipMemcpy(blk + nb, wt.blk_glob.data(), nb, hipMemcpyHostToDevice));
+    MG_HIP(hipMemcpy(blk + 2 * nb, wt.tblocks.data(), wt.tblocks.size() * sizeof(WTBlock), hipMemcpyHostToDevice));
+    f.G = (int)wt.inst.size();
+    f.ntb = (int)wt.tblocks.size();
+    f.nblk = (int)wt.blk_lds.size();
+    f.slots = max_block_gates(wt.blk_lds);
+    int rs = 0;
+    win_sweep_lds_bytes(12, 0, 0, c->lds_limit, &f.acc_off, &f.buf_off, &rs, 2, f.slots);
+    f.lds = f.buf_off + ((size_t)1 << 12) * sizeof(cpx<double>);
+    f.ready = true;
+    return 0;
+}
+
+// arguments of the materialisation of rows [b0, b1) into `rows` (row b at rows + (b - row0) S)
+static WinTileArgs tile_form_args(const Target& t, long b0, long b1, void* rows, long row0)
+{
+    const Target::TileForm& f = t.tf;
+    const size_t nb = (size_t)f.nblk * sizeof(WBlock);
+    unsigned char* blk = (unsigned char*)f.blk.p;
+    WinTileArgs a{};
+    a.L = t.L;
+    a.G = f.G;
+    a.P = 0;
+    a.ntb = f.ntb;
+    a.inst = (const Inst*)f.inst.p;
+    a.coef = f.coef.p;
+    a.tblocks = (const WTBlock*)(blk + 2 * nb);
+    a.blk_lds = (const WBlock*)blk;
+    a.blk_glob = (const WBlock*)(blk + nb);
+    a.b0 = b0;
+    a.b1 = b1;
+    a.nstate = 1;
+    a.acc_off = (int)f.acc_off;
+    a.buf_off = (int)f.buf_off;
+    a.rs = 1;
+    a.slots = f.slots;
+    a.rows = rows;
+    a.row0 = row0;
+    return a;
+}
+
+static int tile_form_grid(const Context* c, const Target& t, long ncols)
+{
+    const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(2, c->lds_limit / t.tf.lds));
+    return (int)std::min<long>(ncols, (long)c->ncu * per_cu);
+}
+
 // ---- evaluation -------------------------------------------------------------------------------
 // Device buffers of one evaluation: the context's grow-only buffers, or (allocation-free
 // evaluation) consecutive 256-byte aligned pieces of the static scratch array.
@@ -599,13 +665,19 @@ static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, in
     // tiled kernel: dense targets whose states do not fit the LDS (global state slices, one LDS
     // tile of 2^kw amplitudes per state and pass)
     const int tile_kw = tile_window_bits(fp32);
+    // program targets beyond 12 qubits are read as dense rows too: the rows of one column chunk
+    // at a time, materialised (12-qubit register tiles) into a chunk buffer just before the tiled
+    // sweep of that chunk.  Applying the target's gates per column in the sweep instead streams
+    // the states through memory once per gate window (HBM-bound at these sizes).
+    const bool chunkable = t.kind == 1 && L > 12 && L - 12 <= 20 && kn.force_win != 0;
     size_t tile_off = 0, tile_lds = 0;
     bool use_tile = false;
-    if (!use_win && !ckpt && !lds_states && t.kind == 0 && L > tile_kw && kn.force_tile != 0 && kn.force_lds == -1 &&
-        kn.force_reg != 1) {
+    if (!use_win && !ckpt && !lds_states && (t.kind == 0 || chunkable) && L > tile_kw && kn.force_tile != 0 &&
+        kn.force_lds == -1 && kn.force_reg != 1) {
         tile_lds = tile_sweep_lds_bytes(p.P, tile_kw, fp32, nstate_tile(mode), &tile_off);
         use_tile = tile_lds <= c->lds_limit;
     }
+    const bool chunk_rows = chunkable && use_tile;
     if (use_tile) reg_T = 0;
     // tiled register-window kernel (FP64, unitary gates): preferred over the LDS tiles
     bool use_wtile = false;
@@ -643,12 +715,26 @@ static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, in
     const size_t ck_per = ckpt ? (size_t)2 * (p.G + 1) * S * sizeof(cpx<R>) : 0;
     if (ckpt) grid = std::min<long>(grid, std::max<long>(1, lim / ck_per));
     grid = std::max<long>(grid, 1);
+    // column chunks of materialised target rows: at most 4 GiB of rows, a multiple of the grid;
+    // each chunk's workgroup partial sums are kept and all are reduced together
+    long chunk = ncols, nchunks = 1;
+    if (chunk_rows) {
+        const long row_bytes = (long)(S * sizeof(cpx<double>));
+        const size_t cb = kn.row_chunk_bytes ? kn.row_chunk_bytes : ((size_t)4 << 30);
+        chunk = std::max<long>(grid, (long)(cb / (size_t)row_bytes) / grid * grid);
+        chunk = std::min<long>(chunk, ncols);
+        nchunks = (ncols + chunk - 1) / chunk;
+    }
 
     const int stride = 2 + 32 * p.P;
-    void *d_partial = nullptr, *d_out = nullptr, *d_scratch = nullptr, *d_ckpt = nullptr;
-    if (ar.get(c->partial, (size_t)grid * stride * sizeof(double), &d_partial) < 0 ||
+    void *d_partial = nullptr, *d_out = nullptr, *d_scratch = nullptr, *d_ckpt = nullptr, *d_rows = nullptr;
+    if (ar.get(c->partial, (size_t)grid * nchunks * stride * sizeof(double), &d_partial) < 0 ||
         ar.get(c->out, (size_t)stride * sizeof(double), &d_out) < 0)
         return -1;
+    if (chunk_rows) {
+        if (ar.get(c->rows, (size_t)chunk * S * sizeof(cpx<double>), &d_rows) < 0) return -1;
+        if (prepare_tile_form(c, t) < 0) return -1;
+    }
     if (use_tile) {
         if (ar.get(c->scratch, (size_t)grid * tile_state_bytes, &d_scratch) < 0) return -1;
     } else if (!lds_states && !reg_T && !use_win) {
@@ -814,7 +900,32 @@ static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, in
         MG_HIP(hipStreamSynchronize(s));
     }
     if (kn.time_kernels) MG_HIP(hipEventRecord(c->ev0, s));
-    if (use_wtile)
+    if (chunk_rows) {
+        for (long ci = 0; ci < nchunks; ci++) {
+            const long c0 = b0 + ci * chunk, c1 = std::min<long>(b1, c0 + chunk);
+            const WinTileArgs m = tile_form_args(t, c0, c1, d_rows, c0);
+            MG_HIP(launch_materialize_win_tile(m, tile_form_grid(c, t, c1 - c0), t.tf.lds, s));
+            double* part = (double*)d_partial + (size_t)ci * grid * stride;
+            if (use_wtile) {
+                WinTileArgs w2 = wta;
+                w2.b0 = c0;
+                w2.b1 = c1;
+                w2.seed = d_rows;
+                w2.row0 = c0;
+                w2.partial = part;
+                MG_HIP(launch_sweep_win_tile(w2, mode, (int)grid, lds, s));
+            } else {
+                TileArgs<R> t2 = ta;
+                t2.b0 = c0;
+                t2.b1 = c1;
+                t2.row0 = c0;
+                t2.seed = fp32 ? nullptr : (const cpx<R>*)d_rows;
+                t2.seed64 = fp32 ? (const cpx<double>*)d_rows : nullptr;
+                t2.partial = part;
+                MG_HIP(launch_sweep_tile<R>(t2, mode, (int)grid, lds, s));
+            }
+        }
+    } else if (use_wtile)
         MG_HIP(launch_sweep_win_tile(wta, mode, (int)grid, lds, s));
     else if (use_tile)
         MG_HIP(launch_sweep_tile<R>(ta, mode, (int)grid, lds, s));
@@ -825,7 +936,7 @@ static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, in
     else
         MG_HIP(launch_sweep<R>(a, mode, lds_states, ckpt, lds_T, (int)grid, lds, s));
     if (kn.time_kernels) MG_HIP(hipEventRecord(c->ev1, s));
-    MG_HIP(launch_reduce((const double*)d_partial, (int)grid, stride, (double*)d_out, s));
+    MG_HIP(launch_reduce((const double*)d_partial, (int)(grid * nchunks), stride, (double*)d_out, s));
     c->out_h.resize(stride);
     MG_HIP(hipMemcpyAsync(c->out_h.data(), d_out, stride * sizeof(double), hipMemcpyDeviceToHost, s));
     MG_HIP(hipStreamSynchronize(s));
@@ -910,7 +1021,8 @@ BatchTuning& batch_tuning()
 bool batch_single_request(const Target& t, const Program& p)
 {
     const BatchTuning& bt = batch_tuning();
-    return t.kind == 0 && p.L >= bt.single_min && p.L <= bt.single_max;
+    // dense targets, and program targets beyond 12 qubits (read in materialised column chunks)
+    return (t.kind == 0 || p.L > 12) && p.L >= bt.single_min && p.L <= bt.single_max;
 }
 
 BatchStats& last_batch_stats()
@@ -1001,48 +1113,10 @@ int target_densify(Target& t)
     // beyond the LDS: 12-qubit tiles of each row in registers (the tiled window kernel's
     // forward sweep), unless the knob asks for the windowed in-place rows
     if (!in_lds && L > 12 && L - 12 <= 20 && knobs().force_win != 0) {
-        WindowedTiles wt;
-        if (!make_windowed_tiles(t.tinst_h, t.tcoef_h, L, &wt)) {
-            set_error("target materialisation: gate blocking failed");
-            return -1;
-        }
-        DevBuf ti, tc, tb;
-        MG_HIP(ti.ensure(wt.inst.size() * sizeof(Inst)));
-        MG_HIP(tc.ensure(wt.coef.size() * sizeof(cpx<double>)));
-        const size_t nb = wt.blk_lds.size() * sizeof(WBlock);
-        MG_HIP(tb.ensure(2 * nb + wt.tblocks.size() * sizeof(WTBlock)));
-        unsigned char* blk = (unsigned char*)tb.p;
-        MG_HIP(hipMemcpy(ti.p, wt.inst.data(), wt.inst.size() * sizeof(Inst), hipMemcpyHostToDevice));
-        MG_HIP(hipMemcpy(tc.p, wt.coef.data(), wt.coef.size() * sizeof(cpx<double>), hipMemcpyHostToDevice));
-        MG_HIP(hipMemcpy(blk, wt.blk_lds.data(), nb, hipMemcpyHostToDevice));
-        MG_HIP(hipMemcpy(blk + nb, wt.blk_glob.data(), nb, hipMemcpyHostToDevice));
-        MG_HIP(hipMemcpy(blk + 2 * nb, wt.tblocks.data(), wt.tblocks.size() * sizeof(WTBlock), hipMemcpyHostToDevice));
-        size_t acc_off = 0, buf_off = 0;
-        int rs = 0;
-        const int slots = max_block_gates(wt.blk_lds);
-        win_sweep_lds_bytes(12, 0, 0, c->lds_limit, &acc_off, &buf_off, &rs, 2, slots);
-        const size_t lds = buf_off + ((size_t)1 << 12) * sizeof(cpx<double>);
-        WinTileArgs a{};
-        a.L = L;
-        a.G = (int)wt.inst.size();
-        a.P = 0;
-        a.ntb = (int)wt.tblocks.size();
-        a.inst = (const Inst*)ti.p;
-        a.coef = tc.p;
-        a.tblocks = (const WTBlock*)(blk + 2 * nb);
-        a.blk_lds = (const WBlock*)blk;
-        a.blk_glob = (const WBlock*)(blk + nb);
-        a.b0 = 0;
-        a.b1 = (long)S;
-        a.nstate = 1;
-        a.acc_off = (int)acc_off;
-        a.buf_off = (int)buf_off;
-        a.rs = 1;
-        a.slots = slots;
-        a.rows = t.seed64.p;
-        const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(2, c->lds_limit / lds));
-        const int grid = (int)std::min<long>((long)S, (long)c->ncu * per_cu);
-        MG_HIP(launch_materialize_win_tile(a, grid, lds, c->s));
+        if (prepare_tile_form(c, t) < 0) return -1;
+        const WinTileArgs a = tile_form_args(t, 0, (long)S, t.seed64.p, 0);
+        const int grid = tile_form_grid(c, t, (long)S);
+        MG_HIP(launch_materialize_win_tile(a, grid, t.tf.lds, c->s));
         MG_HIP(hipStreamSynchronize(c->s));
         t.kind = 0;
         t.seed32_ready = false;

@@ -56,6 +56,16 @@ struct Target {
         int wbits = 0;  // window bits it was built for (0: not built)
     };
     WinCache wc[2];
+    // target program in 12-qubit tile form (FP64 register windows) for materialising its rows
+    // beyond 12 qubits: all at once (target_densify) or one column chunk per pass (evaluations of
+    // program targets with the tiled sweeps)
+    struct TileForm {
+        DevBuf inst, coef, blk;
+        int G = 0, ntb = 0, nblk = 0, slots = 0;
+        size_t acc_off = 0, buf_off = 0, lds = 0;
+        bool ready = false;
+    };
+    TileForm tf;
 };
 
 // U row-major (S x S)
@@ -106,6 +116,7 @@ struct Knobs {
     int force_win = -1;       // -1 auto (register gate-window kernel where it applies), 0 never
     int force_tile = -1;      // -1 auto (tiled kernel for dense targets whose states exceed the LDS), 0 never
     bool time_kernels = false;
+    size_t row_chunk_bytes = 0;  // materialised target rows per column chunk (0: 4 GiB)
 };
 Knobs& knobs();
 
@@ -159,7 +170,8 @@ bool batch_variant_exists(int precision_fp32, int threads, int wbits);
 
 int evaluate_batch(BatchReq* reqs, int n, BatchStats* st);
 // true if evaluate_batch runs a request of this target and program on the single-request
-// kernels (dense target, batch_tuning().single_min <= L <= single_max): it fills the GPU alone
+// kernels (dense target, or a program target beyond 12 qubits, and batch_tuning().single_min <= L
+// <= single_max): it fills the GPU alone
 bool batch_single_request(const Target& t, const Program& p);
 BatchStats& last_batch_stats();
 

@@ -97,7 +97,7 @@ __global__ __launch_bounds__(1 << tile_lt<R>()) void sweep_tile_kernel(TileArgs<
 
     const int nb = a.nblocks;
     for (long b = a.b0 + blockIdx.x; b < a.b1; b += gridDim.x) {
-        const size_t rb = (size_t)b * S;
+        const size_t rb = (size_t)(b - a.row0) * S;
         // seed row entry (FP32 sweeps may read the FP64 rows and round on load: the same values
         // as an FP32 copy of the rows)
         auto row = [&](uint32_t i) -> cpx<R> {

@@ -28,6 +28,7 @@ struct TileArgs {
     const cpx<double>* seed64;  // FP32 sweeps without an FP32 copy of the rows: the FP64 rows,
                                 // rounded to FP32 on load (seed unused); null otherwise
     long b0, b1;            // column range
+    long row0;              // column b's seed row is row b - row0 of seed (seed64)
     double* partial;        // [gridDim.x][2 + 32 P]
     cpx<R>* scratch;        // [gridDim.x][nstate][S]: psi, dpsi, lam, dlam (as the mode needs)
     int nstate;             // states per column: 1 (f), 2 (f + gradient), 4 (+ HVP)

@@ -365,8 +365,8 @@ __global__ __launch_bounds__(512) void sweep_win_tile_kernel(WinTileArgs a)
         asm volatile("" : "+v"(tid));
         Win<R, kW> Lm, P, DL, D;
         double fr = 0.0, fi = 0.0;
-        const cpx<R>* row = MODE == 3 ? nullptr : seed + (size_t)b * S;
-        cpx<R>* f_psi = MODE == 3 ? reinterpret_cast<cpx<R>*>(a.rows) + (size_t)b * S : g_psi;
+        const cpx<R>* row = MODE == 3 ? nullptr : seed + (size_t)(b - a.row0) * S;
+        cpx<R>* f_psi = MODE == 3 ? reinterpret_cast<cpx<R>*>(a.rows) + (size_t)(b - a.row0) * S : g_psi;
         // ---- forward: psi, dpsi, tile block by tile block ------------------------------------
         for (int tb = 0; tb < ntb; tb++) {
             const WTBlock* tbk = a.tblocks + tb;

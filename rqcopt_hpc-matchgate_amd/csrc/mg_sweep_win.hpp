@@ -46,6 +46,7 @@ struct WinTileArgs {
     int acc_off, buf_off, rs;  // as WinSweepArgs (12-qubit tiles)
     int slots;                 // as WinSweepArgs
     void* rows;                // materialisation: cpx<double> [S][S] output rows (column b's state)
+    long row0;                 // column b's seed (or materialised) row is row b - row0 of seed (rows)
 };
 hipError_t launch_sweep_win_tile(const WinTileArgs& a, int mode, int grid, size_t lds, hipStream_t s);
 // rows[b] = -conj(W e_b) for the columns [b0, b1) of the program in `a` (the target's gates,

@@ -126,6 +126,7 @@ _sig("mg_last_sweep_threads", [])
 _sig("mg_last_sweep_kernel", [], ctypes.c_char_p)
 _sig("mg_set_window_kernel", [_I], None)
 _sig("mg_set_tile_kernel", [_I], None)
+_sig("mg_set_row_chunk_bytes", [_L], None)
 _sig("mg_last_kernel_ms", [], ctypes.c_float)
 _sig("mg_last_sweep_info", [_P, _P, _P, _P], None)
 _sig("mg_target_cache_clear", [], None)

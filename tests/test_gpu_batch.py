@@ -369,6 +369,47 @@ def test_large_instances_column_subset_match_oracle(mg, L):
         del tgt
 
 
+def test_program_target_beyond_12_qubits_in_materialised_chunks(mg):
+    """Program targets beyond 12 qubits are read as dense rows materialised one column chunk at a
+    time next to the tiled sweeps.  At L=14 (one chunk by default, 32 chunks of 512 columns with a
+    small chunk size) FP64 f, gradient and HVP equal the batched kernel that applies the target's
+    gates per column (1e-11 relative), FP32 within 2e-5; a column subset matches the oracle."""
+    from rqcopt_mg.workloads import brickwall_job
+
+    rng = np.random.default_rng(41)
+    L = 14
+    job = brickwall_job(rng, L, 5, 1)
+    prog = mg.Program.create(L, job["G"], job["param"], job["wires"])
+    tgt = mg.Target.circuit(job["target_gates"], job["target_wires"], L)
+    V, v = job["gates"], rng.standard_normal(8 * job["G"])
+    req = {"target": tgt, "program": prog, "gates": V, "v": v, "what": mg.EVAL_F_GRAD_HVP}
+    say("case: L=14 program target, per-column batched kernel")
+    mg.batch_set_single_range(0, 0)
+    try:
+        ref = mg.evaluate_batch([dict(req, precision=mg.MG_FP64)])[0]
+    finally:
+        mg.batch_set_single_range(12, 30)
+    outs = []
+    for cb in (0, 1 << 20):
+        say(f"case: L=14 program target, materialised chunks of {cb or 'the default'} bytes")
+        mg.lib.mg_set_row_chunk_bytes(cb)
+        try:
+            outs.append([mg.evaluate_batch([dict(req, precision=pr)])[0] for pr in (mg.MG_FP64, mg.MG_FP32)])
+        finally:
+            mg.lib.mg_set_row_chunk_bytes(0)
+    for o64, o32 in outs:
+        assert abs(o64[0] - ref[0]) <= 1e-11 * abs(ref[0])
+        assert rel(o64[1], ref[1]) < 1e-11 and rel(o64[2], ref[2]) < 1e-11
+        assert abs(o32[0] - ref[0]) <= 2e-5 * abs(ref[0]) and rel(o32[1], ref[1]) < 2e-5 and rel(o32[2], ref[2]) < 2e-5
+    b0, b1 = 5000, 5012
+    Z = orc.real_to_directions(V, v)
+    fo, dg, dh = orc.target_gradient_hvp_program(job["target_gates"], job["target_wires"], V, Z, job["param"],
+                                                 job["wires"], L, b0, b1, nthreads=8)
+    f, g, h = mg.evaluate_batch([dict(req, precision=mg.MG_FP64, b0=b0, b1=b1)])[0]
+    assert abs(f - fo) <= 1e-11 * abs(fo)
+    assert rel(g, orc.riemannian_gradient(V, dg)) < 1e-11 and rel(h, orc.riemannian_hvp(V, dg, Z, dh)) < 1e-11
+
+
 def test_batch_routes_dense_L12_to_single_request_kernels(mg):
     """Dense-target L=12 requests run on the single-request kernels (FP64: register gate
     windows) inside mg_evaluate_batch: bitwise the single path's result; with the routing off
