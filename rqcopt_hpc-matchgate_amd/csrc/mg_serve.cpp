@@ -23,6 +23,7 @@
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
+#include <functional>
 #include <cstring>
 #include <mutex>
 #include <thread>
@@ -91,6 +92,7 @@ struct Server {
     bool stop_big = false;
     int running = 0, running_small = 0;
     std::chrono::steady_clock::time_point t0;
+    std::function<void()> on_eval;  // after each evaluation outside the batched rounds (progress)
     double time_limit_s = 0.0;  // > 0: evaluations that would start later fail (jobs end with status -2)
     bool past_limit() const
     {
@@ -125,6 +127,7 @@ int job_eval(JobRt* j, const zd* gates, const zd* Z, int mode, mg::EvalOut* out)
         j->srv->rounds++;
         j->srv->requests++;
         j->srv->kernel_ms += st.kernel_ms;
+        if (j->srv->on_eval) j->srv->on_eval();
         return rc;
     }
     Posted p;
@@ -530,6 +533,8 @@ int mg_serve(mg_job* jobs, int njobs, const mg_serve_options* opt, mg_serve_stat
         std::fflush(stderr);
     };
 
+    srv.on_eval = [&] { report(srv.rounds, srv.requests); };
+
     auto start_job = [&](int i) -> bool {
         auto j = std::make_unique<JobRt>();
         j->srv = &srv;
@@ -599,6 +604,7 @@ int mg_serve(mg_job* jobs, int njobs, const mg_serve_options* opt, mg_serve_stat
                     p->status = r;
                     p->done = true;
                 }
+                if (srv.on_eval) srv.on_eval();
                 srv.cv_jobs.notify_all();
             }
         });

@@ -22,6 +22,7 @@
 
 #include "mg_quad.hpp"
 #include "mg_sweep_tile.hpp"
+#include "mg_win.hpp"
 
 
 namespace mg {
@@ -32,9 +33,11 @@ namespace {
 template <typename R>
 constexpr int tile_kw() { return sizeof(R) == 8 ? 11 : 12; }
 
-// threads per workgroup (log2): 512 in both precisions (256 measured the same for FP64)
+// threads per workgroup (log2): FP64 512 (256 measured the same); FP32 1024, one quadruple per
+// thread and gate at 4 waves per SIMD with the coefficients in scalar registers (L=16 HVP 1.75 ->
+// 1.52 s against 512 threads with two quadruples per thread and vector coefficients)
 template <typename R>
-constexpr int tile_lt() { return 9; }
+constexpr int tile_lt() { return sizeof(R) == 4 ? 10 : 9; }
 
 __device__ inline uint32_t scatter_bits(uint32_t x, const int* pos, int n)
 {
@@ -133,11 +136,11 @@ __global__ __launch_bounds__(1 << tile_lt<R>()) void sweep_tile_kernel(TileArgs<
                 for (int g = g0; g < g1; g++) {
                     const Inst in = a.inst[g];
                     cpx<R> cf[24];
-                    load_coef<R, 8>(a.coef + (size_t)g * kCoefPerInst, cf);
-                    if constexpr (MODE == 2) load_coef<R, 8>(a.coef + (size_t)g * kCoefPerInst + 16, cf + 16);
+                    load_coef_c<R, 8>(a.coef + (size_t)g * kCoefPerInst, cf);
+                    if constexpr (MODE == 2) load_coef_c<R, 8>(a.coef + (size_t)g * kCoefPerInst + 16, cf + 16);
                     const uint32_t olo = swz_idx<R, true>(1u << in.lo), ohi = swz_idx<R, true>(1u << in.hi);
                     uint32_t q = tid;
-                    if constexpr (MODE == 2)
+                    if constexpr (MODE == 2 && 2 * T <= Qt)
                         for (; q + T < Qt; q += 2 * T)
                             fwd_quads<R, MODE, true, 2>(q, T, in.lo, in.hi, olo, ohi, cf, t_psi, t_psi, t_dpsi, t_dpsi);
                     for (; q < Qt; q += T)
@@ -203,7 +206,7 @@ __global__ __launch_bounds__(1 << tile_lt<R>()) void sweep_tile_kernel(TileArgs<
                     for (int g = g1 - 1; g >= g0; g--) {
                         const Inst in = a.inst[g];
                         cpx<R> cf[24];
-                        load_coef<R, MODE == 2 ? 24 : 16>(a.coef + (size_t)g * kCoefPerInst, cf);
+                        load_coef_c<R, MODE == 2 ? 24 : 16>(a.coef + (size_t)g * kCoefPerInst, cf);
                         cpx<R> ga[8], ha[8];
 #pragma unroll
                         for (int e = 0; e < 8; e++) {
@@ -212,8 +215,9 @@ __global__ __launch_bounds__(1 << tile_lt<R>()) void sweep_tile_kernel(TileArgs<
                         }
                         const uint32_t olo = swz_idx<R, true>(1u << in.lo), ohi = swz_idx<R, true>(1u << in.hi);
                         uint32_t q = tid;
-                        for (; q + T < Qt; q += 2 * T)
-                            rev_quads<R, MODE, true, 2>(q, T, in.lo, in.hi, olo, ohi, cf, t_lam, t_psi, t_dlam, t_dpsi, ga, ha);
+                        if constexpr (2 * T <= Qt)
+                            for (; q + T < Qt; q += 2 * T)
+                                rev_quads<R, MODE, true, 2>(q, T, in.lo, in.hi, olo, ohi, cf, t_lam, t_psi, t_dlam, t_dpsi, ga, ha);
                         if (q < Qt)
                             rev_quads<R, MODE, true, 1>(q, T, in.lo, in.hi, olo, ohi, cf, t_lam, t_psi, t_dlam, t_dpsi, ga, ha);
                         R v[NV];
