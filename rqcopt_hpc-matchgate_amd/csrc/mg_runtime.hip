@@ -75,6 +75,7 @@ struct Context {
     DevBuf inst, coef, partial, out, scratch, ckpt;
     DevBuf rows;  // materialised target rows of one column chunk (program targets beyond 12 qubits)
     DevBuf w_inst, w_coef, w_blk;  // gate-window form of the program (register window kernel)
+    DevBuf w_slot, w_bpar;         // tiled kernel: per-block accumulator slots and their parameters
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::vector<double> out_h;
     // batched evaluation: packed descriptor upload, partials, outputs, scratch, counter
@@ -672,10 +673,36 @@ static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, in
     const bool chunkable = t.kind == 1 && L > 12 && L - 12 <= 20 && kn.force_win != 0;
     size_t tile_off = 0, tile_lds = 0;
     bool use_tile = false;
+    // the tiled kernel's accumulator: one slot per parameter gate, or, when those do not fit next
+    // to the tile, one per parameter of the current tile block, added to the workgroup's partial
+    // sums after each block of each column
+    std::vector<int> tb_order, tb_slot, tb_param;
+    std::vector<TBlock> tb_blocks;
+    bool tile_slots = false;
     if (!use_win && !ckpt && !lds_states && (t.kind == 0 || chunkable) && L > tile_kw && kn.force_tile != 0 &&
         kn.force_lds == -1 && kn.force_reg != 1) {
         tile_lds = tile_sweep_lds_bytes(p.P, tile_kw, fp32, nstate_tile(mode), &tile_off);
         use_tile = tile_lds <= c->lds_limit;
+        if (!use_tile && mode >= 1) {
+            tile_blocks(b.inst, L, tile_kw, &tb_order, &tb_blocks);
+            if (tb_order.size() == b.inst.size()) {
+                tb_slot.resize(tb_order.size());
+                int most = 0;
+                for (TBlock& bk : tb_blocks) {
+                    bk.s0 = (int)tb_param.size();
+                    for (int k = bk.g0; k < bk.g1; k++) {
+                        const int par = b.inst[tb_order[k]].param;
+                        const auto it = std::find(tb_param.begin() + bk.s0, tb_param.end(), par);
+                        tb_slot[k] = (int)(it - (tb_param.begin() + bk.s0));
+                        if (it == tb_param.end()) tb_param.push_back(par);
+                    }
+                    bk.ns = (int)tb_param.size() - bk.s0;
+                    most = std::max(most, bk.ns);
+                }
+                tile_lds = tile_sweep_lds_bytes(most, tile_kw, fp32, nstate_tile(mode), &tile_off);
+                use_tile = tile_slots = tile_lds <= c->lds_limit;
+            }
+        }
     }
     const bool chunk_rows = chunkable && use_tile;
     if (use_tile) reg_T = 0;
@@ -848,7 +875,11 @@ static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, in
     if (use_tile && !use_wtile) {
         std::vector<int> order;
         std::vector<TBlock> blocks;
-        tile_blocks(b.inst, L, tile_kw, &order, &blocks);
+        if (tile_slots) {
+            order = tb_order;
+            blocks = tb_blocks;
+        } else
+            tile_blocks(b.inst, L, tile_kw, &order, &blocks);
         if (order.size() != b.inst.size()) {
             set_error("tiled sweep: gate blocking failed");
             return -1;
@@ -880,6 +911,19 @@ static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, in
         MG_HIP(hipMemcpyAsync(d_wi, ti.data(), ti.size() * sizeof(Inst), hipMemcpyHostToDevice, s));
         MG_HIP(hipMemcpyAsync(d_wc, tc.data(), tc.size() * sizeof(cpx<R>), hipMemcpyHostToDevice, s));
         MG_HIP(hipMemcpyAsync(d_wb, blocks.data(), blocks.size() * sizeof(TBlock), hipMemcpyHostToDevice, s));
+        if (tile_slots) {
+            void *d_sl = nullptr, *d_bp = nullptr;
+            if (ar.get(c->w_slot, tb_slot.size() * sizeof(int), &d_sl) < 0 ||
+                ar.get(c->w_bpar, tb_param.size() * sizeof(int), &d_bp) < 0)
+                return -1;
+            MG_HIP(hipMemcpyAsync(d_sl, tb_slot.data(), tb_slot.size() * sizeof(int), hipMemcpyHostToDevice, s));
+            MG_HIP(hipMemcpyAsync(d_bp, tb_param.data(), tb_param.size() * sizeof(int), hipMemcpyHostToDevice, s));
+            ta.slot = (const int*)d_sl;
+            ta.bparam = (const int*)d_bp;
+            int most = 0;
+            for (const TBlock& bk : blocks) most = std::max(most, bk.ns);
+            ta.nacc = most;
+        }
         ta.L = L;
         ta.G = p.G;
         ta.P = p.P;

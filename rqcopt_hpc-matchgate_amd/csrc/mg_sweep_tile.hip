@@ -78,8 +78,9 @@ __global__ __launch_bounds__(1 << tile_lt<R>()) void sweep_tile_kernel(TileArgs<
     const uint32_t ntiles = 1u << (L - KW);
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
 
-    double* acc = reinterpret_cast<double*>(smem);  // [P][32]
-    double* stage = acc + 32 * a.P;                 // [2][NW][NV]
+    const int nacc = a.slot ? a.nacc : a.P;
+    double* acc = reinterpret_cast<double*>(smem);  // [nacc][32]
+    double* stage = acc + 32 * nacc;                // [2][NW][NV]
     double* fst = stage + 2 * NW * NV;              // [NW][2]
     cpx<R>* tl = reinterpret_cast<cpx<R>*>(smem + a.tile_off);
     // tile slots and global slices: psi, dpsi (MODE 2), lam (MODE >= 1), dlam (MODE 2)
@@ -94,7 +95,10 @@ __global__ __launch_bounds__(1 << tile_lt<R>()) void sweep_tile_kernel(TileArgs<
     cpx<R>* g_lam = sc + (size_t)LAM * S;
     cpx<R>* g_dlam = sc + 3 * (size_t)S;
 
-    for (int k = tid; k < 32 * a.P; k += T) acc[k] = 0.0;
+    double* out = a.partial + (size_t)blockIdx.x * (2 + 32 * a.P);
+    for (int k = tid; k < 32 * nacc; k += T) acc[k] = 0.0;
+    if (a.slot)
+        for (int k = tid; k < 32 * a.P; k += T) out[2 + k] = 0.0;
     double fre = 0.0, fim = 0.0;
     __syncthreads();
 
@@ -240,7 +244,7 @@ __global__ __launch_bounds__(1 << tile_lt<R>()) void sweep_tile_kernel(TileArgs<
                             const int ci = lane >> 1;
                             int e = ci & 7;
                             if (in.flip && e < 4) e = 3 - e;
-                            acc[32 * in.param + 2 * ((ci & 8) | e) + (lane & 1)] += s;
+                            acc[32 * (a.slot ? a.slot[g] : in.param) + 2 * ((ci & 8) | e) + (lane & 1)] += s;
                         }
                     }
                     if (k > 0) {
@@ -258,17 +262,26 @@ __global__ __launch_bounds__(1 << tile_lt<R>()) void sweep_tile_kernel(TileArgs<
                 }
                 // the next block reads, in its own mapping, what other threads stored here
                 __syncthreads();
+                if (a.slot) {
+                    // this block's sums of this column into the workgroup's partial sums
+                    const int* bp = a.bparam + bk->s0;
+                    for (int k2 = tid; k2 < 32 * bk->ns; k2 += T) {
+                        out[2 + 32 * bp[k2 >> 5] + (k2 & 31)] += acc[k2];
+                        acc[k2] = 0.0;
+                    }
+                    __syncthreads();
+                }
             }
         }
         __syncthreads();
     }
     __syncthreads();
-    double* out = a.partial + (size_t)blockIdx.x * (2 + 32 * a.P);
     if (tid == 0) {
         out[0] = fre;
         out[1] = fim;
     }
-    for (int k = tid; k < 32 * a.P; k += T) out[2 + k] = acc[k];
+    if (!a.slot)
+        for (int k = tid; k < 32 * a.P; k += T) out[2 + k] = acc[k];
 }
 
 int tile_window_bits(int fp32) { return fp32 ? tile_kw<float>() : tile_kw<double>(); }

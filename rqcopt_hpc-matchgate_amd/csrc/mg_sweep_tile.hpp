@@ -16,6 +16,7 @@ struct TBlock {
     int g0, g1;
     int pos[kTileMaxBits];
     int other[32 - kTileMaxBits];
+    int s0, ns;  // per-block accumulator slots: the block's parameters are bparam[s0, s0 + ns)
 };
 
 template <typename R>
@@ -33,6 +34,12 @@ struct TileArgs {
     cpx<R>* scratch;        // [gridDim.x][nstate][S]: psi, dpsi, lam, dlam (as the mode needs)
     int nstate;             // states per column: 1 (f), 2 (f + gradient), 4 (+ HVP)
     int tile_off;           // byte offset of the tile states in dynamic LDS
+    // accumulator: null slot = one [32] slot per parameter gate (P slots), kept over the columns;
+    // else slot[g] is instance g's slot among its block's parameters (bparam), nacc slots, added
+    // to the workgroup's partial sums after each block of each column
+    const int* slot;
+    const int* bparam;
+    int nacc;
 };
 
 // Regroups the instances (program order; bit positions hi > lo) into blocks whose gates act

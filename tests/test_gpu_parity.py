@@ -270,13 +270,17 @@ def test_lds_kernel_f_and_gradient_two_workgroups_per_cu(mg, depth):
 
 @pytest.mark.parametrize("L,layout,depth,prec", [(13, "open", 11, "fp64"), (13, "tied", 4, "fp32"), (14, "open", 7, "fp64"),
                                                   (13, "open", 15, "fp32"), (13, "open", 15, "fp64"),
-                                                  (14, "open", 5, "fp32"), (12, "open", 5, "fp64")])
+                                                  (14, "open", 5, "fp32"), (12, "open", 5, "fp64"),
+                                                  (13, "open", 17, "fp32"), (13, "open", 21, "fp64")])
 def test_tile_kernels_match_oracle(mg, L, layout, depth, prec):
     """The tiled kernels for dense targets whose states exceed the LDS, all three modes on a
     column subset against the oracle: FP64 unitary gates run the tiled register-window kernel
     (12-qubit tiles swept in 3-qubit register windows; one or two states per re-layout round)
     and, with it switched off, the LDS-tile kernel; FP32 and the FP64 gates that are only
-    invertible (L=12, unitarity defect 2e-11) run the LDS-tile kernel (2^12 / 2^11 amplitudes)."""
+    invertible (L=12, unitarity defect 2e-11) run the LDS-tile kernel (2^12 / 2^11 amplitudes).
+    Depth 17 FP32 (102 parameter gates) and depth 21 FP64 (126) exceed what the LDS-tile
+    kernel's per-parameter accumulator holds next to the tile: it accumulates per tile block
+    and adds each block's sums to the partial sums in memory."""
     say(f"case: tiled kernels L={L} {layout} depth={depth} {prec}")
     V, Z, param, wires, P, U = build_case(L, layout, depth, seed=L * 7 + depth)
     if L == 12:
