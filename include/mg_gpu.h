@@ -261,6 +261,10 @@ void mg_set_tile_kernel(int mode);
  * dense rows one column chunk at a time next to the tiled sweeps: bytes of rows per chunk (at least
  * one grid of columns; 0 restores the default 4 GiB) */
 void mg_set_row_chunk_bytes(long bytes);
+/* rows of program targets beyond 12 qubits (mg_target_densify and the column chunks above): mode -1
+ * automatic (default: the matrix cores, FP64 MFMA on 4-qubit windows), 0 the 12-qubit register
+ * tiles of the tiled window kernel, 1 the matrix cores */
+void mg_set_materialize_kernel(int mode);
 /* record HIP events around every sweep kernel on the library's stream */
 void mg_set_kernel_timing(int on);
 /* duration of the last sweep kernel in ms (needs mg_set_kernel_timing(1)) */

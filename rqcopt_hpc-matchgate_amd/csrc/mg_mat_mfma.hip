@@ -1,0 +1,246 @@
+// Target materialisation on the matrix cores: rows[b] = -conj(W e_b) for a target program W
+// beyond 12 qubits.
+//
+// The program is cut into 12-qubit tile blocks (tile_blocks, as the tiled sweeps) and, inside
+// each, into 4-qubit windows (window_blocks).  Every gate is a matchgate: it keeps the parity of
+// its two bits, so a window's 16 x 16 product keeps the parity of the window's four bits and is
+// two 8 x 8 blocks.  In real arithmetic an 8 x 8 complex block is one 16 x 16 matrix, applied to
+// 16 columns (the amplitudes sharing the other 8 bits of the tile) by four
+// v_mfma_f64_16x16x4f64: 8 complex MACs per amplitude and window where the 2 x 2 blocks of its
+// ~3.6 gates take ~7.2, at the FP64 matrix rate.
+//
+// Per row: per tile block, per tile (2^(L-12) of them): load the tile of the row (the basis
+// vector e_b in the first block) into LDS, apply the block's windows (one barrier each; every
+// wave owns 4 of the tile's 16 column blocks, so a window updates its entries in place), store
+// the tile back (with -conj in the last block).  Reference: the target map the reference calls
+// per column, src/matchgate_target.c:795, for a brickwall target
+// examples/hubbard1d/hubbard1d_opt.c:30-39 over src/matchgate_brickwall.c:14-43.
+#include <algorithm>
+#include <complex>
+
+#include "mg_mat_mfma.hpp"
+#include "mg_sweep_tile.hpp"
+#include "mg_window.hpp"
+
+namespace mg {
+
+namespace {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+// LDS index of tile entry j: the bits above the lowest four folded onto them, so that the
+// columns of a block (any 4 of the tile's bits) fall in different banks; linear over XOR
+__host__ __device__ inline uint32_t mat_swz(uint32_t j) { return j ^ ((j >> 4) & 15u) ^ ((j >> 8) & 15u); }
+
+__device__ inline uint32_t scatter(uint32_t x, const int* pos, int n)
+{
+    uint32_t r = 0;
+    for (int i = 0; i < n; i++) r |= ((x >> i) & 1u) << pos[i];
+    return r;
+}
+
+}  // namespace
+
+__global__ __launch_bounds__(256) void materialize_mfma_kernel(MatArgs a)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    cpx<double>* tl = reinterpret_cast<cpx<double>*>(smem);  // [4096], index mat_swz(j)
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int L = a.L;
+    const uint32_t S = 1u << L, ntiles = 1u << (L - 12);
+    const int n0 = lane >> 4, col = lane & 15;
+    cpx<double>* rows = reinterpret_cast<cpx<double>*>(a.rows);
+
+    for (long b = a.b0 + blockIdx.x; b < a.b1; b += gridDim.x) {
+        cpx<double>* row = rows + (size_t)(b - a.row0) * S;
+        for (int k = 0; k < a.ntb; k++) {
+            const WTBlock* tb = a.tblocks + k;
+            const uint32_t lo = scatter((uint32_t)tid, tb->tpos, 8);
+            const bool last = k == a.ntb - 1;
+            for (uint32_t t = 0; t < ntiles; t++) {
+                const uint32_t ot = scatter(t, tb->other, L - 12);
+#pragma unroll 4
+                for (uint32_t m = 0; m < 16; m++) {
+                    const uint32_t gi = ot | lo | scatter(m, tb->tpos + 8, 4);
+                    const uint32_t j = (uint32_t)tid | (m << 8);
+                    tl[mat_swz(j)] = k == 0 ? cpx<double>{gi == (uint32_t)b ? 1.0 : 0.0, 0.0} : row[gi];
+                }
+                __syncthreads();
+                for (int w = tb->k0; w < tb->k1; w++) {
+                    const DWin* dw = a.wins + w;
+                    double af[8];
+#pragma unroll
+                    for (int f = 0; f < 8; f++) af[f] = a.afrag[((size_t)w * 8 + f) * 64 + lane];
+                    const uint32_t oj = (uint32_t)dw->offj[col];
+                    // the wave's 8 products (4 column blocks x 2 parities): every read first, so
+                    // that the in-place writes follow them, then 8 independent MFMA chains
+                    uint32_t base[4];
+                    cpx<double> x[8][2];
+#pragma unroll
+                    for (int c4 = 0; c4 < 4; c4++) {
+                        base[c4] = oj ^ (uint32_t)dw->offcb[wv * 4 + c4];
+#pragma unroll
+                        for (int p = 0; p < 2; p++) {
+                            x[2 * c4 + p][0] = tl[base[c4] ^ (uint32_t)dw->offp[p][n0]];
+                            x[2 * c4 + p][1] = tl[base[c4] ^ (uint32_t)dw->offp[p][4 + n0]];
+                        }
+                    }
+                    v4d acc[8];
+#pragma unroll
+                    for (int c = 0; c < 8; c++) acc[c] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                    for (int kk = 0; kk < 4; kk++)
+#pragma unroll
+                        for (int c = 0; c < 8; c++) {
+                            const cpx<double>& xv = x[c][kk >> 1];
+                            acc[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[4 * (c & 1) + kk], (kk & 1) ? xv.im : xv.re,
+                                                                          acc[c], 0, 0, 0);
+                        }
+                    // this lane's results: entries 2 n0 and 2 n0 + 1 of its column
+#pragma unroll
+                    for (int c = 0; c < 8; c++) {
+                        const int p = c & 1;
+                        tl[base[c >> 1] ^ (uint32_t)dw->offp[p][2 * n0]] = {acc[c][0], acc[c][1]};
+                        tl[base[c >> 1] ^ (uint32_t)dw->offp[p][2 * n0 + 1]] = {acc[c][2], acc[c][3]};
+                    }
+                    __syncthreads();
+                }
+#pragma unroll 4
+                for (uint32_t m = 0; m < 16; m++) {
+                    const uint32_t gi = ot | lo | scatter(m, tb->tpos + 8, 4);
+                    const cpx<double> v = tl[mat_swz((uint32_t)tid | (m << 8))];
+                    row[gi] = last ? cpx<double>{-v.re, v.im} : v;
+                }
+                // the next tile's load reuses the LDS tile
+                __syncthreads();
+            }
+        }
+    }
+}
+
+hipError_t launch_materialize_mfma(const MatArgs& a, int grid, hipStream_t s)
+{
+    if (a.L <= 12 || a.L - 12 > 20) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(materialize_mfma_kernel, dim3(grid), dim3(256), kMatMfmaLds, s, a);
+    return hipGetLastError();
+}
+
+// ---- host: the window form ----------------------------------------------------------------
+
+namespace {
+
+// window indices of each parity, ascending
+constexpr int kParIdx[2][8] = {{0, 3, 5, 6, 9, 10, 12, 15}, {1, 2, 4, 7, 8, 11, 13, 14}};
+
+using zc = std::complex<double>;
+
+// M <- G M for the gate with coefficients g (V: center block g0..g3 on {lo set, hi set}, corner
+// g4..g7 on {none, both}) on window-local bits la (its lower bit) and lb
+void apply_gate_left(zc* M, const cpx<double>* g, int la, int lb)
+{
+    const zc v[8] = {{g[0].re, g[0].im}, {g[1].re, g[1].im}, {g[2].re, g[2].im}, {g[3].re, g[3].im},
+                     {g[4].re, g[4].im}, {g[5].re, g[5].im}, {g[6].re, g[6].im}, {g[7].re, g[7].im}};
+    for (int r = 0; r < 16; r++) {
+        if (r & ((1 << la) | (1 << lb))) continue;
+        const int i00 = r, i01 = r | (1 << la), i10 = r | (1 << lb), i11 = r | (1 << la) | (1 << lb);
+        for (int c = 0; c < 16; c++) {
+            const zc x = M[i00 * 16 + c], y = M[i01 * 16 + c], z = M[i10 * 16 + c], w = M[i11 * 16 + c];
+            M[i00 * 16 + c] = v[4] * x + v[5] * w;
+            M[i01 * 16 + c] = v[0] * y + v[1] * z;
+            M[i10 * 16 + c] = v[2] * y + v[3] * z;
+            M[i11 * 16 + c] = v[6] * x + v[7] * w;
+        }
+    }
+}
+
+}  // namespace
+
+bool make_dense_windows(const std::vector<Inst>& inst, const std::vector<cpx<double>>& coef, int L, DenseWindows* d)
+{
+    constexpr int TB = 12, W = 4;
+    d->tblocks.clear();
+    d->wins.clear();
+    d->afrag.clear();
+    d->ngates = (int)inst.size();
+    if (L <= TB || L - TB > 20) return false;
+    std::vector<int> order1;
+    std::vector<TBlock> tbs;
+    tile_blocks(inst, L, TB, &order1, &tbs);
+    if (order1.size() != inst.size()) return false;
+    for (const TBlock& tb : tbs) {
+        auto local = [&](int q) { return (int)(std::find(tb.pos, tb.pos + TB, q) - tb.pos); };
+        std::vector<Inst> li;
+        for (int k = tb.g0; k < tb.g1; k++) {
+            Inst in = inst[order1[k]];
+            in.lo = local(in.lo);
+            in.hi = local(in.hi);
+            if (in.lo >= TB || in.hi >= TB) return false;
+            li.push_back(in);
+        }
+        std::vector<int> order2;
+        std::vector<WBlock> wbs;
+        window_blocks(li, TB, W, 1 << 20, &order2, &wbs);
+        if (order2.size() != li.size()) return false;
+        WTBlock t{};
+        t.k0 = (int)d->wins.size();
+        for (int i = 0; i < TB; i++) t.tpos[i] = tb.pos[i];
+        for (int i = 0; i < L - TB; i++) t.other[i] = tb.other[i];
+        for (const WBlock& wb : wbs) {
+            // the window's dense matrix, gates in program order
+            zc M[256];
+            for (int i = 0; i < 256; i++) M[i] = (i / 16 == i % 16) ? 1.0 : 0.0;
+            for (int k = wb.g0; k < wb.g1; k++) {
+                const Inst& l = li[order2[k]];
+                int la = -1, lb = -1;
+                for (int i = 0; i < W; i++) {
+                    if (wb.pos[i] == l.lo) la = i;
+                    if (wb.pos[i] == l.hi) lb = i;
+                }
+                if (la < 0 || lb < 0) return false;
+                apply_gate_left(M, &coef[(size_t)order1[tb.g0 + order2[k]] * kCoefPerInst], la, lb);
+            }
+            // entries across parities must vanish: the gates are matchgates
+            for (int r = 0; r < 16; r++)
+                for (int c = 0; c < 16; c++)
+                    if ((__builtin_popcount(r) ^ __builtin_popcount(c)) & 1)
+                        if (M[r * 16 + c] != 0.0) return false;
+            // LDS offsets: window bit i is tile bit pos[i]; the other 8 tile bits ascending
+            DWin dw{};
+            int oth[8], no = 0;
+            for (int q = 0; q < TB; q++)
+                if (std::find(wb.pos, wb.pos + W, q) == wb.pos + W) oth[no++] = q;
+            auto put = [&](uint32_t x, const int* pos, int n) {
+                uint32_t r = 0;
+                for (int i = 0; i < n; i++) r |= ((x >> i) & 1u) << pos[i];
+                return (int)mat_swz(r);
+            };
+            for (int p = 0; p < 2; p++)
+                for (int n = 0; n < 8; n++) dw.offp[p][n] = put((uint32_t)kParIdx[p][n], wb.pos, W);
+            for (int j = 0; j < 16; j++) {
+                dw.offj[j] = put((uint32_t)j, oth, 4);
+                dw.offcb[j] = put((uint32_t)j, oth + 4, 4);
+            }
+            d->wins.push_back(dw);
+            // A fragments: lane l holds row l % 16, column 4 kk + l / 16 of block p's real matrix
+            for (int p = 0; p < 2; p++)
+                for (int kk = 0; kk < 4; kk++)
+                    for (int ln = 0; ln < 64; ln++) {
+                        const int r = ln % 16, c = 4 * kk + ln / 16;
+                        // result v of lane l is row 4 v + l / 16: rows g, g + 4 (g + 8, g + 12) are
+                        // the real and imaginary parts of entry 2 g (2 g + 1)
+                        const int m = 2 * (r % 4) + (r / 4) / 2, part = (r / 4) % 2;  // output entry, re/im
+                        const int n = (c / 8) * 4 + c % 4, q = (c / 4) % 2;     // input entry, re/im
+                        const zc e = M[kParIdx[p][m] * 16 + kParIdx[p][n]];
+                        double v;
+                        if (part == 0) v = q == 0 ? e.real() : -e.imag();
+                        else v = q == 0 ? e.imag() : e.real();
+                        d->afrag.push_back(v);
+                    }
+        }
+        t.k1 = (int)d->wins.size();
+        d->tblocks.push_back(t);
+    }
+    return true;
+}
+
+}  // namespace mg

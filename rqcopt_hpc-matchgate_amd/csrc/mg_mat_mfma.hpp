@@ -1,0 +1,50 @@
+// Target materialisation on the matrix cores (mg_mat_mfma.hip).
+#pragma once
+#include <vector>
+
+#include "mg_device.hpp"
+#include "mg_sweep_win.hpp"
+
+namespace mg {
+
+// One 4-qubit window of a tile block.  A product of matchgates keeps the parity of the window's
+// bits, so the window's 16 x 16 matrix is two 8 x 8 blocks, one per parity; each block, written
+// as a real 16 x 16 matrix over (real, imaginary) parts, is one FP64 MFMA tile.  Offsets are
+// indices into the LDS tile (12 bits), already swizzled (the swizzle is linear over XOR, so the
+// index of window entry n of column j of column block cb is offp ^ offj ^ offcb).
+struct DWin {
+    int offp[2][8];  // parity block p, parity index n (window indices ascending within a parity)
+    int offj[16];    // column j of a column block (the 4 lowest of the 8 bits outside the window)
+    int offcb[16];   // column block cb (the 4 highest)
+};
+
+// the target program in that form: tile blocks (WTBlock: window range [k0, k1), the tile's 12
+// positions and the other L - 12), windows, and the MFMA A-operand fragments of every window,
+// [window][4 p + kk][lane] doubles (row lane % 16, column 4 kk + lane / 16 of parity block p's
+// real matrix; columns 0-3: real parts of entries 0-3, 4-7 their imaginary parts, 8-15 the same
+// for entries 4-7; rows g, g + 4, g + 8, g + 12 (g < 4): real and imaginary parts of entries 2 g and
+// 2 g + 1, which the FP64 MFMA returns as results 0-3 of lanes 16 g .. 16 g + 15)
+struct DenseWindows {
+    std::vector<WTBlock> tblocks;
+    std::vector<DWin> wins;
+    std::vector<double> afrag;
+    int ngates = 0;
+};
+
+// fuses the target's instances (program order, global bit positions, coefficients V first) into
+// 4-qubit windows inside 12-qubit tile blocks; false when the blocking fails (L <= 12)
+bool make_dense_windows(const std::vector<Inst>& inst, const std::vector<cpx<double>>& coef, int L, DenseWindows* d);
+
+struct MatArgs {
+    int L, ntb;
+    const WTBlock* tblocks;
+    const DWin* wins;
+    const double* afrag;
+    long b0, b1;   // rows [b0, b1)
+    void* rows;    // cpx<double>: row b at rows + (b - row0) S, -conj(W e_b)
+    long row0;
+};
+constexpr size_t kMatMfmaLds = ((size_t)1 << 12) * sizeof(cpx<double>);  // one 12-qubit FP64 tile
+hipError_t launch_materialize_mfma(const MatArgs& a, int grid, hipStream_t s);
+
+}  // namespace mg

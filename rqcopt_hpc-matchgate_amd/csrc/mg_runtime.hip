@@ -11,6 +11,7 @@
 #include <memory>
 
 #include "mg_batch.hpp"
+#include "mg_mat_mfma.hpp"
 #include "mg_sweep.hpp"
 #include "mg_sweep_tile.hpp"
 #include "mg_sweep_win.hpp"
@@ -556,6 +557,55 @@ static int tile_form_grid(const Context* c, const Target& t, long ncols)
     return (int)std::min<long>(ncols, (long)c->ncu * per_cu);
 }
 
+// the target program's 4-qubit window form for the matrix-core materialisation, uploaded once
+static int prepare_mat_form(Target& t)
+{
+    Target::MatForm& f = t.mf;
+    if (f.ready) return 0;
+    DenseWindows d;
+    if (!make_dense_windows(t.tinst_h, t.tcoef_h, t.L, &d)) {
+        set_error("target materialisation: 4-qubit window blocking failed");
+        return -1;
+    }
+    MG_HIP(f.tb.ensure(d.tblocks.size() * sizeof(WTBlock)));
+    MG_HIP(f.win.ensure(d.wins.size() * sizeof(DWin)));
+    MG_HIP(f.af.ensure(d.afrag.size() * sizeof(double)));
+    MG_HIP(hipMemcpy(f.tb.p, d.tblocks.data(), d.tblocks.size() * sizeof(WTBlock), hipMemcpyHostToDevice));
+    MG_HIP(hipMemcpy(f.win.p, d.wins.data(), d.wins.size() * sizeof(DWin), hipMemcpyHostToDevice));
+    MG_HIP(hipMemcpy(f.af.p, d.afrag.data(), d.afrag.size() * sizeof(double), hipMemcpyHostToDevice));
+    f.ntb = (int)d.tblocks.size();
+    f.ready = true;
+    return 0;
+}
+
+// rows [b0, b1) of a target program beyond 12 qubits, -conj(W e_b), into `rows` (row b at
+// rows + (b - row0) S): on the matrix cores (4-qubit windows, FP64 MFMA) or in 12-qubit register
+// tiles (the tiled window kernel's forward sweep)
+static int launch_target_rows(Context* c, Target& t, long b0, long b1, void* rows, long row0, hipStream_t s)
+{
+    if (knobs().mat_kernel != 0) {
+        if (prepare_mat_form(t) < 0) return -1;
+        MatArgs m{};
+        m.L = t.L;
+        m.ntb = t.mf.ntb;
+        m.tblocks = (const WTBlock*)t.mf.tb.p;
+        m.wins = (const DWin*)t.mf.win.p;
+        m.afrag = (const double*)t.mf.af.p;
+        m.b0 = b0;
+        m.b1 = b1;
+        m.rows = rows;
+        m.row0 = row0;
+        const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(2, c->lds_limit / kMatMfmaLds));
+        const int grid = (int)std::min<long>(b1 - b0, (long)c->ncu * per_cu);
+        MG_HIP(launch_materialize_mfma(m, grid, s));
+        return 0;
+    }
+    if (prepare_tile_form(c, t) < 0) return -1;
+    const WinTileArgs a = tile_form_args(t, b0, b1, rows, row0);
+    MG_HIP(launch_materialize_win_tile(a, tile_form_grid(c, t, b1 - b0), t.tf.lds, s));
+    return 0;
+}
+
 // ---- evaluation -------------------------------------------------------------------------------
 // Device buffers of one evaluation: the context's grow-only buffers, or (allocation-free
 // evaluation) consecutive 256-byte aligned pieces of the static scratch array.
@@ -760,7 +810,7 @@ static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, in
         return -1;
     if (chunk_rows) {
         if (ar.get(c->rows, (size_t)chunk * S * sizeof(cpx<double>), &d_rows) < 0) return -1;
-        if (prepare_tile_form(c, t) < 0) return -1;
+        if ((kn.mat_kernel != 0 ? prepare_mat_form(t) : prepare_tile_form(c, t)) < 0) return -1;
     }
     if (use_tile) {
         if (ar.get(c->scratch, (size_t)grid * tile_state_bytes, &d_scratch) < 0) return -1;
@@ -947,8 +997,7 @@ static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, in
     if (chunk_rows) {
         for (long ci = 0; ci < nchunks; ci++) {
             const long c0 = b0 + ci * chunk, c1 = std::min<long>(b1, c0 + chunk);
-            const WinTileArgs m = tile_form_args(t, c0, c1, d_rows, c0);
-            MG_HIP(launch_materialize_win_tile(m, tile_form_grid(c, t, c1 - c0), t.tf.lds, s));
+            if (launch_target_rows(c, t, c0, c1, d_rows, c0, s) < 0) return -1;
             double* part = (double*)d_partial + (size_t)ci * grid * stride;
             if (use_wtile) {
                 WinTileArgs w2 = wta;
@@ -1157,10 +1206,7 @@ int target_densify(Target& t)
     // beyond the LDS: 12-qubit tiles of each row in registers (the tiled window kernel's
     // forward sweep), unless the knob asks for the windowed in-place rows
     if (!in_lds && L > 12 && L - 12 <= 20 && knobs().force_win != 0) {
-        if (prepare_tile_form(c, t) < 0) return -1;
-        const WinTileArgs a = tile_form_args(t, 0, (long)S, t.seed64.p, 0);
-        const int grid = tile_form_grid(c, t, (long)S);
-        MG_HIP(launch_materialize_win_tile(a, grid, t.tf.lds, c->s));
+        if (launch_target_rows(c, t, 0, (long)S, t.seed64.p, 0, c->s) < 0) return -1;
         MG_HIP(hipStreamSynchronize(c->s));
         t.kind = 0;
         t.seed32_ready = false;

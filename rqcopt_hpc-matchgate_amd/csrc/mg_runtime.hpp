@@ -66,6 +66,13 @@ struct Target {
         bool ready = false;
     };
     TileForm tf;
+    // the target program in 4-qubit windows for the matrix-core materialisation (L > 12)
+    struct MatForm {
+        DevBuf tb, win, af;
+        int ntb = 0;
+        bool ready = false;
+    };
+    MatForm mf;
 };
 
 // U row-major (S x S)
@@ -117,6 +124,7 @@ struct Knobs {
     int force_tile = -1;      // -1 auto (tiled kernel for dense targets whose states exceed the LDS), 0 never
     bool time_kernels = false;
     size_t row_chunk_bytes = 0;  // materialised target rows per column chunk (0: 4 GiB)
+    int mat_kernel = -1;         // target rows beyond 12 qubits: -1 auto, 0 register tiles, 1 matrix cores
 };
 Knobs& knobs();
 

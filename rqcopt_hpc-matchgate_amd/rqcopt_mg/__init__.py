@@ -127,6 +127,7 @@ _sig("mg_last_sweep_kernel", [], ctypes.c_char_p)
 _sig("mg_set_window_kernel", [_I], None)
 _sig("mg_set_tile_kernel", [_I], None)
 _sig("mg_set_row_chunk_bytes", [_L], None)
+_sig("mg_set_materialize_kernel", [_I], None)
 _sig("mg_last_kernel_ms", [], ctypes.c_float)
 _sig("mg_last_sweep_info", [_P, _P, _P, _P], None)
 _sig("mg_target_cache_clear", [], None)
@@ -534,6 +535,12 @@ def set_window_kernel(mode):
 def set_tile_kernel(mode):
     """-1: tiled kernel where it applies (default), 0: never"""
     lib.mg_set_tile_kernel(int(mode))
+
+
+def set_materialize_kernel(mode):
+    """rows of program targets beyond 12 qubits: -1 automatic (default, matrix cores), 0 register
+    tiles, 1 matrix cores"""
+    lib.mg_set_materialize_kernel(int(mode))
 
 
 def last_sweep_kernel():

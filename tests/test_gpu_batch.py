@@ -154,28 +154,33 @@ def test_densified_target_matches_program(mg):
 
 @pytest.mark.parametrize("L", [13, 14])
 def test_densify_beyond_lds_matches_circuit_target(mg, L):
-    """Targets of L > 12 qubits are materialised in 12-qubit register tiles (each row its own
-    column's state between tile blocks); the alternative path (3-qubit windows applied in place
-    in the rows) must give the same rows.  Checked through f, gradient and HVP over all 2^L
-    columns (every row enters), FP64, against the circuit target applied per column."""
+    """Targets of L > 12 qubits are materialised on the matrix cores (4-qubit windows as two
+    8 x 8 parity blocks, FP64 MFMA, in 12-qubit LDS tiles) by default; the alternatives (12-qubit
+    register tiles, each row its own column's state between tile blocks; 3-qubit windows applied
+    in place in the rows) must give the same rows.  Checked through f, gradient and HVP over all
+    2^L columns (every row enters), FP64, against the circuit target applied per column."""
     rng = np.random.default_rng(40 + L)
     job, prog, tgt, v = make_instance(mg, rng, L, 3)
-    say(f"case: densify L={L} ({job['Gt']} target gates): tiles vs in-place windows vs circuit target")
+    say(f"case: densify L={L} ({job['Gt']} target gates): matrix cores vs register tiles vs in-place windows "
+        "vs circuit target")
     ref = mg.riemannian_gradient_hvp(tgt, prog, job["gates"], v, mg.MG_FP64)
     outs = []
     try:
-        for knob in (-1, 0):
-            mg.set_window_kernel(knob)
+        for win, mat in ((-1, -1), (-1, 0), (0, -1)):
+            mg.set_window_kernel(win)
+            mg.set_materialize_kernel(mat)
             dense = mg.Target.circuit(job["target_gates"], job["target_wires"], L).densify()
             mg.set_window_kernel(-1)
+            mg.set_materialize_kernel(-1)
             outs.append(mg.riemannian_gradient_hvp(dense, prog, job["gates"], v, mg.MG_FP64))
             del dense
     finally:
         mg.set_window_kernel(-1)
+        mg.set_materialize_kernel(-1)
     for out in outs:
         assert abs(out[0] - ref[0]) <= 1e-12 * abs(ref[0])
         assert rel(out[1], ref[1]) < 1e-12 and rel(out[2], ref[2]) < 1e-12
-    assert abs(outs[0][0] - outs[1][0]) <= 1e-13 * abs(ref[0])
+    assert abs(outs[0][0] - outs[1][0]) <= 1e-13 * abs(ref[0]) and abs(outs[0][0] - outs[2][0]) <= 1e-13 * abs(ref[0])
 
 
 def _jobs(mg, seed, spec):
