@@ -733,7 +733,10 @@ static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, in
         kn.force_lds == -1 && kn.force_reg != 1) {
         tile_lds = tile_sweep_lds_bytes(p.P, tile_kw, fp32, nstate_tile(mode), &tile_off);
         use_tile = tile_lds <= c->lds_limit;
-        if (!use_tile && mode >= 1) {
+        // f + gradient (two states, 64 KB): per-block slots also when they let two workgroups
+        // share a CU and the full accumulator does not
+        const bool want_two = mode == 1 && 2 * tile_lds > c->lds_limit;
+        if ((!use_tile || want_two) && mode >= 1) {
             tile_blocks(b.inst, L, tile_kw, &tb_order, &tb_blocks);
             if (tb_order.size() == b.inst.size()) {
                 tb_slot.resize(tb_order.size());
@@ -749,8 +752,13 @@ static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, in
                     bk.ns = (int)tb_param.size() - bk.s0;
                     most = std::max(most, bk.ns);
                 }
-                tile_lds = tile_sweep_lds_bytes(most, tile_kw, fp32, nstate_tile(mode), &tile_off);
-                use_tile = tile_slots = tile_lds <= c->lds_limit;
+                size_t slot_off = 0;
+                const size_t slot_lds = tile_sweep_lds_bytes(most, tile_kw, fp32, nstate_tile(mode), &slot_off);
+                if (slot_lds <= c->lds_limit && (!use_tile || 2 * slot_lds <= c->lds_limit)) {
+                    use_tile = tile_slots = true;
+                    tile_lds = slot_lds;
+                    tile_off = slot_off;
+                }
             }
         }
     }

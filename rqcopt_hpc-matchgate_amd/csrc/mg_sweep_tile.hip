@@ -33,11 +33,16 @@ namespace {
 template <typename R>
 constexpr int tile_kw() { return sizeof(R) == 8 ? 11 : 12; }
 
-// threads per workgroup (log2): FP64 512 (256 measured the same); FP32 1024, one quadruple per
-// thread and gate at 4 waves per SIMD with the coefficients in scalar registers (L=16 HVP 1.75 ->
-// 1.52 s against 512 threads with two quadruples per thread and vector coefficients)
-template <typename R>
-constexpr int tile_lt() { return sizeof(R) == 4 ? 10 : 9; }
+// threads per workgroup (log2): FP64 512 (256 measured the same); FP32 HVP 1024, one quadruple
+// per thread and gate at 4 waves per SIMD with the coefficients in scalar registers (L=16 HVP 1.75
+// -> 1.52 s against 512 threads with two quadruples per thread and vector coefficients); FP32 f
+// and f + gradient 512, so that two workgroups (two tiles of one or two states) share a CU
+template <typename R, int MODE>
+constexpr int tile_lt() { return sizeof(R) == 4 && MODE == 2 ? 10 : 9; }
+// waves per SIMD the register allocation must allow: FP32 f + gradient 4 (two 512-thread
+// workgroups per CU, at most 128 VGPRs)
+template <typename R, int MODE>
+constexpr int tile_min_waves() { return sizeof(R) == 4 && MODE == 1 ? 4 : 1; }
 
 __device__ inline uint32_t scatter_bits(uint32_t x, const int* pos, int n)
 {
@@ -64,9 +69,9 @@ __device__ inline void tile_map(const TBlock* blk, int tid, TileMap<PER>* m)
 }  // namespace
 
 template <typename R, int MODE>
-__global__ __launch_bounds__(1 << tile_lt<R>()) void sweep_tile_kernel(TileArgs<R> a)
+__global__ __launch_bounds__((1 << tile_lt<R, MODE>()), (tile_min_waves<R, MODE>())) void sweep_tile_kernel(TileArgs<R> a)
 {
-    constexpr int LT = tile_lt<R>(), T = 1 << LT, NW = T / 64;
+    constexpr int LT = tile_lt<R, MODE>(), T = 1 << LT, NW = T / 64;
     constexpr int NV = MODE == 2 ? 32 : 16;
     constexpr int SHIFT = 6 - log2c<NV>();
     constexpr int KW = tile_kw<R>();
@@ -80,7 +85,7 @@ __global__ __launch_bounds__(1 << tile_lt<R>()) void sweep_tile_kernel(TileArgs<
 
     const int nacc = a.slot ? a.nacc : a.P;
     double* acc = reinterpret_cast<double*>(smem);  // [nacc][32]
-    double* stage = acc + 32 * nacc;                // [2][NW][NV]
+    double* stage = acc + NV * nacc;                // [2][NW][NV]
     double* fst = stage + 2 * NW * NV;              // [NW][2]
     cpx<R>* tl = reinterpret_cast<cpx<R>*>(smem + a.tile_off);
     // tile slots and global slices: psi, dpsi (MODE 2), lam (MODE >= 1), dlam (MODE 2)
@@ -96,7 +101,7 @@ __global__ __launch_bounds__(1 << tile_lt<R>()) void sweep_tile_kernel(TileArgs<
     cpx<R>* g_dlam = sc + 3 * (size_t)S;
 
     double* out = a.partial + (size_t)blockIdx.x * (2 + 32 * a.P);
-    for (int k = tid; k < 32 * nacc; k += T) acc[k] = 0.0;
+    for (int k = tid; k < NV * nacc; k += T) acc[k] = 0.0;
     if (a.slot)
         for (int k = tid; k < 32 * a.P; k += T) out[2 + k] = 0.0;
     double fre = 0.0, fim = 0.0;
@@ -244,7 +249,7 @@ __global__ __launch_bounds__(1 << tile_lt<R>()) void sweep_tile_kernel(TileArgs<
                             const int ci = lane >> 1;
                             int e = ci & 7;
                             if (in.flip && e < 4) e = 3 - e;
-                            acc[32 * (a.slot ? a.slot[g] : in.param) + 2 * ((ci & 8) | e) + (lane & 1)] += s;
+                            acc[NV * (a.slot ? a.slot[g] : in.param) + 2 * ((ci & 8) | e) + (lane & 1)] += s;
                         }
                     }
                     if (k > 0) {
@@ -265,8 +270,8 @@ __global__ __launch_bounds__(1 << tile_lt<R>()) void sweep_tile_kernel(TileArgs<
                 if (a.slot) {
                     // this block's sums of this column into the workgroup's partial sums
                     const int* bp = a.bparam + bk->s0;
-                    for (int k2 = tid; k2 < 32 * bk->ns; k2 += T) {
-                        out[2 + 32 * bp[k2 >> 5] + (k2 & 31)] += acc[k2];
+                    for (int k2 = tid; k2 < NV * bk->ns; k2 += T) {
+                        out[2 + 32 * bp[k2 / NV] + (k2 % NV)] += acc[k2];
                         acc[k2] = 0.0;
                     }
                     __syncthreads();
@@ -281,15 +286,18 @@ __global__ __launch_bounds__(1 << tile_lt<R>()) void sweep_tile_kernel(TileArgs<
         out[1] = fim;
     }
     if (!a.slot)
-        for (int k = tid; k < 32 * a.P; k += T) out[2 + k] = acc[k];
+        for (int k = tid; k < 32 * a.P; k += T) out[2 + k] = (k & 31) < NV ? acc[(k >> 5) * NV + (k & 31)] : 0.0;
 }
 
 int tile_window_bits(int fp32) { return fp32 ? tile_kw<float>() : tile_kw<double>(); }
 
 size_t tile_sweep_lds_bytes(int P, int kw, int fp32, int nstate, size_t* tile_off)
 {
-    const int NW = (1 << (fp32 ? tile_lt<float>() : tile_lt<double>())) / 64;
-    const size_t head = sizeof(double) * (32 * (size_t)P + 2 * NW * 32 + 2 * NW);
+    // nstate 4: the HVP mode (32 accumulated doubles per parameter), else 16
+    const int hvp = nstate == 4;
+    const int NW = (1 << (fp32 ? (hvp ? tile_lt<float, 2>() : tile_lt<float, 1>()) : tile_lt<double, 2>())) / 64;
+    const int NV = hvp ? 32 : 16;
+    const size_t head = sizeof(double) * (NV * (size_t)P + 2 * NW * NV + 2 * NW);
     const size_t off = (head + 255) & ~(size_t)255;
     if (tile_off) *tile_off = off;
     return off + (size_t)nstate * ((size_t)1 << kw) * (fp32 ? sizeof(cpx<float>) : sizeof(cpx<double>));
@@ -303,7 +311,7 @@ static hipError_t launch_tile_m(const TileArgs<R>& a, int grid, size_t lds, hipS
         hipError_t e = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL(k, dim3(grid), dim3(1 << tile_lt<R>()), lds, s, a);
+    hipLaunchKernelGGL(k, dim3(grid), dim3(1 << tile_lt<R, MODE>()), lds, s, a);
     return hipGetLastError();
 }
 
