@@ -389,8 +389,10 @@ def large_bench(mg, L=16):
         dt = time.perf_counter() - t0
         km = mg.batch_last_kernel_ms()
         out[name] = {"s": dt, "kernel_ms": km, "tflops": fl / (km * 1e-3) / 1e12,
-                     "kernels": "materialize_mfma_kernel (target rows of a column chunk) + "
-                                + ("sweep_tile_kernel<float,2>" if prec == mg.MG_FP32 else "sweep_win_tile_kernel<2,2>")}
+                     "kernels": ("materialize_mfma_kernel<float> (FP32 target rows of a column chunk, FP32 MFMA) + "
+                                 "sweep_tile_kernel<float,2>") if prec == mg.MG_FP32 else
+                                ("materialize_mfma_kernel<double> (FP64 target rows of a column chunk, FP64 MFMA) + "
+                                 "sweep_win_tile_kernel<2,2>")}
         log(f"  {dt:.2f} s, kernel {km:.1f} ms, {out[name]['tflops']:.2f} TFLOP/s")
     # the target's rows in HBM: first with the register-tile kernel (timed, then freed), then on
     # the matrix cores (the default), which the evaluations below read
@@ -407,6 +409,11 @@ def large_bench(mg, L=16):
     out["densify_s"] = time.perf_counter() - t0
     out["densify_flops"] = 8.0 * 4 ** w["L"] * 2 * w["Gt"]
     out["densify_kernel"] = "materialize_mfma_kernel (4-qubit windows, two 8x8 parity blocks, FP64 MFMA)"
+    # algorithmic FLOPs (2 complex MACs per amplitude and target gate) over the wall time of the
+    # materialisation (one launch), against the FP64 matrix peak
+    ach = out["densify_flops"] / out["densify_s"] / 1e12
+    out["densify_roofline"] = {"bound": "mfma_fp64", "achieved": ach, "peak": PEAK_TFLOPS["f64"], "unit": "TFLOP/s",
+                               "frac": ach / PEAK_TFLOPS["f64"]}
     log(f"  register tiles {out['densify_register_tiles_s']:.2f} s, matrix cores {out['densify_s']:.2f} s")
     fl_dense = sweep_flops(w["L"], w["G"], 0)
     for name, prec in (("fp32", mg.MG_FP32), ("fp64", mg.MG_FP64)):

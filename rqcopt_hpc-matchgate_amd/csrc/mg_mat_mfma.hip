@@ -26,7 +26,21 @@ namespace mg {
 
 namespace {
 
-typedef double v4d __attribute__((ext_vector_type(4)));
+// one 16x16x4 MFMA per precision: D += A B with lane l supplying A[l % 16][l / 16] and
+// B[l / 16][l % 16]; FP64 returns rows 4 v + l / 16 of column l % 16 as results v = 0..3, FP32
+// rows 4 (l / 16) + v (both checked on the device; the host orders the A rows to match)
+template <typename R>
+struct Mfma;
+template <>
+struct Mfma<double> {
+    typedef double v4 __attribute__((ext_vector_type(4)));
+    static __device__ inline v4 mac(double a, double b, v4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+};
+template <>
+struct Mfma<float> {
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    static __device__ inline v4 mac(float a, float b, v4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+};
 
 // LDS index of tile entry j: the bits above the lowest four folded onto them, so that the
 // columns of a block (any 4 of the tile's bits) fall in different banks; linear over XOR
@@ -41,18 +55,21 @@ __device__ inline uint32_t scatter(uint32_t x, const int* pos, int n)
 
 }  // namespace
 
+template <typename R>
 __global__ __launch_bounds__(256) void materialize_mfma_kernel(MatArgs a)
 {
+    typedef typename Mfma<R>::v4 v4;
     extern __shared__ __align__(16) unsigned char smem[];
-    cpx<double>* tl = reinterpret_cast<cpx<double>*>(smem);  // [4096], index mat_swz(j)
+    cpx<R>* tl = reinterpret_cast<cpx<R>*>(smem);  // [4096], index mat_swz(j)
+    const R* afrag = reinterpret_cast<const R*>(a.afrag);
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int L = a.L;
     const uint32_t S = 1u << L, ntiles = 1u << (L - 12);
     const int n0 = lane >> 4, col = lane & 15;
-    cpx<double>* rows = reinterpret_cast<cpx<double>*>(a.rows);
+    cpx<R>* rows = reinterpret_cast<cpx<R>*>(a.rows);
 
     for (long b = a.b0 + blockIdx.x; b < a.b1; b += gridDim.x) {
-        cpx<double>* row = rows + (size_t)(b - a.row0) * S;
+        cpx<R>* row = rows + (size_t)(b - a.row0) * S;
         for (int k = 0; k < a.ntb; k++) {
             const WTBlock* tb = a.tblocks + k;
             const uint32_t lo = scatter((uint32_t)tid, tb->tpos, 8);
@@ -63,19 +80,19 @@ __global__ __launch_bounds__(256) void materialize_mfma_kernel(MatArgs a)
                 for (uint32_t m = 0; m < 16; m++) {
                     const uint32_t gi = ot | lo | scatter(m, tb->tpos + 8, 4);
                     const uint32_t j = (uint32_t)tid | (m << 8);
-                    tl[mat_swz(j)] = k == 0 ? cpx<double>{gi == (uint32_t)b ? 1.0 : 0.0, 0.0} : row[gi];
+                    tl[mat_swz(j)] = k == 0 ? cpx<R>{R(gi == (uint32_t)b ? 1 : 0), R(0)} : row[gi];
                 }
                 __syncthreads();
                 for (int w = tb->k0; w < tb->k1; w++) {
                     const DWin* dw = a.wins + w;
-                    double af[8];
+                    R af[8];
 #pragma unroll
-                    for (int f = 0; f < 8; f++) af[f] = a.afrag[((size_t)w * 8 + f) * 64 + lane];
+                    for (int f = 0; f < 8; f++) af[f] = afrag[((size_t)w * 8 + f) * 64 + lane];
                     const uint32_t oj = (uint32_t)dw->offj[col];
                     // the wave's 8 products (4 column blocks x 2 parities): every read first, so
                     // that the in-place writes follow them, then 8 independent MFMA chains
                     uint32_t base[4];
-                    cpx<double> x[8][2];
+                    cpx<R> x[8][2];
 #pragma unroll
                     for (int c4 = 0; c4 < 4; c4++) {
                         base[c4] = oj ^ (uint32_t)dw->offcb[wv * 4 + c4];
@@ -85,16 +102,15 @@ __global__ __launch_bounds__(256) void materialize_mfma_kernel(MatArgs a)
                             x[2 * c4 + p][1] = tl[base[c4] ^ (uint32_t)dw->offp[p][4 + n0]];
                         }
                     }
-                    v4d acc[8];
+                    v4 acc[8];
 #pragma unroll
-                    for (int c = 0; c < 8; c++) acc[c] = {0.0, 0.0, 0.0, 0.0};
+                    for (int c = 0; c < 8; c++) acc[c] = v4{R(0), R(0), R(0), R(0)};
 #pragma unroll
                     for (int kk = 0; kk < 4; kk++)
 #pragma unroll
                         for (int c = 0; c < 8; c++) {
-                            const cpx<double>& xv = x[c][kk >> 1];
-                            acc[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[4 * (c & 1) + kk], (kk & 1) ? xv.im : xv.re,
-                                                                          acc[c], 0, 0, 0);
+                            const cpx<R>& xv = x[c][kk >> 1];
+                            acc[c] = Mfma<R>::mac(af[4 * (c & 1) + kk], (kk & 1) ? xv.im : xv.re, acc[c]);
                         }
                     // this lane's results: entries 2 n0 and 2 n0 + 1 of its column
 #pragma unroll
@@ -108,8 +124,8 @@ __global__ __launch_bounds__(256) void materialize_mfma_kernel(MatArgs a)
 #pragma unroll 4
                 for (uint32_t m = 0; m < 16; m++) {
                     const uint32_t gi = ot | lo | scatter(m, tb->tpos + 8, 4);
-                    const cpx<double> v = tl[mat_swz((uint32_t)tid | (m << 8))];
-                    row[gi] = last ? cpx<double>{-v.re, v.im} : v;
+                    const cpx<R> v = tl[mat_swz((uint32_t)tid | (m << 8))];
+                    row[gi] = last ? cpx<R>{-v.re, v.im} : v;
                 }
                 // the next tile's load reuses the LDS tile
                 __syncthreads();
@@ -118,10 +134,13 @@ __global__ __launch_bounds__(256) void materialize_mfma_kernel(MatArgs a)
     }
 }
 
-hipError_t launch_materialize_mfma(const MatArgs& a, int grid, hipStream_t s)
+hipError_t launch_materialize_mfma(const MatArgs& a, int fp32, int grid, hipStream_t s)
 {
     if (a.L <= 12 || a.L - 12 > 20) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(materialize_mfma_kernel, dim3(grid), dim3(256), kMatMfmaLds, s, a);
+    if (fp32)
+        hipLaunchKernelGGL(materialize_mfma_kernel<float>, dim3(grid), dim3(256), mat_mfma_lds(1), s, a);
+    else
+        hipLaunchKernelGGL(materialize_mfma_kernel<double>, dim3(grid), dim3(256), mat_mfma_lds(0), s, a);
     return hipGetLastError();
 }
 
@@ -161,6 +180,7 @@ bool make_dense_windows(const std::vector<Inst>& inst, const std::vector<cpx<dou
     d->tblocks.clear();
     d->wins.clear();
     d->afrag.clear();
+    d->afrag32.clear();
     d->ngates = (int)inst.size();
     if (L <= TB || L - TB > 20) return false;
     std::vector<int> order1;
@@ -221,20 +241,22 @@ bool make_dense_windows(const std::vector<Inst>& inst, const std::vector<cpx<dou
                 dw.offcb[j] = put((uint32_t)j, oth + 4, 4);
             }
             d->wins.push_back(dw);
-            // A fragments: lane l holds row l % 16, column 4 kk + l / 16 of block p's real matrix
+            // A fragments: lane l holds row l % 16, column 4 kk + l / 16 of block p's real matrix.
+            // Rows are ordered so that lane group g's four results are the real and imaginary
+            // parts of entries 2 g and 2 g + 1: FP64 returns rows g, g + 4, g + 8, g + 12 there,
+            // FP32 rows 4 g .. 4 g + 3.
             for (int p = 0; p < 2; p++)
                 for (int kk = 0; kk < 4; kk++)
                     for (int ln = 0; ln < 64; ln++) {
                         const int r = ln % 16, c = 4 * kk + ln / 16;
-                        // result v of lane l is row 4 v + l / 16: rows g, g + 4 (g + 8, g + 12) are
-                        // the real and imaginary parts of entry 2 g (2 g + 1)
-                        const int m = 2 * (r % 4) + (r / 4) / 2, part = (r / 4) % 2;  // output entry, re/im
-                        const int n = (c / 8) * 4 + c % 4, q = (c / 4) % 2;     // input entry, re/im
-                        const zc e = M[kParIdx[p][m] * 16 + kParIdx[p][n]];
-                        double v;
-                        if (part == 0) v = q == 0 ? e.real() : -e.imag();
-                        else v = q == 0 ? e.imag() : e.real();
-                        d->afrag.push_back(v);
+                        const int n = (c / 8) * 4 + c % 4, q = (c / 4) % 2;  // input entry, re/im
+                        auto value = [&](int m, int part) {
+                            const zc e = M[kParIdx[p][m] * 16 + kParIdx[p][n]];
+                            if (part == 0) return q == 0 ? e.real() : -e.imag();
+                            return q == 0 ? e.imag() : e.real();
+                        };
+                        d->afrag.push_back(value(2 * (r % 4) + (r / 4) / 2, (r / 4) % 2));
+                        d->afrag32.push_back((float)value(2 * (r / 4) + (r % 4) / 2, (r % 4) % 2));
                     }
         }
         t.k1 = (int)d->wins.size();

@@ -27,7 +27,8 @@ struct DWin {
 struct DenseWindows {
     std::vector<WTBlock> tblocks;
     std::vector<DWin> wins;
-    std::vector<double> afrag;
+    std::vector<double> afrag;   // FP64 MFMA
+    std::vector<float> afrag32;  // FP32 MFMA (its own result layout, rows 4 g .. 4 g + 3)
     int ngates = 0;
 };
 
@@ -39,12 +40,13 @@ struct MatArgs {
     int L, ntb;
     const WTBlock* tblocks;
     const DWin* wins;
-    const double* afrag;
-    long b0, b1;   // rows [b0, b1)
-    void* rows;    // cpx<double>: row b at rows + (b - row0) S, -conj(W e_b)
+    const void* afrag;  // double (FP64 rows) or float (FP32 rows) fragments
+    long b0, b1;        // rows [b0, b1)
+    void* rows;         // cpx<double> or cpx<float>: row b at rows + (b - row0) S, -conj(W e_b)
     long row0;
 };
-constexpr size_t kMatMfmaLds = ((size_t)1 << 12) * sizeof(cpx<double>);  // one 12-qubit FP64 tile
-hipError_t launch_materialize_mfma(const MatArgs& a, int grid, hipStream_t s);
+// one 12-qubit tile of the rows' precision in LDS
+inline size_t mat_mfma_lds(int fp32) { return ((size_t)1 << 12) * (fp32 ? sizeof(cpx<float>) : sizeof(cpx<double>)); }
+hipError_t launch_materialize_mfma(const MatArgs& a, int fp32, int grid, hipStream_t s);
 
 }  // namespace mg

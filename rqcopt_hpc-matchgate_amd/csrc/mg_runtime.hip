@@ -570,35 +570,45 @@ static int prepare_mat_form(Target& t)
     MG_HIP(f.tb.ensure(d.tblocks.size() * sizeof(WTBlock)));
     MG_HIP(f.win.ensure(d.wins.size() * sizeof(DWin)));
     MG_HIP(f.af.ensure(d.afrag.size() * sizeof(double)));
+    MG_HIP(f.af32.ensure(d.afrag32.size() * sizeof(float)));
     MG_HIP(hipMemcpy(f.tb.p, d.tblocks.data(), d.tblocks.size() * sizeof(WTBlock), hipMemcpyHostToDevice));
     MG_HIP(hipMemcpy(f.win.p, d.wins.data(), d.wins.size() * sizeof(DWin), hipMemcpyHostToDevice));
     MG_HIP(hipMemcpy(f.af.p, d.afrag.data(), d.afrag.size() * sizeof(double), hipMemcpyHostToDevice));
+    MG_HIP(hipMemcpy(f.af32.p, d.afrag32.data(), d.afrag32.size() * sizeof(float), hipMemcpyHostToDevice));
     f.ntb = (int)d.tblocks.size();
     f.ready = true;
     return 0;
 }
 
 // rows [b0, b1) of a target program beyond 12 qubits, -conj(W e_b), into `rows` (row b at
-// rows + (b - row0) S): on the matrix cores (4-qubit windows, FP64 MFMA) or in 12-qubit register
-// tiles (the tiled window kernel's forward sweep)
-static int launch_target_rows(Context* c, Target& t, long b0, long b1, void* rows, long row0, hipStream_t s)
+// rows + (b - row0) S): on the matrix cores (4-qubit windows; FP64 MFMA, or FP32 MFMA into FP32
+// rows for FP32 sweeps) or in 12-qubit register tiles (the tiled window kernel's forward sweep,
+// FP64 rows only)
+static bool rows_on_matrix_cores() { return knobs().mat_kernel != 0; }
+
+static int launch_target_rows(Context* c, Target& t, long b0, long b1, void* rows, long row0, hipStream_t s,
+                              int fp32 = 0)
 {
-    if (knobs().mat_kernel != 0) {
+    if (rows_on_matrix_cores()) {
         if (prepare_mat_form(t) < 0) return -1;
         MatArgs m{};
         m.L = t.L;
         m.ntb = t.mf.ntb;
         m.tblocks = (const WTBlock*)t.mf.tb.p;
         m.wins = (const DWin*)t.mf.win.p;
-        m.afrag = (const double*)t.mf.af.p;
+        m.afrag = fp32 ? t.mf.af32.p : t.mf.af.p;
         m.b0 = b0;
         m.b1 = b1;
         m.rows = rows;
         m.row0 = row0;
-        const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(2, c->lds_limit / kMatMfmaLds));
+        const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(2, c->lds_limit / mat_mfma_lds(fp32)));
         const int grid = (int)std::min<long>(b1 - b0, (long)c->ncu * per_cu);
-        MG_HIP(launch_materialize_mfma(m, grid, s));
+        MG_HIP(launch_materialize_mfma(m, fp32, grid, s));
         return 0;
+    }
+    if (fp32) {
+        set_error("target materialisation: FP32 rows need the matrix-core kernel");
+        return -1;
     }
     if (prepare_tile_form(c, t) < 0) return -1;
     const WinTileArgs a = tile_form_args(t, b0, b1, rows, row0);
@@ -763,6 +773,10 @@ static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, in
         }
     }
     const bool chunk_rows = chunkable && use_tile;
+    // FP32 tiled sweeps of a chunked target read FP32 rows made on the matrix cores (the FP64
+    // register-window tiles need FP64 rows)
+    const bool rows32 = chunk_rows && fp32 && rows_on_matrix_cores();
+    auto row_bytes_of = [](bool r32, size_t n) { return n * (r32 ? sizeof(cpx<float>) : sizeof(cpx<double>)); };
     if (use_tile) reg_T = 0;
     // tiled register-window kernel (FP64, unitary gates): preferred over the LDS tiles
     bool use_wtile = false;
@@ -804,7 +818,7 @@ static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, in
     // each chunk's workgroup partial sums are kept and all are reduced together
     long chunk = ncols, nchunks = 1;
     if (chunk_rows) {
-        const long row_bytes = (long)(S * sizeof(cpx<double>));
+        const long row_bytes = (long)(S * (rows32 ? sizeof(cpx<float>) : sizeof(cpx<double>)));
         const size_t cb = kn.row_chunk_bytes ? kn.row_chunk_bytes : ((size_t)4 << 30);
         chunk = std::max<long>(grid, (long)(cb / (size_t)row_bytes) / grid * grid);
         chunk = std::min<long>(chunk, ncols);
@@ -817,7 +831,7 @@ static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, in
         ar.get(c->out, (size_t)stride * sizeof(double), &d_out) < 0)
         return -1;
     if (chunk_rows) {
-        if (ar.get(c->rows, (size_t)chunk * S * sizeof(cpx<double>), &d_rows) < 0) return -1;
+        if (ar.get(c->rows, (size_t)chunk * (size_t)row_bytes_of(rows32, S), &d_rows) < 0) return -1;
         if ((kn.mat_kernel != 0 ? prepare_mat_form(t) : prepare_tile_form(c, t)) < 0) return -1;
     }
     if (use_tile) {
@@ -1005,7 +1019,7 @@ static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, in
     if (chunk_rows) {
         for (long ci = 0; ci < nchunks; ci++) {
             const long c0 = b0 + ci * chunk, c1 = std::min<long>(b1, c0 + chunk);
-            if (launch_target_rows(c, t, c0, c1, d_rows, c0, s) < 0) return -1;
+            if (launch_target_rows(c, t, c0, c1, d_rows, c0, s, rows32 ? 1 : 0) < 0) return -1;
             double* part = (double*)d_partial + (size_t)ci * grid * stride;
             if (use_wtile) {
                 WinTileArgs w2 = wta;
@@ -1020,8 +1034,9 @@ static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, in
                 t2.b0 = c0;
                 t2.b1 = c1;
                 t2.row0 = c0;
-                t2.seed = fp32 ? nullptr : (const cpx<R>*)d_rows;
-                t2.seed64 = fp32 ? (const cpx<double>*)d_rows : nullptr;
+                // FP32 sweeps read FP32 rows from the matrix cores, else the FP64 rows (rounded on load)
+                t2.seed = (!fp32 || rows32) ? (const cpx<R>*)d_rows : nullptr;
+                t2.seed64 = (fp32 && !rows32) ? (const cpx<double>*)d_rows : nullptr;
                 t2.partial = part;
                 MG_HIP(launch_sweep_tile<R>(t2, mode, (int)grid, lds, s));
             }

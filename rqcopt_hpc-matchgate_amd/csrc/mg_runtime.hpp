@@ -68,7 +68,7 @@ struct Target {
     TileForm tf;
     // the target program in 4-qubit windows for the matrix-core materialisation (L > 12)
     struct MatForm {
-        DevBuf tb, win, af;
+        DevBuf tb, win, af, af32;
         int ntb = 0;
         bool ready = false;
     };
