@@ -136,7 +136,8 @@ __global__ __launch_bounds__(256) void materialize_mfma_kernel(MatArgs a)
 
 hipError_t launch_materialize_mfma(const MatArgs& a, int fp32, int grid, hipStream_t s)
 {
-    if (a.L <= 12 || a.L - 12 > 20) return hipErrorInvalidValue;
+    // 32-bit amplitude indices: at most 30 qubits (the tiled sweeps' limit too)
+    if (a.L <= 12 || a.L > 30) return hipErrorInvalidValue;
     if (fp32)
         hipLaunchKernelGGL(materialize_mfma_kernel<float>, dim3(grid), dim3(256), mat_mfma_lds(1), s, a);
     else
