@@ -654,6 +654,7 @@ void mg_set_window_kernel(int mode) { mg::knobs().force_win = mode; }
 void mg_set_tile_kernel(int mode) { mg::knobs().force_tile = mode != 0 ? -1 : 0; }
 void mg_set_row_chunk_bytes(long bytes) { mg::knobs().row_chunk_bytes = bytes > 0 ? (size_t)bytes : 0; }
 void mg_set_materialize_kernel(int mode) { mg::knobs().mat_kernel = mode; }
+void mg_set_forward_mfma(int mode) { mg::knobs().fwd_mfma = mode; }
 float mg_last_kernel_ms(void) { return g_last_stats.kernel_ms; }
 void mg_last_sweep_info(int* lds_states, int* ckpt, int* grid, long* lds_bytes)
 {

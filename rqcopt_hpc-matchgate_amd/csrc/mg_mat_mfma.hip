@@ -175,6 +175,69 @@ void apply_gate_left(zc* M, const cpx<double>* g, int la, int lb)
 
 }  // namespace
 
+namespace {
+
+// the FP32 tile kernel's LDS index (lds_swz of cpx<float>: bits 4 and 5 folded into bits 0-3)
+int tile_swz32(uint32_t a) { return (int)(a ^ (((a >> 4) & 1u) * 5u) ^ (((a >> 5) & 1u) * 10u)); }
+
+// lane fragments of one parity block of a 16 x 16 window matrix, FP32 result layout (rows 4 g ..
+// 4 g + 3: real and imaginary parts of entries 2 g and 2 g + 1)
+void push_fragments32(const zc* M, std::vector<float>* af)
+{
+    for (int p = 0; p < 2; p++)
+        for (int kk = 0; kk < 4; kk++)
+            for (int ln = 0; ln < 64; ln++) {
+                const int r = ln % 16, c = 4 * kk + ln / 16;
+                const int n = (c / 8) * 4 + c % 4, q = (c / 4) % 2;
+                const int m = 2 * (r / 4) + (r % 4) / 2, part = (r % 4) % 2;
+                const zc e = M[kParIdx[p][m] * 16 + kParIdx[p][n]];
+                af->push_back((float)(part == 0 ? (q == 0 ? e.real() : -e.imag()) : (q == 0 ? e.imag() : e.real())));
+            }
+}
+
+}  // namespace
+
+int forward_windows(const std::vector<Inst>& inst, const std::vector<cpx<double>>& coef, std::vector<int>* order,
+                    std::vector<FWin>* wins, std::vector<float>* af)
+{
+    constexpr int TB = 12, W = 4;
+    std::vector<WBlock> wbs;
+    window_blocks(inst, TB, W, 1 << 20, order, &wbs);
+    if (order->size() != inst.size()) return -1;
+    for (const WBlock& wb : wbs) {
+        zc M[256];
+        for (int i = 0; i < 256; i++) M[i] = (i / 16 == i % 16) ? 1.0 : 0.0;
+        for (int k = wb.g0; k < wb.g1; k++) {
+            const Inst& l = inst[(*order)[k]];
+            int la = -1, lb = -1;
+            for (int i = 0; i < W; i++) {
+                if (wb.pos[i] == l.lo) la = i;
+                if (wb.pos[i] == l.hi) lb = i;
+            }
+            if (la < 0 || lb < 0) return -1;
+            apply_gate_left(M, &coef[(size_t)(*order)[k] * kCoefPerInst], la, lb);
+        }
+        FWin fw{};
+        int oth[8], no = 0;
+        for (int q = 0; q < TB; q++)
+            if (std::find(wb.pos, wb.pos + W, q) == wb.pos + W) oth[no++] = q;
+        auto put = [&](uint32_t x, const int* pos, int n) {
+            uint32_t r = 0;
+            for (int i = 0; i < n; i++) r |= ((x >> i) & 1u) << pos[i];
+            return tile_swz32(r);
+        };
+        for (int p = 0; p < 2; p++)
+            for (int n = 0; n < 8; n++) fw.offp[p][n] = put((uint32_t)kParIdx[p][n], wb.pos, W);
+        for (int j = 0; j < 16; j++) {
+            fw.offj[j] = put((uint32_t)j, oth, 4);
+            fw.offcb[j] = put((uint32_t)j, oth + 4, 4);
+        }
+        wins->push_back(fw);
+        push_fragments32(M, af);
+    }
+    return (int)wbs.size();
+}
+
 bool make_dense_windows(const std::vector<Inst>& inst, const std::vector<cpx<double>>& coef, int L, DenseWindows* d)
 {
     constexpr int TB = 12, W = 4;

@@ -36,6 +36,22 @@ struct DenseWindows {
 // 4-qubit windows inside 12-qubit tile blocks; false when the blocking fails (L <= 12)
 bool make_dense_windows(const std::vector<Inst>& inst, const std::vector<cpx<double>>& coef, int L, DenseWindows* d);
 
+// The forward pass of an f or f + gradient sweep (psi <- V psi per gate) over 4-qubit windows of a
+// tile block: per window psi_out = M psi_in with M the product of the window's V, which keeps parity
+// and is two 8 x 8 blocks, FP32 MFMA tiles as for the materialisation.  Offsets are LDS indices of
+// the FP32 tile kernel (lds_swz, linear over XOR).
+struct FWin {
+    int offp[2][8];
+    int offj[16];
+    int offcb[16];
+};
+
+// `inst` (tile-local bit positions) and `coef` (V first in each instance's coefficients) are one
+// tile block's gates in program order; fills `order` with a window order of them, appends the
+// windows and their FP32 fragments [window][8][lane] and returns the number of windows
+int forward_windows(const std::vector<Inst>& inst, const std::vector<cpx<double>>& coef, std::vector<int>* order,
+                    std::vector<FWin>* wins, std::vector<float>* af);
+
 struct MatArgs {
     int L, ntb;
     const WTBlock* tblocks;

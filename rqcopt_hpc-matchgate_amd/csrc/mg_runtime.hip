@@ -77,6 +77,7 @@ struct Context {
     DevBuf rows;  // materialised target rows of one column chunk (program targets beyond 12 qubits)
     DevBuf w_inst, w_coef, w_blk;  // gate-window form of the program (register window kernel)
     DevBuf w_slot, w_bpar;         // tiled kernel: per-block accumulator slots and their parameters
+    DevBuf w_fwin, w_fwaf;         // tiled kernel: forward-pass windows and their MFMA fragments
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::vector<double> out_h;
     // batched evaluation: packed descriptor upload, partials, outputs, scratch, counter
@@ -956,6 +957,43 @@ static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, in
             set_error("tiled sweep: gate blocking failed");
             return -1;
         }
+        // FP32 f and f + gradient: the forward pass of each tile block on the matrix cores, over
+        // 4-qubit windows (the block's instances reordered into window order; the accumulator slots
+        // follow them)
+        std::vector<FWin> fwins;
+        std::vector<float> fwaf;
+        if (fp32 && mode <= 1 && kn.fwd_mfma != 0) {
+            std::vector<int> order_w(order.size()), slot_w(tb_slot.size());
+            for (TBlock& bk : blocks) {
+                std::vector<Inst> li;
+                std::vector<cpx<double>> lc;
+                for (int k = bk.g0; k < bk.g1; k++) {
+                    Inst in = b.inst[order[k]];
+                    in.lo = (int)(std::find(bk.pos, bk.pos + tile_kw, in.lo) - bk.pos);
+                    in.hi = (int)(std::find(bk.pos, bk.pos + tile_kw, in.hi) - bk.pos);
+                    if (in.lo >= tile_kw || in.hi >= tile_kw) {
+                        set_error("tiled sweep: gate outside its window");
+                        return -1;
+                    }
+                    li.push_back(in);
+                    lc.insert(lc.end(), b.coef.begin() + (size_t)order[k] * kCoefPerInst,
+                              b.coef.begin() + (size_t)(order[k] + 1) * kCoefPerInst);
+                }
+                std::vector<int> o2;
+                bk.w0 = (int)fwins.size();
+                if (forward_windows(li, lc, &o2, &fwins, &fwaf) < 0) {
+                    set_error("tiled sweep: forward window blocking failed");
+                    return -1;
+                }
+                bk.w1 = (int)fwins.size();
+                for (int i = 0; i < bk.g1 - bk.g0; i++) {
+                    order_w[bk.g0 + i] = order[bk.g0 + o2[i]];
+                    if (tile_slots) slot_w[bk.g0 + i] = tb_slot[bk.g0 + o2[i]];
+                }
+            }
+            order.swap(order_w);
+            if (tile_slots) tb_slot.swap(slot_w);
+        }
         std::vector<Inst> ti(order.size());
         std::vector<cpx<R>> tc(order.size() * kCoefPerInst);
         for (const TBlock& bk : blocks)
@@ -983,6 +1021,16 @@ static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, in
         MG_HIP(hipMemcpyAsync(d_wi, ti.data(), ti.size() * sizeof(Inst), hipMemcpyHostToDevice, s));
         MG_HIP(hipMemcpyAsync(d_wc, tc.data(), tc.size() * sizeof(cpx<R>), hipMemcpyHostToDevice, s));
         MG_HIP(hipMemcpyAsync(d_wb, blocks.data(), blocks.size() * sizeof(TBlock), hipMemcpyHostToDevice, s));
+        if (!fwins.empty()) {
+            void *d_fw = nullptr, *d_fa = nullptr;
+            if (ar.get(c->w_fwin, fwins.size() * sizeof(FWin), &d_fw) < 0 ||
+                ar.get(c->w_fwaf, fwaf.size() * sizeof(float), &d_fa) < 0)
+                return -1;
+            MG_HIP(hipMemcpyAsync(d_fw, fwins.data(), fwins.size() * sizeof(FWin), hipMemcpyHostToDevice, s));
+            MG_HIP(hipMemcpyAsync(d_fa, fwaf.data(), fwaf.size() * sizeof(float), hipMemcpyHostToDevice, s));
+            ta.fwin = d_fw;
+            ta.fwaf = (const float*)d_fa;
+        }
         if (tile_slots) {
             void *d_sl = nullptr, *d_bp = nullptr;
             if (ar.get(c->w_slot, tb_slot.size() * sizeof(int), &d_sl) < 0 ||

@@ -17,6 +17,7 @@ struct TBlock {
     int pos[kTileMaxBits];
     int other[32 - kTileMaxBits];
     int s0, ns;  // per-block accumulator slots: the block's parameters are bparam[s0, s0 + ns)
+    int w0, w1;  // forward-pass windows of the block on the matrix cores (TileArgs.fwin)
 };
 
 template <typename R>
@@ -40,6 +41,10 @@ struct TileArgs {
     const int* slot;
     const int* bparam;
     int nacc;
+    // FP32 f and f + gradient: the forward pass on the matrix cores, 4-qubit windows [w0, w1) per block (struct FWin
+    // of mg_mat_mfma.hpp; the block's instances in window order); null: gate by gate
+    const void* fwin;
+    const float* fwaf;  // [window][8][lane] A fragments of the windows
 };
 
 // Regroups the instances (program order; bit positions hi > lo) into blocks whose gates act

@@ -128,6 +128,7 @@ _sig("mg_set_window_kernel", [_I], None)
 _sig("mg_set_tile_kernel", [_I], None)
 _sig("mg_set_row_chunk_bytes", [_L], None)
 _sig("mg_set_materialize_kernel", [_I], None)
+_sig("mg_set_forward_mfma", [_I], None)
 _sig("mg_last_kernel_ms", [], ctypes.c_float)
 _sig("mg_last_sweep_info", [_P, _P, _P, _P], None)
 _sig("mg_target_cache_clear", [], None)
@@ -535,6 +536,11 @@ def set_window_kernel(mode):
 def set_tile_kernel(mode):
     """-1: tiled kernel where it applies (default), 0: never"""
     lib.mg_set_tile_kernel(int(mode))
+
+
+def set_forward_mfma(mode):
+    """FP32 tiled sweeps: -1 forward pass on the matrix cores (default), 0 gate by gate"""
+    lib.mg_set_forward_mfma(int(mode))
 
 
 def set_materialize_kernel(mode):

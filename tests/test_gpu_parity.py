@@ -318,6 +318,17 @@ def test_tile_kernels_match_oracle(mg, L, layout, depth, prec):
     tol_g = 1e-12 if prec == "fp64" else 2e-5
     for r in results:
         assert abs(r[0] - g[0]) / abs(g[0]) < tol_g and rel(r[1], g[1]) < tol_g and rel(r[2], g[2]) < tol_g
+    if prec == "fp32" and L > 13:
+        # FP32 f and f + gradient: forward pass gate by gate on the VALU against the default
+        # (4-qubit windows on the matrix cores), both against the oracle
+        mg.set_forward_mfma(0)
+        try:
+            v1 = mg.evaluate_columns(t, p, V, None, mg.EVAL_F_GRAD, precision, c0, c1)
+            v0 = mg.evaluate_columns(t, p, V, None, mg.EVAL_F, precision, c0, c1)
+            assert mg.last_sweep_info()["kernel"] == "sweep_tile_kernel"
+        finally:
+            mg.set_forward_mfma(-1)
+        assert abs(v1[0] - fo) / abs(fo) < tol and rel(v1[1], dgo) < tol and abs(v0[0] - fo) / abs(fo) < tol
 
 
 def test_register_kernel_brickwall_target(mg):
