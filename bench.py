@@ -67,8 +67,8 @@ def host_info():
 
 
 def cpu_threads():
-    """Threads for the CPU baselines: OMP_NUM_THREADS where it is set (the CPU share a GPU box
-    gives one GPU), else every core this process may run on."""
+    """Threads for the CPU baselines: OMP_NUM_THREADS where it is set (the host cores that go with
+    one GPU on a shared machine), else every core this process may run on."""
     aff = len(os.sched_getaffinity(0))
     env = os.environ.get("OMP_NUM_THREADS")
     return max(1, min(aff, int(env))) if env and env.isdigit() else aff
@@ -211,7 +211,7 @@ def device_free_bytes():
     return free.value, total.value
 
 
-BASELINE_RUNS = {
+ONE_JOB_BASELINES = {
     "plain": ("baseline_fp64_one_job_plain", "speedup_vs_plain", dict(densify_qubits=0),
               "one job at a time, FP64, one launch per evaluation, circuit targets applied per column, the "
               "reference trust-region control flow"),
@@ -231,7 +231,7 @@ def served_bench(mg, njobs, seed=1, Ls=(8, 10, 12, 14, 16), baselines=("plain",)
     targets of 3x the depth; FP32 sweeps with the last 3 iterations in FP64, continuous batching
     (evaluations that fill the GPU alone in the server's second lane), targets materialised at
     admission, each job's trust region reusing its own products (tr_reuse), admission by device
-    memory.  Against the same stream run one job at a time in FP64 (see BASELINE_RUNS), each
+    memory.  Against the same stream run one job at a time in FP64 (see ONE_JOB_BASELINES), each
     baseline stopped after limit_s seconds: a baseline that does not finish is reported as such
     (jobs done, seconds), with the lower bound that its unfinished run puts on the speedup."""
     from rqcopt_mg.workloads import served_stream
@@ -275,7 +275,7 @@ def served_bench(mg, njobs, seed=1, Ls=(8, 10, 12, 14, 16), baselines=("plain",)
            "kernel_s": st["kernel_ms"] * 1e-3, "wall_s": st["wall_s"], "graph_replays": st["graph_replays"],
            "hvp_sweeps": int(sum(r["nhvp"] for r in res))}
     for name in baselines:
-        key, skey, kw, note = BASELINE_RUNS[name]
+        key, skey, kw, note = ONE_JOB_BASELINES[name]
         log(f"the same stream, {note} (stopped after {limit_s:.0f} s) ...")
         r1, s1 = mg.serve(jobs, batched=False, precision=mg.MG_FP64, realtime=True, progress_s=20.0,
                           time_limit_s=limit_s, **kw)
