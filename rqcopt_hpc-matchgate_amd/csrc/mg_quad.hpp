@@ -6,15 +6,23 @@
 namespace mg {
 
 // ---- wave-level transposing reduction -------------------------------------------
-// 32-bit exchange with lane ^ O.  Offsets 1 and 2 are DPP quad permutations
-// ([1,0,3,2] = 0xB1, [2,3,0,1] = 0x4E), offsets 4..16 ds_swizzle in bit-mask mode
-// (and 0x1F, or 0, xor O), 32 ds_bpermute.  DPP and ds_swizzle take the pattern as an
-// immediate, so no per-lane address register stays live across the sweep.
+// 32-bit exchange with a partner lane whose index differs from this lane's in bit O
+// (log2 O) and possibly in lower bits, which is all that a butterfly reduction over the
+// bits 5..0 needs: each step's partner mask has a new highest bit, so the masks stay
+// independent and every lane still sums all 64.  O = 1, 2: DPP quad permutations
+// ([1,0,3,2] = 0xB1, [2,3,0,1] = 0x4E: lane ^ 1, lane ^ 2); O = 4: DPP row_half_mirror
+// (0x141: lane ^ 7 within each 8-lane half row); O = 8: DPP row_ror:8 (0x128: lane ^ 8
+// within each 16-lane row); O = 16: ds_swizzle in bit-mask mode (and 0x1F, or 0, xor 16);
+// O = 32: ds_bpermute.  The DPP forms are plain vector ALU moves: no LDS-unit round trip
+// and no wait on the LDS counter.  The patterns are immediates, so no per-lane address
+// register stays live across the sweep.
 template <int O>
 __device__ inline uint32_t xlane32(uint32_t x)
 {
     if constexpr (O == 1) return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0xB1, 0xF, 0xF, true);
     else if constexpr (O == 2) return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x4E, 0xF, 0xF, true);
+    else if constexpr (O == 4) return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x141, 0xF, 0xF, true);
+    else if constexpr (O == 8) return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x128, 0xF, 0xF, true);
     else if constexpr (O < 32) return (uint32_t)__builtin_amdgcn_ds_swizzle((int)x, (O << 10) | 0x1F);
     else return (uint32_t)__shfl_xor((int)x, O);
 }

@@ -137,15 +137,28 @@ __device__ inline void wg_backward(const cpx<R>* cf, Win<R, W>& Lm, Win<R, W>& P
     }
 }
 
-// dispatch on the window-local pair index (la, lb): (1,0) (2,0) (2,1) (3,0) (3,1) (3,2)
-#define MG_WCASES(W, CALL)                                   \
-    switch (wcase) {                                         \
-        case 0: CALL(W, 1, 0); break;                        \
-        case 1: if constexpr (W >= 3) { CALL(W, 2, 0); } break; \
-        case 2: if constexpr (W >= 3) { CALL(W, 2, 1); } break; \
-        case 3: if constexpr (W >= 4) { CALL(W, 3, 0); } break; \
-        case 4: if constexpr (W >= 4) { CALL(W, 3, 1); } break; \
-        default: if constexpr (W >= 4) { CALL(W, 3, 2); } break; \
+// dispatch on the window-local pair index (la, lb): (1,0) (2,0) (2,1) (3,0) (3,1) (3,2).
+// The host only emits pairs inside the window, so the last pair of a W-bit window is the
+// default: every path applies the gate, and no path passes the states through unchanged
+// (a pass-through path makes the compiler copy every state register before the switch).
+#define MG_WCASES(W, CALL)                                                  \
+    if constexpr (W == 2) {                                                 \
+        CALL(W, 1, 0);                                                      \
+    } else if constexpr (W == 3) {                                          \
+        switch (wcase) {                                                    \
+            case 0: CALL(W, 1, 0); break;                                   \
+            case 1: CALL(W, 2, 0); break;                                   \
+            default: CALL(W, 2, 1); break;                                  \
+        }                                                                   \
+    } else {                                                                \
+        switch (wcase) {                                                    \
+            case 0: CALL(W, 1, 0); break;                                   \
+            case 1: CALL(W, 2, 0); break;                                   \
+            case 2: CALL(W, 2, 1); break;                                   \
+            case 3: CALL(W, 3, 0); break;                                   \
+            case 4: CALL(W, 3, 1); break;                                   \
+            default: CALL(W, 3, 2); break;                                  \
+        }                                                                   \
     }
 
 template <typename R, int W, bool SWZ>
