@@ -302,10 +302,20 @@ __device__ inline Quad<R> gapplyH(const cpx<R>* g, const Quad<R>& q)
             cfma_conj(g[1], q.y, cmul_conj(g[3], q.z)), cfma_conj(g[5], q.x, cmul_conj(g[7], q.w))};
 }
 
+// c - a b, as two fused multiply-adds per part
 template <typename R>
-__device__ inline Quad<R> qsub(const Quad<R>& a, const Quad<R>& b)
+__host__ __device__ inline cpx<R> cfms(cpx<R> a, cpx<R> b, cpx<R> c)
 {
-    return {csub(a.x, b.x), csub(a.y, b.y), csub(a.z, b.z), csub(a.w, b.w)};
+    return {rfma(-a.re, b.re, rfma(a.im, b.im, c.re)), rfma(-a.re, b.im, rfma(-a.im, b.re, c.im))};
+}
+
+// a - Z q  (the uncompute's dpsi_{g+1} - Z_g psi_g), FMAs only: no products rounded apart and
+// no separate subtraction
+template <typename R>
+__device__ inline Quad<R> gapply_sub(const cpx<R>* g, const Quad<R>& q, const Quad<R>& a)
+{
+    return {cfms(g[4], q.x, cfms(g[5], q.w, a.x)), cfms(g[0], q.y, cfms(g[1], q.z, a.y)),
+            cfms(g[2], q.y, cfms(g[3], q.z, a.z)), cfms(g[6], q.x, cfms(g[7], q.w, a.w))};
 }
 
 // acc[0..7] += d (x) p  (reference src/gate.c:753-757): center then corner entries
@@ -368,7 +378,7 @@ __device__ inline void rev_quads(uint32_t q0, uint32_t stride, int lo, int hi, u
         store_quad_o(psi, sb[i], olo, ohi, p);
         outer_acc(ga, l[i], p);
         if constexpr (MODE == 2) {
-            const Quad<R> dp = gapply(cf + 8, qsub(dn[i], gapply(cf + 16, p)));
+            const Quad<R> dp = gapply(cf + 8, gapply_sub(cf + 16, p, dn[i]));
             store_quad_o(dpsi, sb[i], olo, ohi, dp);
             outer_acc(ha, dl[i], p);
             outer_acc(ha, l[i], dp);

@@ -174,7 +174,7 @@ __global__ __launch_bounds__(T) void sweep_kernel(SweepArgs<R> a)
                         store_quad_o(psi, sb, olo, ohi, p);
                         if constexpr (MODE == 2) {
                             const Quad<R> dn = load_quad_o(dpsi, sb, olo, ohi);
-                            dp = gapply(cf + 8, qsub(dn, gapply(cf + 16, p)));
+                            dp = gapply(cf + 8, gapply_sub(cf + 16, p, dn));
                             store_quad_o(dpsi, sb, olo, ohi, dp);
                         }
                     }

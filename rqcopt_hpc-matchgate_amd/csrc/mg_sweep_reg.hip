@@ -207,7 +207,7 @@ __global__ __launch_bounds__(T) void sweep_reg_kernel(SweepArgs<R> a)
                     for (int k = 0; k < QPT; k++) {
                         const Quad<R> p = gapplyH(cf, st[1].q[k]);
                         st[1].q[k] = p;
-                        if constexpr (MODE == 2) st[3].q[k] = gapplyH(cf, qsub(st[3].q[k], gapply(cf + 16, p)));
+                        if constexpr (MODE == 2) st[3].q[k] = gapplyH(cf, gapply_sub(cf + 16, p, st[3].q[k]));
                         outer_acc(ga, st[0].q[k], p);
                         // keep the quadruples' temporaries from overlapping (VGPR pressure)
                         __builtin_amdgcn_sched_barrier(0);

@@ -50,7 +50,7 @@ __device__ inline void pass_a(const cpx<R>* V, const cpx<R>* Z, const Win<R, kW>
         const Quad<R> p = gapplyH(V, wq_get<R, kW, LA, LB>(P, r));
         wq_put<R, kW, LA, LB>(P, r, p);
         if constexpr (MODE == 2)
-            wq_put<R, kW, LA, LB>(D, r, gapplyH(V, qsub(wq_get<R, kW, LA, LB>(D, r), gapply(Z, p))));
+            wq_put<R, kW, LA, LB>(D, r, gapplyH(V, gapply_sub(Z, p, wq_get<R, kW, LA, LB>(D, r))));
         outer_acc(ga, wq_get<R, kW, LA, LB>(Lm, r), p);
         __builtin_amdgcn_sched_barrier(0);  // one quadruple's temporaries at a time
     }

@@ -126,7 +126,7 @@ __device__ inline void wg_backward(const cpx<R>* cf, Win<R, W>& Lm, Win<R, W>& P
         wq_put<R, W, LA, LB>(P, r, p);
         outer_acc(ga, l, p);
         if constexpr (MODE == 2) {
-            const Quad<R> dp = gapply(cf + 8, qsub(wq_get<R, W, LA, LB>(D, r), gapply(cf + 16, p)));
+            const Quad<R> dp = gapply(cf + 8, gapply_sub(cf + 16, p, wq_get<R, W, LA, LB>(D, r)));
             wq_put<R, W, LA, LB>(D, r, dp);
             const Quad<R> dl = wq_get<R, W, LA, LB>(DL, r);
             outer_acc(ha, dl, p);
