@@ -96,6 +96,27 @@ __device__ inline void stage8(const cpx<R>* s, double* slot, int lane)
     if ((lane & 3) == 0) slot[lane >> 2] = (double)v[0];
 }
 
+// after the barrier that follows a window block's reverse sweep, the wave that owns a gate's
+// parameter adds the gate's staged sums: gates of one block that share a parameter (tied layers)
+// are added by one wave, in order
+template <int kNW, int NV>
+__device__ inline void win_rev_adds(const WinSweepArgs& a, int g0, int g1, const double* stage, double* acc, int wv,
+                                    int lane)
+{
+    for (int g = g0; g < g1; g++) {
+        const int gparam = cptr(a.inst)[g].param, gflip = cptr(a.inst)[g].flip;
+        if (wv == gparam % kNW && lane < NV) {
+            const double* sg = stage + (size_t)(g - g0) * kNW * 32;
+            double s = 0.0;
+            for (int w = 0; w < kNW; w++) s += sg[w * 32 + lane];
+            const int ci = lane >> 1;
+            int e = ci & 7;
+            if (gflip && e < 4) e = 3 - e;
+            acc[NV * gparam + 2 * ((ci & 8) | e) + (lane & 1)] += s;
+        }
+    }
+}
+
 }  // namespace
 
 template <typename R, int MODE, int RS>
@@ -104,7 +125,11 @@ __global__ __launch_bounds__(WinShape<R>::T) void sweep_win_kernel(WinSweepArgs 
     constexpr int kT = WinShape<R>::T, kW = WinShape<R>::W, kNW = kT / 64;
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr int NV = MODE == 2 ? 32 : 16;
-    // RS states per re-layout round (the buffer holds RS states): FP32 4, FP64 2 or 1
+    // RS states per re-layout round (the buffer holds RS states): FP32 4, FP64 2 or 1.  With two
+    // per round the reverse sweep stores psi for the re-layout while the block's last lam update
+    // runs: with the HVP psi and dpsi form the first round and lam and dlam the second, with the
+    // gradient psi and lam share the round
+    constexpr bool kEarly = RS == 2 && MODE >= 1;
     const uint32_t S = 1u << a.L;
     const int tid0 = threadIdx.x, lane = tid0 & 63, wv = tid0 >> 6;
 
@@ -204,6 +229,7 @@ __global__ __launch_bounds__(WinShape<R>::T) void sweep_win_kernel(WinSweepArgs 
             for (int j = 0; j < (1 << kW); j++) DL.a[j] = {R(0), R(0)};
             for (int k = nb - 1; k >= 0; k--) {
                 const BlockRegs<kW> bk = load_block<kW>(a.blk_lds + k);
+                const uint32_t sb = lin<R, true>(deposit<kW>((uint32_t)tid, bk.pos));
                 for (int g = bk.g1 - 1; g >= bk.g0; g--) {
                     const int wcase = cptr(a.inst)[g].lo;
                     double* slot = stage + ((g - bk.g0) * kNW + wv) * 32;
@@ -228,6 +254,18 @@ __global__ __launch_bounds__(WinShape<R>::T) void sweep_win_kernel(WinSweepArgs 
 #undef MG_PB
                         stage8(ha, slot + 16, lane);
                     }
+                    if constexpr (kEarly) {
+                        // psi (and dpsi) are final for this block once the block's first gate has
+                        // uncomputed them: their re-layout stores overlap the last lam update
+                        if (g == bk.g0 && k > 0) {
+                            if constexpr (MODE == 2) {
+                                win_store<R, kW, true>(buf, sb, bk.off, P);
+                                win_store<R, kW, true>(buf + S, sb, bk.off, D);
+                            } else {
+                                win_store<R, kW, true>(buf + S, sb, bk.off, P);
+                            }
+                        }
+                    }
 #define MG_PC(WW, A, B) pass_c<R, WW, A, B, MODE>(V, Z, Lm, DL)
                     MG_WCASES(kW, MG_PC)
 #undef MG_PC
@@ -237,31 +275,33 @@ __global__ __launch_bounds__(WinShape<R>::T) void sweep_win_kernel(WinSweepArgs 
                 const bool move = k > 0;
                 BlockRegs<kW> pv = bk;
                 if (move) pv = load_block<kW>(a.blk_lds + k - 1);
-                const uint32_t sb = lin<R, true>(deposit<kW>((uint32_t)tid, bk.pos));
                 const uint32_t sn = lin<R, true>(deposit<kW>((uint32_t)tid, pv.pos));
+                if constexpr (kEarly && MODE == 2) {
+                    __syncthreads();
+                    win_rev_adds<kNW, NV>(a, bk.g0, bk.g1, stage, acc, wv, lane);
+                    if (move) {
+                        win_load<R, kW, true>(buf, sn, pv.off, P);
+                        if constexpr (MODE == 2) win_load<R, kW, true>(buf + S, sn, pv.off, D);
+                        __syncthreads();
+                        win_store<R, kW, true>(buf, sb, bk.off, Lm);
+                        if constexpr (MODE == 2) win_store<R, kW, true>(buf + S, sb, bk.off, DL);
+                        __syncthreads();
+                        win_load<R, kW, true>(buf, sn, pv.off, Lm);
+                        if constexpr (MODE == 2) win_load<R, kW, true>(buf + S, sn, pv.off, DL);
+                    }
+                    __syncthreads();
+                    continue;
+                }
                 if (move) {
                     win_store<R, kW, true>(buf, sb, bk.off, Lm);
-                    if constexpr (RS >= 2) win_store<R, kW, true>(buf + S, sb, bk.off, P);
+                    if constexpr (RS >= 2 && !kEarly) win_store<R, kW, true>(buf + S, sb, bk.off, P);
                     if constexpr (MODE == 2 && RS >= 4) {
                         win_store<R, kW, true>(buf + 2 * S, sb, bk.off, DL);
                         win_store<R, kW, true>(buf + 3 * S, sb, bk.off, D);
                     }
                 }
                 __syncthreads();
-                // the wave that owns a gate's parameter adds its sums: gates of one block that
-                // share a parameter (tied layers) are added by one wave, in order
-                for (int g = bk.g0; g < bk.g1; g++) {
-                    const int gparam = cptr(a.inst)[g].param, gflip = cptr(a.inst)[g].flip;
-                    if (wv == gparam % kNW && lane < NV) {
-                        const double* sg = stage + (size_t)(g - bk.g0) * kNW * 32;
-                        double s = 0.0;
-                        for (int w = 0; w < kNW; w++) s += sg[w * 32 + lane];
-                        const int ci = lane >> 1;
-                        int e = ci & 7;
-                        if (gflip && e < 4) e = 3 - e;
-                        acc[NV * gparam + 2 * ((ci & 8) | e) + (lane & 1)] += s;
-                    }
-                }
+                win_rev_adds<kNW, NV>(a, bk.g0, bk.g1, stage, acc, wv, lane);
                 if (move) {
                     win_load<R, kW, true>(buf, sn, pv.off, Lm);
                     if constexpr (RS >= 2) win_load<R, kW, true>(buf + S, sn, pv.off, P);
@@ -324,6 +364,25 @@ __device__ inline uint32_t scatter_c(uint32_t x, const int* p, int n)
 
 }  // namespace
 
+// the staged sums of a window block's gates into the accumulators (as win_rev_adds)
+template <int kNW, int NV>
+__device__ inline void win_tile_rev_adds(const WinTileArgs& a, int g0, int g1, const double* stage, double* acc, int wv,
+                                         int lane)
+{
+    for (int g = g0; g < g1; g++) {
+        const int gparam = cptr(a.inst)[g].param, gflip = cptr(a.inst)[g].flip;
+        if (wv == gparam % kNW && lane < NV) {
+            const double* sg = stage + (size_t)(g - g0) * kNW * 32;
+            double s = 0.0;
+            for (int w = 0; w < kNW; w++) s += sg[w * 32 + lane];
+            const int ci = lane >> 1;
+            int e = ci & 7;
+            if (gflip && e < 4) e = 3 - e;
+            acc[NV * gparam + 2 * ((ci & 8) | e) + (lane & 1)] += s;
+        }
+    }
+}
+
 // Columns of 2^L > 2^12 amplitudes in 12-qubit tiles (512 threads x 8 amplitudes in
 // registers), each tile block's gates applied in 3-qubit register windows.  MODE 0/1/2: f,
 // + gradient, + HVP against the dense target rows (states between tile blocks in the
@@ -336,6 +395,10 @@ __global__ __launch_bounds__(512) void sweep_win_tile_kernel(WinTileArgs a)
     constexpr int kT = 512, kW = 3, kNW = kT / 64, TB = 12;
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr int NV = MODE == 2 ? 32 : 16;
+    // psi stored for the reverse re-layout while the block's last lam update runs (as in
+    // sweep_win_kernel) only for the gradient: for the HVP the longer live ranges raise this
+    // kernel's spills (L=16 dense rows 2.17 -> 2.36 s measured)
+    constexpr bool kEarly = RS == 2 && MODE == 1;
     constexpr uint32_t TSZ = 1u << TB;  // tile amplitudes (the re-layout buffer's index space)
     const int L = a.L;
     const uint32_t S = 1u << L, ntiles = 1u << (L - TB);
@@ -486,6 +549,7 @@ __global__ __launch_bounds__(512) void sweep_win_tile_kernel(WinTileArgs a)
                     }
                     for (int k = k1 - 1; k >= k0; k--) {
                         const BlockRegs<kW> bk = load_block<kW>(a.blk_lds + k);
+                        const uint32_t sb = lin<R, true>(deposit<kW>((uint32_t)tid, bk.pos));
                         for (int g = bk.g1 - 1; g >= bk.g0; g--) {
                             const int wcase = cptr(a.inst)[g].lo;
                             double* slot = stage + ((g - bk.g0) * kNW + wv) * 32;
@@ -510,6 +574,18 @@ __global__ __launch_bounds__(512) void sweep_win_tile_kernel(WinTileArgs a)
 #undef MG_PB
                                 stage8(ha, slot + 16, lane);
                             }
+                            if constexpr (kEarly) {
+                                // psi (and dpsi) final for this window block: stored for the
+                                // re-layout while the last lam update runs
+                                if (g == bk.g0 && k > k0) {
+                                    if constexpr (MODE == 2) {
+                                        win_store<R, kW, true>(buf, sb, bk.off, P);
+                                        win_store<R, kW, true>(buf + TSZ, sb, bk.off, D);
+                                    } else {
+                                        win_store<R, kW, true>(buf + TSZ, sb, bk.off, P);
+                                    }
+                                }
+                            }
 #define MG_PC(WW, A, B) pass_c<R, WW, A, B, MODE>(V, Z, Lm, DL)
                             MG_WCASES(kW, MG_PC)
 #undef MG_PC
@@ -519,26 +595,29 @@ __global__ __launch_bounds__(512) void sweep_win_tile_kernel(WinTileArgs a)
                         const bool move = k > k0;
                         BlockRegs<kW> pv = bk;
                         if (move) pv = load_block<kW>(a.blk_lds + k - 1);
-                        const uint32_t sb = lin<R, true>(deposit<kW>((uint32_t)tid, bk.pos));
                         const uint32_t sn = lin<R, true>(deposit<kW>((uint32_t)tid, pv.pos));
+                        if constexpr (kEarly && MODE == 2) {
+                            __syncthreads();
+                            win_tile_rev_adds<kNW, NV>(a, bk.g0, bk.g1, stage, acc, wv, lane);
+                            if (move) {
+                                win_load<R, kW, true>(buf, sn, pv.off, P);
+                                win_load<R, kW, true>(buf + TSZ, sn, pv.off, D);
+                                __syncthreads();
+                                win_store<R, kW, true>(buf, sb, bk.off, Lm);
+                                win_store<R, kW, true>(buf + TSZ, sb, bk.off, DL);
+                                __syncthreads();
+                                win_load<R, kW, true>(buf, sn, pv.off, Lm);
+                                win_load<R, kW, true>(buf + TSZ, sn, pv.off, DL);
+                            }
+                            __syncthreads();
+                            continue;
+                        }
                         if (move) {
                             win_store<R, kW, true>(buf, sb, bk.off, Lm);
-                            if constexpr (RS >= 2) win_store<R, kW, true>(buf + TSZ, sb, bk.off, P);
+                            if constexpr (RS >= 2 && !kEarly) win_store<R, kW, true>(buf + TSZ, sb, bk.off, P);
                         }
                         __syncthreads();
-                        // one wave per parameter: gates sharing it are added in order
-                        for (int g = bk.g0; g < bk.g1; g++) {
-                            const int gparam = cptr(a.inst)[g].param, gflip = cptr(a.inst)[g].flip;
-                            if (wv == gparam % kNW && lane < NV) {
-                                const double* sg = stage + (size_t)(g - bk.g0) * kNW * 32;
-                                double s = 0.0;
-                                for (int w = 0; w < kNW; w++) s += sg[w * 32 + lane];
-                                const int ci = lane >> 1;
-                                int e = ci & 7;
-                                if (gflip && e < 4) e = 3 - e;
-                                acc[NV * gparam + 2 * ((ci & 8) | e) + (lane & 1)] += s;
-                            }
-                        }
+                        win_tile_rev_adds<kNW, NV>(a, bk.g0, bk.g1, stage, acc, wv, lane);
                         if (move) {
                             win_load<R, kW, true>(buf, sn, pv.off, Lm);
                             if constexpr (RS >= 2) win_load<R, kW, true>(buf + TSZ, sn, pv.off, P);
