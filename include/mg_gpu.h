@@ -265,9 +265,10 @@ void mg_set_row_chunk_bytes(long bytes);
  * automatic (default: the matrix cores, FP64 MFMA on 4-qubit windows), 0 the 12-qubit register
  * tiles of the tiled window kernel, 1 the matrix cores */
 void mg_set_materialize_kernel(int mode);
-/* FP32 tiled f and f + gradient sweeps (L >= 13): mode -1 automatic (default: the forward pass of each
- * tile block on the matrix cores, 4-qubit windows as two 8 x 8 parity blocks, FP32 MFMA), 0 gate by
- * gate on the VALU */
+/* Forward passes of FP32 sweeps on the matrix cores: the tiled f and f + gradient sweeps (L >= 13)
+ * and the f + gradient + HVP sweep with its states in LDS (L = 12).  Mode -1 automatic (default: the
+ * forward pass over 4-qubit windows, each window's gates as two 8 x 8 parity blocks, FP32 MFMA;
+ * for the HVP also dpsi <- M dpsi + N psi), 0 gate by gate on the VALU */
 void mg_set_forward_mfma(int mode);
 /* record HIP events around every sweep kernel on the library's stream */
 void mg_set_kernel_timing(int on);

@@ -156,19 +156,25 @@ using zc = std::complex<double>;
 
 // M <- G M for the gate with coefficients g (V: center block g0..g3 on {lo set, hi set}, corner
 // g4..g7 on {none, both}) on window-local bits la (its lower bit) and lb
+// a b + c d, written out (std::complex products call the library's IEEE-special-case routine,
+// which would dominate the host time of the per-evaluation window matrices)
+inline zc mul2(const cpx<double>& a, const zc& b, const cpx<double>& c, const zc& d)
+{
+    return {(a.re * b.real() - a.im * b.imag()) + (c.re * d.real() - c.im * d.imag()),
+            (a.re * b.imag() + a.im * b.real()) + (c.re * d.imag() + c.im * d.real())};
+}
+
 void apply_gate_left(zc* M, const cpx<double>* g, int la, int lb)
 {
-    const zc v[8] = {{g[0].re, g[0].im}, {g[1].re, g[1].im}, {g[2].re, g[2].im}, {g[3].re, g[3].im},
-                     {g[4].re, g[4].im}, {g[5].re, g[5].im}, {g[6].re, g[6].im}, {g[7].re, g[7].im}};
     for (int r = 0; r < 16; r++) {
         if (r & ((1 << la) | (1 << lb))) continue;
         const int i00 = r, i01 = r | (1 << la), i10 = r | (1 << lb), i11 = r | (1 << la) | (1 << lb);
         for (int c = 0; c < 16; c++) {
             const zc x = M[i00 * 16 + c], y = M[i01 * 16 + c], z = M[i10 * 16 + c], w = M[i11 * 16 + c];
-            M[i00 * 16 + c] = v[4] * x + v[5] * w;
-            M[i01 * 16 + c] = v[0] * y + v[1] * z;
-            M[i10 * 16 + c] = v[2] * y + v[3] * z;
-            M[i11 * 16 + c] = v[6] * x + v[7] * w;
+            M[i00 * 16 + c] = mul2(g[4], x, g[5], w);
+            M[i01 * 16 + c] = mul2(g[0], y, g[1], z);
+            M[i10 * 16 + c] = mul2(g[2], y, g[3], z);
+            M[i11 * 16 + c] = mul2(g[6], x, g[7], w);
         }
     }
 }
@@ -198,15 +204,18 @@ void push_fragments32(const zc* M, std::vector<float>* af)
 }  // namespace
 
 int forward_windows(const std::vector<Inst>& inst, const std::vector<cpx<double>>& coef, std::vector<int>* order,
-                    std::vector<FWin>* wins, std::vector<float>* af)
+                    std::vector<FWin>* wins, std::vector<float>* af, bool hvp)
 {
     constexpr int TB = 12, W = 4;
     std::vector<WBlock> wbs;
     window_blocks(inst, TB, W, 1 << 20, order, &wbs);
     if (order->size() != inst.size()) return -1;
     for (const WBlock& wb : wbs) {
-        zc M[256];
-        for (int i = 0; i < 256; i++) M[i] = (i / 16 == i % 16) ? 1.0 : 0.0;
+        zc M[256], N[256], T[256];
+        for (int i = 0; i < 256; i++) {
+            M[i] = (i / 16 == i % 16) ? 1.0 : 0.0;
+            N[i] = 0.0;
+        }
         for (int k = wb.g0; k < wb.g1; k++) {
             const Inst& l = inst[(*order)[k]];
             int la = -1, lb = -1;
@@ -215,7 +224,15 @@ int forward_windows(const std::vector<Inst>& inst, const std::vector<cpx<double>
                 if (wb.pos[i] == l.hi) lb = i;
             }
             if (la < 0 || lb < 0) return -1;
-            apply_gate_left(M, &coef[(size_t)(*order)[k] * kCoefPerInst], la, lb);
+            const cpx<double>* g = &coef[(size_t)(*order)[k] * kCoefPerInst];
+            if (hvp) {
+                // N <- V N + Z M (with the M before this gate), then M <- V M
+                std::copy(M, M + 256, T);
+                apply_gate_left(T, g + 16, la, lb);
+                apply_gate_left(N, g, la, lb);
+                for (int i = 0; i < 256; i++) N[i] += T[i];
+            }
+            apply_gate_left(M, g, la, lb);
         }
         FWin fw{};
         int oth[8], no = 0;
@@ -234,6 +251,7 @@ int forward_windows(const std::vector<Inst>& inst, const std::vector<cpx<double>
         }
         wins->push_back(fw);
         push_fragments32(M, af);
+        if (hvp) push_fragments32(N, af);
     }
     return (int)wbs.size();
 }

@@ -48,9 +48,13 @@ struct FWin {
 
 // `inst` (tile-local bit positions) and `coef` (V first in each instance's coefficients) are one
 // tile block's gates in program order; fills `order` with a window order of them, appends the
-// windows and their FP32 fragments [window][8][lane] and returns the number of windows
+// windows and their FP32 fragments [window][8][lane] and returns the number of windows.  With
+// `hvp`, the forward pass of the Hessian-vector product, whose direction dpsi follows
+// dpsi <- V dpsi + Z psi per gate (Z the gate's direction coefficients, 16..23): per window
+// dpsi_out = M dpsi_in + N psi_in with N = sum_i V_k..V_{i+1} Z_i V_{i-1}..V_1, which keeps parity
+// like M; the fragments are then [window][16][lane], M's 8 then N's 8.
 int forward_windows(const std::vector<Inst>& inst, const std::vector<cpx<double>>& coef, std::vector<int>* order,
-                    std::vector<FWin>* wins, std::vector<float>* af);
+                    std::vector<FWin>* wins, std::vector<float>* af, bool hvp = false);
 
 struct MatArgs {
     int L, ntb;

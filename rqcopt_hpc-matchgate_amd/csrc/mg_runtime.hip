@@ -873,6 +873,42 @@ static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, in
     a.ckpt = (cpx<R>*)d_ckpt;
     a.nstate = reg_T ? reg_nst : nstate;
     a.lds_state_offset = (int)(reg_T ? head_reg : head);
+    // FP32 sweeps with the states in LDS at L = 12: the forward pass (psi, and dpsi for the HVP) on
+    // the matrix cores over 4-qubit windows; the instances go to the kernel in the windows' order (a
+    // topological order of the program: the same product, and the reverse sweep runs it backwards)
+    if (fp32 && lds_states && !ckpt && !reg_T && !use_win && !use_tile && L == 12 && kn.fwd_mfma != 0 && NW <= 16) {
+        std::vector<int> o2;
+        std::vector<FWin> fw;
+        std::vector<float> fa;
+        if (forward_windows(b.inst, b.coef, &o2, &fw, &fa, mode == 2) < 0 || o2.size() != b.inst.size()) {
+            set_error("LDS sweep: forward window blocking failed");
+            return -1;
+        }
+        std::vector<Inst> wi(o2.size());
+        std::vector<cpx<R>> wc(o2.size() * kCoefPerInst);
+        for (size_t k = 0; k < o2.size(); k++) {
+            wi[k] = b.inst[o2[k]];
+            for (int e = 0; e < kCoefPerInst; e++) {
+                const cpx<double> v = b.coef[(size_t)o2[k] * kCoefPerInst + e];
+                wc[k * kCoefPerInst + e] = {(R)v.re, (R)v.im};
+            }
+        }
+        void *d_wi = nullptr, *d_wc = nullptr, *d_fw = nullptr, *d_fa = nullptr;
+        if (ar.get(c->w_inst, wi.size() * sizeof(Inst), &d_wi) < 0 || ar.get(c->w_coef, wc.size() * sizeof(cpx<R>), &d_wc) < 0 ||
+            ar.get(c->w_fwin, fw.size() * sizeof(FWin), &d_fw) < 0 || ar.get(c->w_fwaf, fa.size() * sizeof(float), &d_fa) < 0)
+            return -1;
+        MG_HIP(hipMemcpyAsync(d_wi, wi.data(), wi.size() * sizeof(Inst), hipMemcpyHostToDevice, s));
+        MG_HIP(hipMemcpyAsync(d_wc, wc.data(), wc.size() * sizeof(cpx<R>), hipMemcpyHostToDevice, s));
+        MG_HIP(hipMemcpyAsync(d_fw, fw.data(), fw.size() * sizeof(FWin), hipMemcpyHostToDevice, s));
+        MG_HIP(hipMemcpyAsync(d_fa, fa.data(), fa.size() * sizeof(float), hipMemcpyHostToDevice, s));
+        // the copies read pageable host vectors: complete them before the vectors go away
+        MG_HIP(hipStreamSynchronize(s));
+        a.inst = (const Inst*)d_wi;
+        a.coef = (const cpx<R>*)d_wc;
+        a.fwin = d_fw;
+        a.fwaf = (const float*)d_fa;
+        a.nwin = (int)fw.size();
+    }
 
     WinSweepArgs wa{};
     if (use_win) {

@@ -125,7 +125,7 @@ struct Knobs {
     bool time_kernels = false;
     size_t row_chunk_bytes = 0;  // materialised target rows per column chunk (0: 4 GiB)
     int mat_kernel = -1;         // target rows beyond 12 qubits: -1 auto, 0 register tiles, 1 matrix cores
-    int fwd_mfma = -1;           // FP32 tiled sweeps: forward pass on the matrix cores (-1 auto, 0 gate by gate)
+    int fwd_mfma = -1;           // FP32 forward passes on the matrix cores (tiled f, f + gradient; L=12 HVP in LDS): -1 auto, 0 gate by gate
 };
 Knobs& knobs();
 

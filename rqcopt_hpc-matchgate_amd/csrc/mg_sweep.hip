@@ -18,6 +18,7 @@
 // Workgroup partials are summed in block order by reduce_partials: results are
 // bitwise reproducible run to run.
 #include "mg_device.hpp"
+#include "mg_fwd_mfma.hpp"
 #include "mg_quad.hpp"
 #include "mg_sweep.hpp"
 
@@ -91,6 +92,15 @@ __global__ __launch_bounds__(T) void sweep_kernel(SweepArgs<R> a)
             if constexpr (MODE == 2) sel<R, LDS && !CKPT>(d0, i) = {R(0), R(0)};
         }
         __syncthreads();
+        if constexpr (sizeof(R) == 4 && LDS && !CKPT) {
+            if (a.fwin) {
+                if constexpr (MODE == 2)
+                    fwd_windows_hvp<NW>(reinterpret_cast<const FWin*>(a.fwin), a.fwaf, 0, a.nwin, psi, dpsi, lane, wv);
+                else
+                    fwd_windows<NW>(reinterpret_cast<const FWin*>(a.fwin), a.fwaf, 0, a.nwin, psi, lane, wv);
+                goto forward_done;
+            }
+        }
         for (int g = 0; g < a.G; g++) {
             const Inst in = a.inst[g];
             cpx<R> cf[24];
@@ -112,6 +122,7 @@ __global__ __launch_bounds__(T) void sweep_kernel(SweepArgs<R> a)
                 fwd_quads<R, MODE, LDS && !CKPT, 1>(q, T, in.lo, in.hi, olo, ohi, cf, pin, pout, din, dout);
             __syncthreads();
         }
+    forward_done:
         // ---- f_b = s_b . psi_G -----------------------------------------------------------
         {
             const cpx<R>* pG = CKPT ? ck + (size_t)2 * a.G * S : psi;

@@ -22,6 +22,12 @@ struct SweepArgs {
     cpx<R>* ckpt;           // checkpoints [gridDim.x][G+1][2][S] (checkpoint mode)
     int nstate;             // state vectors per workgroup
     int lds_state_offset;   // byte offset of the states inside dynamic LDS (LDS mode)
+    // FP32 sweeps with the states in LDS at L = 12: the forward pass on the matrix cores over nwin
+    // 4-qubit windows (struct FWin; fragments [window][8][lane], or [window][16][lane] for the HVP;
+    // the instances in window order), or none (fwin null: gate by gate)
+    const void* fwin;
+    const float* fwaf;
+    int nwin;
 };
 
 // doubles of LDS bookkeeping in front of the states: acc [P][32] (sweep_kernel: [P][16] below

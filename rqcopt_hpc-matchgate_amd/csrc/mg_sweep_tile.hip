@@ -20,6 +20,7 @@
 // src/matchgate_brickwall.c:52-78, backward kernel src/gate.c:659-772.
 #include <algorithm>
 
+#include "mg_fwd_mfma.hpp"
 #include "mg_mat_mfma.hpp"
 #include "mg_quad.hpp"
 #include "mg_sweep_tile.hpp"
@@ -70,65 +71,9 @@ __device__ inline void tile_map(const TBlock* blk, int tid, TileMap<PER>* m)
 }  // namespace
 
 // The forward pass of one FP32 tile over the block's 4-qubit windows on the matrix cores (f and f
-// + gradient: psi only): psi <- M psi per window, in place in the LDS tile.  The tile's 16 column
-// blocks (16 columns of the 256 that share the window's complement) are shared by the NW waves;
-// each wave reads all its operands of a window before writing (as the materialisation kernel),
-// then one barrier per window.  (The HVP's forward pass, psi and dpsi <- M dpsi + N psi, measured
-// L=16 1543 -> 1505 ms but L=14 depth 15 108.7 -> 112.3 ms: compiled into the HVP kernel it raises
-// the reverse pass's register spills, so the HVP stays gate by gate.)
-typedef float v4f __attribute__((ext_vector_type(4)));
-
-template <int NW>
-__device__ inline void fwd_windows(const TileArgs<float>& a, const TBlock* bk, cpx<float>* t_psi, int lane, int wv)
-{
-    constexpr int CB = 16 / NW;  // column blocks per wave
-    const FWin* fwin = reinterpret_cast<const FWin*>(a.fwin);
-    const int n0 = lane >> 4, col = lane & 15;
-    for (int w = bk->w0; w < bk->w1; w++) {
-        const FWin* fw = fwin + w;
-        float am[8];
-#pragma unroll
-        for (int f = 0; f < 8; f++) am[f] = a.fwaf[((size_t)w * 8 + f) * 64 + lane];
-        const uint32_t oj = (uint32_t)fw->offj[col];
-        uint32_t base[CB];
-        cpx<float> x[CB][2][2];
-#pragma unroll
-        for (int c = 0; c < CB; c++) {
-            base[c] = oj ^ (uint32_t)fw->offcb[wv * CB + c];
-#pragma unroll
-            for (int p = 0; p < 2; p++)
-#pragma unroll
-                for (int h = 0; h < 2; h++) {
-                    x[c][p][h] = t_psi[base[c] ^ (uint32_t)fw->offp[p][4 * h + n0]];
-                }
-        }
-        v4f ap[CB][2];
-#pragma unroll
-        for (int c = 0; c < CB; c++)
-#pragma unroll
-            for (int p = 0; p < 2; p++) ap[c][p] = v4f{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int kk = 0; kk < 4; kk++)
-#pragma unroll
-            for (int c = 0; c < CB; c++)
-#pragma unroll
-                for (int p = 0; p < 2; p++) {
-                    const cpx<float>& xv = x[c][p][kk >> 1];
-                    ap[c][p] = __builtin_amdgcn_mfma_f32_16x16x4f32(am[4 * p + kk], (kk & 1) ? xv.im : xv.re, ap[c][p],
-                                                                    0, 0, 0);
-                }
-        // this lane's results: entries 2 n0 and 2 n0 + 1 of its column
-#pragma unroll
-        for (int c = 0; c < CB; c++)
-#pragma unroll
-            for (int p = 0; p < 2; p++) {
-                const uint32_t o0 = base[c] ^ (uint32_t)fw->offp[p][2 * n0], o1 = base[c] ^ (uint32_t)fw->offp[p][2 * n0 + 1];
-                t_psi[o0] = {ap[c][p][0], ap[c][p][1]};
-                t_psi[o1] = {ap[c][p][2], ap[c][p][3]};
-            }
-        __syncthreads();
-    }
-}
+// + gradient: psi only; fwd_windows, mg_fwd_mfma.hpp).  (The HVP's forward pass, psi and dpsi <-
+// M dpsi + N psi, measured L=16 1543 -> 1505 ms but L=14 depth 15 108.7 -> 112.3 ms: compiled into
+// the HVP kernel it raises the reverse pass's register spills, so the HVP stays gate by gate.)
 
 template <typename R, int MODE>
 __global__ __launch_bounds__((1 << tile_lt<R, MODE>()), (tile_min_waves<R, MODE>())) void sweep_tile_kernel(TileArgs<R> a)
@@ -206,7 +151,7 @@ __global__ __launch_bounds__((1 << tile_lt<R, MODE>()), (tile_min_waves<R, MODE>
                 __syncthreads();
                 if constexpr (sizeof(R) == 4 && MODE <= 1) {
                     if (a.fwin) {
-                        fwd_windows<NW>(a, bk, t_psi, lane, wv);
+                        fwd_windows<NW>(reinterpret_cast<const FWin*>(a.fwin), a.fwaf, bk->w0, bk->w1, t_psi, lane, wv);
                         goto forward_done;
                     }
                 }

@@ -250,6 +250,41 @@ def test_window_register_kernel_matches_oracle(mg, layout, depth, prec):
     assert abs(r1[0] - fo) / abs(fo) < tol and rel(r1[1], dgo) < tol and abs(r0[0] - fo) / abs(fo) < tol
 
 
+@pytest.mark.parametrize("layout,depth", [("tied", 3), ("open", 9), ("open", 15)])
+def test_lds_kernel_forward_on_matrix_cores(mg, layout, depth):
+    """FP32 L=12 f, f + gradient and f + gradient + HVP on the LDS-resident kernel: the forward pass
+    (psi, and dpsi for the HVP) over 4-qubit windows on the matrix cores (the default) and gate by
+    gate on the VALU (mg_set_forward_mfma(0)), both against the oracle (2e-5); tied brickwalls with
+    random permutations give windows of scattered qubits.  The two paths round differently, so
+    equal bits would mean the matrix cores were not used."""
+    L = 12
+    say(f"case: LDS kernel FP32 L={L} {layout} depth={depth}, forward pass on the matrix cores")
+    V, Z, param, wires, P, U = build_case(L, layout, depth, seed=depth * 11 + 4)
+    t = mg.Target.dense(U)
+    p = mg.Program.create(L, P, param, wires)
+
+    def run():
+        r = mg.evaluate_columns(t, p, V, Z, mg.EVAL_F_GRAD_HVP, mg.MG_FP32, 7, 71)
+        info = mg.last_sweep_info()
+        assert info["kernel"] == "sweep_kernel" and info["lds_states"], info
+        r1 = mg.evaluate_columns(t, p, V, None, mg.EVAL_F_GRAD, mg.MG_FP32, 7, 71)
+        assert mg.last_sweep_info()["kernel"] == "sweep_kernel"
+        r0 = mg.evaluate_columns(t, p, V, None, mg.EVAL_F, mg.MG_FP32, 7, 71)
+        return r, r1, r0
+
+    r, r1, r0 = run()
+    mg.set_forward_mfma(0)
+    try:
+        v, v1, v0 = run()
+    finally:
+        mg.set_forward_mfma(-1)
+    fo, dgo, dho = orc.target_gradient_hvp(U, V, Z, param, wires, L, 7, 71)
+    for x, x1, x0 in ((r, r1, r0), (v, v1, v0)):
+        assert abs(x[0] - fo) / abs(fo) < 2e-5 and rel(x[1], dgo) < 2e-5 and rel(x[2], dho) < 2e-5
+        assert abs(x1[0] - fo) / abs(fo) < 2e-5 and rel(x1[1], dgo) < 2e-5 and abs(x0[0] - fo) / abs(fo) < 2e-5
+    assert np.any(r[2] != v[2]) and np.any(r1[1] != v1[1]) and r0[0] != v0[0]
+
+
 @pytest.mark.parametrize("depth", [5, 15])
 def test_lds_kernel_f_and_gradient_two_workgroups_per_cu(mg, depth):
     """FP32 L=12 f and f + gradient on the LDS-resident kernel: 256-thread workgroups whose
