@@ -42,10 +42,6 @@ struct Mfma<float> {
     static __device__ inline v4 mac(float a, float b, v4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
 };
 
-// LDS index of tile entry j: the bits above the lowest four folded onto them, so that the
-// columns of a block (any 4 of the tile's bits) fall in different banks; linear over XOR
-__host__ __device__ inline uint32_t mat_swz(uint32_t j) { return j ^ ((j >> 4) & 15u) ^ ((j >> 8) & 15u); }
-
 __device__ inline uint32_t scatter(uint32_t x, const int* pos, int n)
 {
     uint32_t r = 0;
@@ -179,6 +175,58 @@ void apply_gate_left(zc* M, const cpx<double>* g, int la, int lb)
     }
 }
 
+// M <- V M and, with N, N <- V N + Z M (the M before the gate): v, z the gate's coefficients, on
+// the 16 x 16 matrices of a window, which conserve parity (only entries whose row and column have
+// the same parity can be nonzero, and the gate keeps it so: a column's entries of the other parity
+// stay zero and are not computed)
+void apply_gate_pair(zc* M, zc* N, const cpx<double>* v, const cpx<double>* z, int la, int lb)
+{
+    // columns of each parity
+    static const struct Cols {
+        int c[2][8];
+        Cols()
+        {
+            int n[2] = {0, 0};
+            for (int c0 = 0; c0 < 16; c0++) {
+                const int q = __builtin_popcount(c0) & 1;
+                c[q][n[q]++] = c0;
+            }
+        }
+    } cols;
+    for (int r = 0; r < 16; r++) {
+        if (r & ((1 << la) | (1 << lb))) continue;
+        const int pr = __builtin_popcount(r) & 1;
+        // rows i00, i11 keep the parity of r and pair with its columns; rows i01, i10 the others
+        for (int h = 0; h < 2; h++) {
+            const int ia = h == 0 ? r : (r | (1 << la)), ib = h == 0 ? (r | (1 << la) | (1 << lb)) : (r | (1 << lb));
+            const cpx<double>* va = h == 0 ? v + 4 : v;  // (v4, v5 | v6, v7) or (v0, v1 | v2, v3)
+            const cpx<double>* za = h == 0 ? z + 4 : z;
+            const int* cl = cols.c[h == 0 ? pr : 1 - pr];
+            zc* Ma = M + ia * 16;
+            zc* Mb = M + ib * 16;
+            if (N) {
+                zc* Na = N + ia * 16;
+                zc* Nb = N + ib * 16;
+                for (int k = 0; k < 8; k++) {
+                    const int c = cl[k];
+                    const zc x = Ma[c], w = Mb[c], nx = Na[c], nw = Nb[c];
+                    Na[c] = mul2(va[0], nx, va[1], nw) + mul2(za[0], x, za[1], w);
+                    Nb[c] = mul2(va[2], nx, va[3], nw) + mul2(za[2], x, za[3], w);
+                    Ma[c] = mul2(va[0], x, va[1], w);
+                    Mb[c] = mul2(va[2], x, va[3], w);
+                }
+            } else {
+                for (int k = 0; k < 8; k++) {
+                    const int c = cl[k];
+                    const zc x = Ma[c], w = Mb[c];
+                    Ma[c] = mul2(va[0], x, va[1], w);
+                    Mb[c] = mul2(va[2], x, va[3], w);
+                }
+            }
+        }
+    }
+}
+
 }  // namespace
 
 namespace {
@@ -188,18 +236,48 @@ int tile_swz32(uint32_t a) { return (int)(a ^ (((a >> 4) & 1u) * 5u) ^ (((a >> 5
 
 // lane fragments of one parity block of a 16 x 16 window matrix, FP32 result layout (rows 4 g ..
 // 4 g + 3: real and imaginary parts of entries 2 g and 2 g + 1)
-void push_fragments32(const zc* M, std::vector<float>* af)
+// the 512 A-fragment values of a window matrix (2 parities x 4 kk x 64 lanes): matrix entry and
+// which part (0: re, 1: -im, 2: im), for the FP64 (w = 0) and FP32 (w = 1) result layouts
+struct FragTable {
+    short entry[2][512];
+    unsigned char part[2][512];
+    FragTable()
+    {
+        for (int w = 0; w < 2; w++)
+            for (int p = 0; p < 2; p++)
+                for (int kk = 0; kk < 4; kk++)
+                    for (int ln = 0; ln < 64; ln++) {
+                        const int i = (p * 4 + kk) * 64 + ln;
+                        const int r = ln % 16, c = 4 * kk + ln / 16;
+                        const int n = (c / 8) * 4 + c % 4, q = (c / 4) % 2;
+                        const int m = w == 0 ? 2 * (r % 4) + (r / 4) / 2 : 2 * (r / 4) + (r % 4) / 2;
+                        const int pt = w == 0 ? (r / 4) % 2 : (r % 4) % 2;
+                        entry[w][i] = (short)(kParIdx[p][m] * 16 + kParIdx[p][n]);
+                        part[w][i] = (unsigned char)(pt == 0 ? (q == 0 ? 0 : 1) : (q == 0 ? 2 : 0));
+                    }
+    }
+};
+const FragTable& frag_table()
 {
-    for (int p = 0; p < 2; p++)
-        for (int kk = 0; kk < 4; kk++)
-            for (int ln = 0; ln < 64; ln++) {
-                const int r = ln % 16, c = 4 * kk + ln / 16;
-                const int n = (c / 8) * 4 + c % 4, q = (c / 4) % 2;
-                const int m = 2 * (r / 4) + (r % 4) / 2, part = (r % 4) % 2;
-                const zc e = M[kParIdx[p][m] * 16 + kParIdx[p][n]];
-                af->push_back((float)(part == 0 ? (q == 0 ? e.real() : -e.imag()) : (q == 0 ? e.imag() : e.real())));
-            }
+    static const FragTable t;
+    return t;
 }
+
+template <typename T>
+void push_fragments(const zc* M, int w, std::vector<T>* af)
+{
+    const FragTable& t = frag_table();
+    const size_t n0 = af->size();
+    af->resize(n0 + 512);
+    T* o = af->data() + n0;
+    for (int i = 0; i < 512; i++) {
+        const zc e = M[t.entry[w][i]];
+        const int pt = t.part[w][i];
+        o[i] = (T)(pt == 0 ? e.real() : pt == 1 ? -e.imag() : e.imag());
+    }
+}
+
+void push_fragments32(const zc* M, std::vector<float>* af) { push_fragments<float>(M, 1, af); }
 
 }  // namespace
 
@@ -211,7 +289,7 @@ int forward_windows(const std::vector<Inst>& inst, const std::vector<cpx<double>
     window_blocks(inst, TB, W, 1 << 20, order, &wbs);
     if (order->size() != inst.size()) return -1;
     for (const WBlock& wb : wbs) {
-        zc M[256], N[256], T[256];
+        zc M[256], N[256];
         for (int i = 0; i < 256; i++) {
             M[i] = (i / 16 == i % 16) ? 1.0 : 0.0;
             N[i] = 0.0;
@@ -225,14 +303,7 @@ int forward_windows(const std::vector<Inst>& inst, const std::vector<cpx<double>
             }
             if (la < 0 || lb < 0) return -1;
             const cpx<double>* g = &coef[(size_t)(*order)[k] * kCoefPerInst];
-            if (hvp) {
-                // N <- V N + Z M (with the M before this gate), then M <- V M
-                std::copy(M, M + 256, T);
-                apply_gate_left(T, g + 16, la, lb);
-                apply_gate_left(N, g, la, lb);
-                for (int i = 0; i < 256; i++) N[i] += T[i];
-            }
-            apply_gate_left(M, g, la, lb);
+            apply_gate_pair(M, hvp ? N : nullptr, g, g + 16, la, lb);
         }
         FWin fw{};
         int oth[8], no = 0;
@@ -252,6 +323,61 @@ int forward_windows(const std::vector<Inst>& inst, const std::vector<cpx<double>
         wins->push_back(fw);
         push_fragments32(M, af);
         if (hvp) push_fragments32(N, af);
+    }
+    return (int)wbs.size();
+}
+
+namespace {
+
+// lane fragments of one parity block of a 16 x 16 window matrix, FP64 result layout (rows g, g + 4,
+// g + 8, g + 12 of lane group g: real and imaginary parts of entries 2 g and 2 g + 1)
+void push_fragments64(const zc* M, std::vector<double>* af) { push_fragments<double>(M, 0, af); }
+
+}  // namespace
+
+int forward_windows64(const std::vector<Inst>& inst, const std::vector<cpx<double>>& coef, std::vector<DWin>* wins,
+                      std::vector<double>* af, bool hvp)
+{
+    constexpr int TB = 12, W = 4;
+    std::vector<int> order;
+    std::vector<WBlock> wbs;
+    window_blocks(inst, TB, W, 1 << 20, &order, &wbs);
+    if (order.size() != inst.size()) return -1;
+    for (const WBlock& wb : wbs) {
+        zc M[256], N[256];
+        for (int i = 0; i < 256; i++) {
+            M[i] = (i / 16 == i % 16) ? 1.0 : 0.0;
+            N[i] = 0.0;
+        }
+        for (int k = wb.g0; k < wb.g1; k++) {
+            const Inst& l = inst[order[k]];
+            int la = -1, lb = -1;
+            for (int i = 0; i < W; i++) {
+                if (wb.pos[i] == l.lo) la = i;
+                if (wb.pos[i] == l.hi) lb = i;
+            }
+            if (la < 0 || lb < 0) return -1;
+            const cpx<double>* g = &coef[(size_t)order[k] * kCoefPerInst];
+            apply_gate_pair(M, hvp ? N : nullptr, g, g + 16, la, lb);
+        }
+        DWin dw{};
+        int oth[8], no = 0;
+        for (int q = 0; q < TB; q++)
+            if (std::find(wb.pos, wb.pos + W, q) == wb.pos + W) oth[no++] = q;
+        auto put = [&](uint32_t x, const int* pos, int n) {
+            uint32_t r = 0;
+            for (int i = 0; i < n; i++) r |= ((x >> i) & 1u) << pos[i];
+            return (int)mat_swz(r);
+        };
+        for (int p = 0; p < 2; p++)
+            for (int n = 0; n < 8; n++) dw.offp[p][n] = put((uint32_t)kParIdx[p][n], wb.pos, W);
+        for (int j = 0; j < 16; j++) {
+            dw.offj[j] = put((uint32_t)j, oth, 4);
+            dw.offcb[j] = put((uint32_t)j, oth + 4, 4);
+        }
+        wins->push_back(dw);
+        push_fragments64(M, af);
+        if (hvp) push_fragments64(N, af);
     }
     return (int)wbs.size();
 }

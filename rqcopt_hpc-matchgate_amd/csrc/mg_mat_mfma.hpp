@@ -12,6 +12,10 @@ namespace mg {
 // as a real 16 x 16 matrix over (real, imaginary) parts, is one FP64 MFMA tile.  Offsets are
 // indices into the LDS tile (12 bits), already swizzled (the swizzle is linear over XOR, so the
 // index of window entry n of column j of column block cb is offp ^ offj ^ offcb).
+// LDS index of tile entry j: the bits above the lowest four folded onto them, so that the
+// columns of a block (any 4 of the tile's bits) fall in different banks; linear over XOR
+__host__ __device__ inline uint32_t mat_swz(uint32_t j) { return j ^ ((j >> 4) & 15u) ^ ((j >> 8) & 15u); }
+
 struct DWin {
     int offp[2][8];  // parity block p, parity index n (window indices ascending within a parity)
     int offj[16];    // column j of a column block (the 4 lowest of the 8 bits outside the window)
@@ -55,6 +59,15 @@ struct FWin {
 // like M; the fragments are then [window][16][lane], M's 8 then N's 8.
 int forward_windows(const std::vector<Inst>& inst, const std::vector<cpx<double>>& coef, std::vector<int>* order,
                     std::vector<FWin>* wins, std::vector<float>* af, bool hvp = false);
+
+// The forward pass of an FP64 sweep of a 12-qubit column (L = 12) over 4-qubit windows, as
+// forward_windows but in the FP64 MFMA form of the materialisation (struct DWin, offsets into an
+// LDS column laid out by mat_swz; fragments [window][8][lane] doubles, or [window][16][lane] with
+// `hvp`: M's 8 then N's 8).  The window order of the program's gates is the one the matrices are
+// formed in; the sweep's reverse pass keeps its own gate order (any topological order of the
+// program has the same product).  Returns the number of windows, or -1.
+int forward_windows64(const std::vector<Inst>& inst, const std::vector<cpx<double>>& coef, std::vector<DWin>* wins,
+                      std::vector<double>* af, bool hvp);
 
 struct MatArgs {
     int L, ntb;

@@ -940,6 +940,24 @@ static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, in
         wa.buf_off = (int)win_buf;
         wa.rs = win_rs;
         wa.slots = win_slots;
+        // FP64: the forward pass (psi, and dpsi for the HVP) on the matrix cores over 4-qubit
+        // windows of the column held in the re-layout buffer (the HVP needs two states per round)
+        std::vector<DWin> fw;
+        std::vector<double> fa;
+        if (!fp32 && L == 12 && kn.fwd_mfma != 0 && (mode < 2 || win_rs >= 2)) {
+            if (forward_windows64(b.inst, b.coef, &fw, &fa, mode == 2) < 0) {
+                set_error("window sweep: forward window blocking failed");
+                return -1;
+            }
+            void *d_fw = nullptr, *d_fa = nullptr;
+            if (ar.get(c->w_fwin, fw.size() * sizeof(DWin), &d_fw) < 0 || ar.get(c->w_fwaf, fa.size() * sizeof(double), &d_fa) < 0)
+                return -1;
+            MG_HIP(hipMemcpyAsync(d_fw, fw.data(), fw.size() * sizeof(DWin), hipMemcpyHostToDevice, s));
+            MG_HIP(hipMemcpyAsync(d_fa, fa.data(), fa.size() * sizeof(double), hipMemcpyHostToDevice, s));
+            wa.fwin = d_fw;
+            wa.fwaf = (const double*)d_fa;
+            wa.nfwin = (int)fw.size();
+        }
         // the copies above read pageable host vectors: complete them before w goes away
         MG_HIP(hipStreamSynchronize(s));
     }

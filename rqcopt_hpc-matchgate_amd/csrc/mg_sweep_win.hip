@@ -20,6 +20,7 @@
 //
 // Reference: per-column loop src/matchgate_target.c:788-837, forward
 // src/matchgate_brickwall.c:52-78, backward kernel src/gate.c:659-772.
+#include "mg_fwd_mfma.hpp"
 #include "mg_sweep_win.hpp"
 #include "mg_win.hpp"
 
@@ -151,6 +152,32 @@ __global__ __launch_bounds__(WinShape<R>::T) void sweep_win_kernel(WinSweepArgs 
         asm volatile("" : "+v"(tid));
         Win<R, kW> Lm, P, DL, D;
         // ---- forward: psi, dpsi ------------------------------------------------------------
+        if constexpr (sizeof(R) == 8 && (MODE < 2 || RS >= 2)) {
+            if (a.fwin) {
+                // on the matrix cores, the column (and its derivative) in the re-layout buffer, then
+                // into registers in the last window block's tuple order
+                cpx<R>* cpsi = buf;
+                cpx<R>* cdpsi = buf + S;
+                for (uint32_t j = (uint32_t)tid; j < S; j += kT) {
+                    cpsi[mat_swz(j)] = {j == (uint32_t)b ? R(1) : R(0), R(0)};
+                    if constexpr (MODE == 2) cdpsi[mat_swz(j)] = {R(0), R(0)};
+                }
+                __syncthreads();
+                fwd_windows64<kNW, MODE == 2>(reinterpret_cast<const DWin*>(a.fwin), a.fwaf, a.nfwin, cpsi, cdpsi, lane,
+                                              wv);
+                const BlockRegs<kW> bl = load_block<kW>(a.blk_glob + nb - 1);
+                const uint32_t base = deposit<kW>((uint32_t)tid, bl.pos);
+#pragma unroll
+                for (int j = 0; j < (1 << kW); j++) {
+                    const uint32_t o = mat_swz(base | bl.off[j]);
+                    P.a[j] = cpsi[o];
+                    if constexpr (MODE == 2) D.a[j] = cdpsi[o];
+                }
+                // the reverse sweep's re-layouts reuse the buffer
+                __syncthreads();
+                goto forward_done;
+            }
+        }
         {
             const BlockRegs<kW> b0 = load_block<kW>(a.blk_glob);
             const uint32_t base = deposit<kW>((uint32_t)tid, b0.pos);
@@ -189,6 +216,7 @@ __global__ __launch_bounds__(WinShape<R>::T) void sweep_win_kernel(WinSweepArgs 
                 }
             }
         }
+    forward_done:
         // ---- seed lam = -conj(U e_b) in the last block's tuple order (after the forward sweep,
         //      which does not need it) ---------------------------------------------------------
         {

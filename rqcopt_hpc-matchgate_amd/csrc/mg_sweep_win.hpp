@@ -19,6 +19,12 @@ struct WinSweepArgs {
     int acc_off, buf_off;      // byte offsets in dynamic LDS
     int rs;                    // states per re-layout round (FP32: 4; FP64: 2, or 1 for many parameters)
     int slots;                 // staging slots: gates of the largest window block (<= kWinRegGates)
+    // FP64: the forward pass on the matrix cores over nfwin 4-qubit windows of the column held in
+    // the re-layout buffer (struct DWin, fragments [window][8 or 16][lane], forward_windows64;
+    // psi, and dpsi for the HVP, which needs two states per round), or none (fwin null)
+    const void* fwin;
+    const double* fwaf;
+    int nfwin;
 };
 
 // Tiled variant (FP64, L > 12): the program is cut into tile blocks acting inside 12-qubit

@@ -251,6 +251,43 @@ def test_window_register_kernel_matches_oracle(mg, layout, depth, prec):
 
 
 @pytest.mark.parametrize("layout,depth", [("tied", 3), ("open", 9), ("open", 15)])
+def test_window_kernel_forward_on_matrix_cores(mg, layout, depth):
+    """FP64 L=12 f, f + gradient and f + gradient + HVP on the register-window kernel: the forward
+    pass over 4-qubit windows on the FP64 matrix cores, the column in the re-layout buffer (the
+    default; for the HVP only when two states fit a re-layout round), and gate by gate in registers
+    (mg_set_forward_mfma(0)), both against the oracle (1e-11).  Equal bits would mean the matrix
+    cores were not used."""
+    L = 12
+    say(f"case: window kernel FP64 L={L} {layout} depth={depth}, forward pass on the matrix cores")
+    V, Z, param, wires, P, U = build_case(L, layout, depth, seed=depth * 13 + 6)
+    t = mg.Target.dense(U)
+    p = mg.Program.create(L, P, param, wires)
+
+    def run():
+        r = mg.evaluate_columns(t, p, V, Z, mg.EVAL_F_GRAD_HVP, mg.MG_FP64, 7, 71)
+        info = mg.last_sweep_info()
+        assert info["kernel"] == "sweep_win_kernel", info
+        r1 = mg.evaluate_columns(t, p, V, None, mg.EVAL_F_GRAD, mg.MG_FP64, 7, 71)
+        r0 = mg.evaluate_columns(t, p, V, None, mg.EVAL_F, mg.MG_FP64, 7, 71)
+        return r, r1, r0, info
+
+    r, r1, r0, info = run()
+    mg.set_forward_mfma(0)
+    try:
+        v, v1, v0, _ = run()
+    finally:
+        mg.set_forward_mfma(-1)
+    fo, dgo, dho = orc.target_gradient_hvp(U, V, Z, param, wires, L, 7, 71)
+    for x, x1, x0 in ((r, r1, r0), (v, v1, v0)):
+        assert abs(x[0] - fo) / abs(fo) < 1e-11 and rel(x[1], dgo) < 1e-11 and rel(x[2], dho) < 1e-11
+        assert abs(x1[0] - fo) / abs(fo) < 1e-11 and rel(x1[1], dgo) < 1e-11 and abs(x0[0] - fo) / abs(fo) < 1e-11
+    assert np.any(r1[1] != v1[1]) and r0[0] != v0[0]
+    if info["lds_bytes"] > 128 * 1024:
+        # two states per re-layout round: the HVP's forward pass ran on the matrix cores too
+        assert np.any(r[2] != v[2])
+
+
+@pytest.mark.parametrize("layout,depth", [("tied", 3), ("open", 9), ("open", 15)])
 def test_lds_kernel_forward_on_matrix_cores(mg, layout, depth):
     """FP32 L=12 f, f + gradient and f + gradient + HVP on the LDS-resident kernel: the forward pass
     (psi, and dpsi for the HVP) over 4-qubit windows on the matrix cores (the default) and gate by
