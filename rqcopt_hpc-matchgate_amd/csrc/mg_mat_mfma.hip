@@ -141,6 +141,92 @@ hipError_t launch_materialize_mfma(const MatArgs& a, int fp32, int grid, hipStre
     return hipGetLastError();
 }
 
+// ---- window fragments on the device -------------------------------------------------------
+
+// Block w forms window w's matrices: thread t < 16 owns column n = t % 8 of parity block p = t / 8
+// (entry (m, n) of block p is the window entry (kParIdx[p][m], kParIdx[p][n]); the position of
+// window index i in its parity block is i / 2).  Left-multiplying by a gate mixes a column's
+// entries only among themselves, so each thread applies the window's gates to its own column,
+// M <- V M and N <- V N + Z M (with the M before the gate), then writes the A fragments that
+// read its column.  Fragment slot (4 p + kk, lane) holds row r = lane % 16, column c = 4 kk +
+// lane / 16 of block p's real 16 x 16 matrix: column c is entry n = 4 (c / 8) + c % 4 of the input,
+// real (c / 4 even) or imaginary part, and row r is the real or imaginary part of output entry m
+// (FP64 result layout: m = 2 (r % 4) + (r / 4) / 2, part (r / 4) % 2; FP32: m = 2 (r / 4) +
+// (r % 4) / 2, part (r % 4) % 2).
+__global__ __launch_bounds__(64) void window_frags_kernel(const int* wstart, const WinGate* gates,
+                                                          const cpx<double>* coef, int hvp, double* out64,
+                                                          float* out32)
+{
+    __shared__ double col[16][4][8];  // per thread: M re, M im, N re, N im of its column
+    const int w = blockIdx.x, t = threadIdx.x;
+    if (t >= 16) return;  // no barrier below: every thread works on its own column
+    const int p = t >> 3, n = t & 7;
+    double* mr = col[t][0];
+    double* mi = col[t][1];
+    double* nr = col[t][2];
+    double* ni = col[t][3];
+    for (int m = 0; m < 8; m++) {
+        mr[m] = m == n ? 1.0 : 0.0;
+        mi[m] = nr[m] = ni[m] = 0.0;
+    }
+    for (int j = wstart[w]; j < wstart[w + 1]; j++) {
+        const WinGate g = gates[j];
+        const cpx<double>* v = coef + (size_t)g.inst * kCoefPerInst;
+        const cpx<double>* z = v + 16;
+        const int bl = 1 << g.la, bh = 1 << g.lb;
+        for (int r = 0; r < 16; r++) {
+            if (r & (bl | bh)) continue;
+            // rows r, r | both bits keep the parity of r and mix by the corner block; rows with one
+            // of the bits set have the other parity and mix by the center block
+            const bool same = (__builtin_popcount(r) & 1) == p;
+            const int a = (same ? r : (r | bl)) >> 1, b = (same ? (r | bl | bh) : (r | bh)) >> 1;
+            const cpx<double>* c = same ? v + 4 : v;
+            const cpx<double>* zz = same ? z + 4 : z;
+            const double xr = mr[a], xi = mi[a], wr = mr[b], wi = mi[b];
+            if (hvp) {
+                const double yr = nr[a], yi = ni[a], ur = nr[b], ui = ni[b];
+                nr[a] = ((c[0].re * yr - c[0].im * yi) + (c[1].re * ur - c[1].im * ui)) +
+                        ((zz[0].re * xr - zz[0].im * xi) + (zz[1].re * wr - zz[1].im * wi));
+                ni[a] = ((c[0].re * yi + c[0].im * yr) + (c[1].re * ui + c[1].im * ur)) +
+                        ((zz[0].re * xi + zz[0].im * xr) + (zz[1].re * wi + zz[1].im * wr));
+                nr[b] = ((c[2].re * yr - c[2].im * yi) + (c[3].re * ur - c[3].im * ui)) +
+                        ((zz[2].re * xr - zz[2].im * xi) + (zz[3].re * wr - zz[3].im * wi));
+                ni[b] = ((c[2].re * yi + c[2].im * yr) + (c[3].re * ui + c[3].im * ur)) +
+                        ((zz[2].re * xi + zz[2].im * xr) + (zz[3].re * wi + zz[3].im * wr));
+            }
+            mr[a] = (c[0].re * xr - c[0].im * xi) + (c[1].re * wr - c[1].im * wi);
+            mi[a] = (c[0].re * xi + c[0].im * xr) + (c[1].re * wi + c[1].im * wr);
+            mr[b] = (c[2].re * xr - c[2].im * xi) + (c[3].re * wr - c[3].im * wi);
+            mi[b] = (c[2].re * xi + c[2].im * xr) + (c[3].re * wi + c[3].im * wr);
+        }
+    }
+    const int NF = hvp ? 16 : 8;
+    for (int X = 0; X <= (hvp ? 1 : 0); X++) {
+        const double* re = X ? nr : mr;
+        const double* im = X ? ni : mi;
+        for (int q = 0; q < 2; q++) {
+            const int cc = 8 * (n >> 2) + 4 * q + (n & 3), kk = cc >> 2;
+            for (int r = 0; r < 16; r++) {
+                const int ln = 16 * (cc & 3) + r;
+                const int m = out64 ? 2 * (r & 3) + ((r >> 2) >> 1) : 2 * (r >> 2) + ((r & 3) >> 1);
+                const int part = out64 ? (r >> 2) & 1 : (r & 3) & 1;
+                const double val = part == 0 ? (q == 0 ? re[m] : -im[m]) : (q == 0 ? im[m] : re[m]);
+                const size_t idx = ((size_t)w * NF + 8 * X + 4 * p + kk) * 64 + ln;
+                if (out64) out64[idx] = val;
+                else out32[idx] = (float)val;
+            }
+        }
+    }
+}
+
+hipError_t launch_window_frags(int nwin, const int* wstart, const WinGate* gates, const cpx<double>* coef, int hvp,
+                               double* out64, float* out32, hipStream_t s)
+{
+    if (nwin < 1 || (!out64) == (!out32)) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(window_frags_kernel, dim3(nwin), dim3(64), 0, s, wstart, gates, coef, hvp, out64, out32);
+    return hipGetLastError();
+}
+
 // ---- host: the window form ----------------------------------------------------------------
 
 namespace {
@@ -175,55 +261,79 @@ void apply_gate_left(zc* M, const cpx<double>* g, int la, int lb)
     }
 }
 
-// M <- V M and, with N, N <- V N + Z M (the M before the gate): v, z the gate's coefficients, on
-// the 16 x 16 matrices of a window, which conserve parity (only entries whose row and column have
-// the same parity can be nonzero, and the gate keeps it so: a column's entries of the other parity
-// stay zero and are not computed)
-void apply_gate_pair(zc* M, zc* N, const cpx<double>* v, const cpx<double>* z, int la, int lb)
+// A window's 16 x 16 matrix as its two 8 x 8 parity blocks, real and imaginary parts apart: entry
+// (m, n) of block p is the entry (kParIdx[p][m], kParIdx[p][n]); the entries across parities are
+// zero for a product of matchgates and are not stored
+struct ParMat {
+    double re[2][8][8], im[2][8][8];
+    void set(bool identity)
+    {
+        for (int p = 0; p < 2; p++)
+            for (int m = 0; m < 8; m++)
+                for (int n = 0; n < 8; n++) {
+                    re[p][m][n] = identity && m == n ? 1.0 : 0.0;
+                    im[p][m][n] = 0.0;
+                }
+    }
+};
+
+// position of window index i in its parity block (the inverse of kParIdx)
+struct ParPos {
+    int pos[16];
+    ParPos()
+    {
+        for (int p = 0; p < 2; p++)
+            for (int m = 0; m < 8; m++) pos[kParIdx[p][m]] = m;
+    }
+};
+
+// rows a, b of block p: M <- (c0 a + c1 b, c2 a + c3 b) and, with N, N <- the same of N's rows
+// plus (z0 a + z1 b, z2 a + z3 b) of M's rows before the update
+void mix_rows(ParMat& M, ParMat* N, int p, int a, int b, const cpx<double>* c, const cpx<double>* z)
 {
-    // columns of each parity
-    static const struct Cols {
-        int c[2][8];
-        Cols()
-        {
-            int n[2] = {0, 0};
-            for (int c0 = 0; c0 < 16; c0++) {
-                const int q = __builtin_popcount(c0) & 1;
-                c[q][n[q]++] = c0;
-            }
+    double* __restrict__ ar = M.re[p][a];
+    double* __restrict__ ai = M.im[p][a];
+    double* __restrict__ br = M.re[p][b];
+    double* __restrict__ bi = M.im[p][b];
+    if (N) {
+        double* __restrict__ nar = N->re[p][a];
+        double* __restrict__ nai = N->im[p][a];
+        double* __restrict__ nbr = N->re[p][b];
+        double* __restrict__ nbi = N->im[p][b];
+        for (int n = 0; n < 8; n++) {
+            const double xr = ar[n], xi = ai[n], wr = br[n], wi = bi[n];
+            const double yr = nar[n], yi = nai[n], vr = nbr[n], vi = nbi[n];
+            nar[n] = ((c[0].re * yr - c[0].im * yi) + (c[1].re * vr - c[1].im * vi)) +
+                     ((z[0].re * xr - z[0].im * xi) + (z[1].re * wr - z[1].im * wi));
+            nai[n] = ((c[0].re * yi + c[0].im * yr) + (c[1].re * vi + c[1].im * vr)) +
+                     ((z[0].re * xi + z[0].im * xr) + (z[1].re * wi + z[1].im * wr));
+            nbr[n] = ((c[2].re * yr - c[2].im * yi) + (c[3].re * vr - c[3].im * vi)) +
+                     ((z[2].re * xr - z[2].im * xi) + (z[3].re * wr - z[3].im * wi));
+            nbi[n] = ((c[2].re * yi + c[2].im * yr) + (c[3].re * vi + c[3].im * vr)) +
+                     ((z[2].re * xi + z[2].im * xr) + (z[3].re * wi + z[3].im * wr));
         }
-    } cols;
+    }
+    for (int n = 0; n < 8; n++) {
+        const double xr = ar[n], xi = ai[n], wr = br[n], wi = bi[n];
+        ar[n] = (c[0].re * xr - c[0].im * xi) + (c[1].re * wr - c[1].im * wi);
+        ai[n] = (c[0].re * xi + c[0].im * xr) + (c[1].re * wi + c[1].im * wr);
+        br[n] = (c[2].re * xr - c[2].im * xi) + (c[3].re * wr - c[3].im * wi);
+        bi[n] = (c[2].re * xi + c[2].im * xr) + (c[3].re * wi + c[3].im * wr);
+    }
+}
+
+// M <- V M and, with N, N <- V N + Z M (the M before the gate): v, z the gate's coefficients
+// (center block 0-3 on the window indices with exactly one of the gate's bits set, corner block
+// 4-7 on those with none or both), la, lb the window bits of the gate's lo and hi wires
+void apply_gate_pair(ParMat& M, ParMat* N, const cpx<double>* v, const cpx<double>* z, int la, int lb)
+{
+    static const ParPos pp;
     for (int r = 0; r < 16; r++) {
         if (r & ((1 << la) | (1 << lb))) continue;
+        const int i00 = r, i01 = r | (1 << la), i10 = r | (1 << lb), i11 = r | (1 << la) | (1 << lb);
         const int pr = __builtin_popcount(r) & 1;
-        // rows i00, i11 keep the parity of r and pair with its columns; rows i01, i10 the others
-        for (int h = 0; h < 2; h++) {
-            const int ia = h == 0 ? r : (r | (1 << la)), ib = h == 0 ? (r | (1 << la) | (1 << lb)) : (r | (1 << lb));
-            const cpx<double>* va = h == 0 ? v + 4 : v;  // (v4, v5 | v6, v7) or (v0, v1 | v2, v3)
-            const cpx<double>* za = h == 0 ? z + 4 : z;
-            const int* cl = cols.c[h == 0 ? pr : 1 - pr];
-            zc* Ma = M + ia * 16;
-            zc* Mb = M + ib * 16;
-            if (N) {
-                zc* Na = N + ia * 16;
-                zc* Nb = N + ib * 16;
-                for (int k = 0; k < 8; k++) {
-                    const int c = cl[k];
-                    const zc x = Ma[c], w = Mb[c], nx = Na[c], nw = Nb[c];
-                    Na[c] = mul2(va[0], nx, va[1], nw) + mul2(za[0], x, za[1], w);
-                    Nb[c] = mul2(va[2], nx, va[3], nw) + mul2(za[2], x, za[3], w);
-                    Ma[c] = mul2(va[0], x, va[1], w);
-                    Mb[c] = mul2(va[2], x, va[3], w);
-                }
-            } else {
-                for (int k = 0; k < 8; k++) {
-                    const int c = cl[k];
-                    const zc x = Ma[c], w = Mb[c];
-                    Ma[c] = mul2(va[0], x, va[1], w);
-                    Mb[c] = mul2(va[2], x, va[3], w);
-                }
-            }
-        }
+        mix_rows(M, N, pr, pp.pos[i00], pp.pos[i11], v + 4, z + 4);
+        mix_rows(M, N, 1 - pr, pp.pos[i01], pp.pos[i10], v, z);
     }
 }
 
@@ -252,7 +362,7 @@ struct FragTable {
                         const int n = (c / 8) * 4 + c % 4, q = (c / 4) % 2;
                         const int m = w == 0 ? 2 * (r % 4) + (r / 4) / 2 : 2 * (r / 4) + (r % 4) / 2;
                         const int pt = w == 0 ? (r / 4) % 2 : (r % 4) % 2;
-                        entry[w][i] = (short)(kParIdx[p][m] * 16 + kParIdx[p][n]);
+                        entry[w][i] = (short)(p * 64 + m * 8 + n);
                         part[w][i] = (unsigned char)(pt == 0 ? (q == 0 ? 0 : 1) : (q == 0 ? 2 : 0));
                     }
     }
@@ -264,20 +374,21 @@ const FragTable& frag_table()
 }
 
 template <typename T>
-void push_fragments(const zc* M, int w, std::vector<T>* af)
+void push_fragments(const ParMat& M, int w, std::vector<T>* af)
 {
     const FragTable& t = frag_table();
+    const double* re = &M.re[0][0][0];
+    const double* im = &M.im[0][0][0];
     const size_t n0 = af->size();
     af->resize(n0 + 512);
     T* o = af->data() + n0;
     for (int i = 0; i < 512; i++) {
-        const zc e = M[t.entry[w][i]];
-        const int pt = t.part[w][i];
-        o[i] = (T)(pt == 0 ? e.real() : pt == 1 ? -e.imag() : e.imag());
+        const int e = t.entry[w][i], pt = t.part[w][i];
+        o[i] = (T)(pt == 0 ? re[e] : pt == 1 ? -im[e] : im[e]);
     }
 }
 
-void push_fragments32(const zc* M, std::vector<float>* af) { push_fragments<float>(M, 1, af); }
+void push_fragments32(const ParMat& M, std::vector<float>* af) { push_fragments<float>(M, 1, af); }
 
 }  // namespace
 
@@ -288,12 +399,12 @@ int forward_windows(const std::vector<Inst>& inst, const std::vector<cpx<double>
     std::vector<WBlock> wbs;
     window_blocks(inst, TB, W, 1 << 20, order, &wbs);
     if (order->size() != inst.size()) return -1;
+    wins->reserve(wins->size() + wbs.size());
+    af->reserve(af->size() + wbs.size() * (hvp ? 1024 : 512));
     for (const WBlock& wb : wbs) {
-        zc M[256], N[256];
-        for (int i = 0; i < 256; i++) {
-            M[i] = (i / 16 == i % 16) ? 1.0 : 0.0;
-            N[i] = 0.0;
-        }
+        ParMat M, N;
+        M.set(true);
+        N.set(false);
         for (int k = wb.g0; k < wb.g1; k++) {
             const Inst& l = inst[(*order)[k]];
             int la = -1, lb = -1;
@@ -303,7 +414,7 @@ int forward_windows(const std::vector<Inst>& inst, const std::vector<cpx<double>
             }
             if (la < 0 || lb < 0) return -1;
             const cpx<double>* g = &coef[(size_t)(*order)[k] * kCoefPerInst];
-            apply_gate_pair(M, hvp ? N : nullptr, g, g + 16, la, lb);
+            apply_gate_pair(M, hvp ? &N : nullptr, g, g + 16, la, lb);
         }
         FWin fw{};
         int oth[8], no = 0;
@@ -327,38 +438,27 @@ int forward_windows(const std::vector<Inst>& inst, const std::vector<cpx<double>
     return (int)wbs.size();
 }
 
-namespace {
-
-// lane fragments of one parity block of a 16 x 16 window matrix, FP64 result layout (rows g, g + 4,
-// g + 8, g + 12 of lane group g: real and imaginary parts of entries 2 g and 2 g + 1)
-void push_fragments64(const zc* M, std::vector<double>* af) { push_fragments<double>(M, 0, af); }
-
-}  // namespace
-
-int forward_windows64(const std::vector<Inst>& inst, const std::vector<cpx<double>>& coef, std::vector<DWin>* wins,
-                      std::vector<double>* af, bool hvp)
+int forward_window_form(const std::vector<Inst>& inst, bool fp64, std::vector<DWin>* wins, std::vector<int>* wstart,
+                        std::vector<WinGate>* gates, std::vector<int>* order)
 {
     constexpr int TB = 12, W = 4;
-    std::vector<int> order;
     std::vector<WBlock> wbs;
-    window_blocks(inst, TB, W, 1 << 20, &order, &wbs);
-    if (order.size() != inst.size()) return -1;
+    window_blocks(inst, TB, W, 1 << 20, order, &wbs);
+    if (order->size() != inst.size()) return -1;
+    wins->clear();
+    wstart->clear();
+    gates->clear();
     for (const WBlock& wb : wbs) {
-        zc M[256], N[256];
-        for (int i = 0; i < 256; i++) {
-            M[i] = (i / 16 == i % 16) ? 1.0 : 0.0;
-            N[i] = 0.0;
-        }
+        wstart->push_back((int)gates->size());
         for (int k = wb.g0; k < wb.g1; k++) {
-            const Inst& l = inst[order[k]];
-            int la = -1, lb = -1;
+            const Inst& l = inst[(*order)[k]];
+            WinGate g{(*order)[k], -1, -1, 0};
             for (int i = 0; i < W; i++) {
-                if (wb.pos[i] == l.lo) la = i;
-                if (wb.pos[i] == l.hi) lb = i;
+                if (wb.pos[i] == l.lo) g.la = i;
+                if (wb.pos[i] == l.hi) g.lb = i;
             }
-            if (la < 0 || lb < 0) return -1;
-            const cpx<double>* g = &coef[(size_t)order[k] * kCoefPerInst];
-            apply_gate_pair(M, hvp ? N : nullptr, g, g + 16, la, lb);
+            if (g.la < 0 || g.lb < 0) return -1;
+            gates->push_back(g);
         }
         DWin dw{};
         int oth[8], no = 0;
@@ -367,7 +467,7 @@ int forward_windows64(const std::vector<Inst>& inst, const std::vector<cpx<doubl
         auto put = [&](uint32_t x, const int* pos, int n) {
             uint32_t r = 0;
             for (int i = 0; i < n; i++) r |= ((x >> i) & 1u) << pos[i];
-            return (int)mat_swz(r);
+            return fp64 ? (int)mat_swz(r) : tile_swz32(r);
         };
         for (int p = 0; p < 2; p++)
             for (int n = 0; n < 8; n++) dw.offp[p][n] = put((uint32_t)kParIdx[p][n], wb.pos, W);
@@ -376,9 +476,8 @@ int forward_windows64(const std::vector<Inst>& inst, const std::vector<cpx<doubl
             dw.offcb[j] = put((uint32_t)j, oth + 4, 4);
         }
         wins->push_back(dw);
-        push_fragments64(M, af);
-        if (hvp) push_fragments64(N, af);
     }
+    wstart->push_back((int)gates->size());
     return (int)wbs.size();
 }
 

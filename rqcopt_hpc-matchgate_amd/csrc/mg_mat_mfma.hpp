@@ -44,11 +44,7 @@ bool make_dense_windows(const std::vector<Inst>& inst, const std::vector<cpx<dou
 // tile block: per window psi_out = M psi_in with M the product of the window's V, which keeps parity
 // and is two 8 x 8 blocks, FP32 MFMA tiles as for the materialisation.  Offsets are LDS indices of
 // the FP32 tile kernel (lds_swz, linear over XOR).
-struct FWin {
-    int offp[2][8];
-    int offj[16];
-    int offcb[16];
-};
+using FWin = DWin;  // the same offsets, into the FP32 tile kernel's LDS layout (lds_swz)
 
 // `inst` (tile-local bit positions) and `coef` (V first in each instance's coefficients) are one
 // tile block's gates in program order; fills `order` with a window order of them, appends the
@@ -60,14 +56,25 @@ struct FWin {
 int forward_windows(const std::vector<Inst>& inst, const std::vector<cpx<double>>& coef, std::vector<int>* order,
                     std::vector<FWin>* wins, std::vector<float>* af, bool hvp = false);
 
-// The forward pass of an FP64 sweep of a 12-qubit column (L = 12) over 4-qubit windows, as
-// forward_windows but in the FP64 MFMA form of the materialisation (struct DWin, offsets into an
-// LDS column laid out by mat_swz; fragments [window][8][lane] doubles, or [window][16][lane] with
-// `hvp`: M's 8 then N's 8).  The window order of the program's gates is the one the matrices are
-// formed in; the sweep's reverse pass keeps its own gate order (any topological order of the
-// program has the same product).  Returns the number of windows, or -1.
-int forward_windows64(const std::vector<Inst>& inst, const std::vector<cpx<double>>& coef, std::vector<DWin>* wins,
-                      std::vector<double>* af, bool hvp);
+// one gate of a forward-pass window: program instance, window bits of its lo and hi wires
+struct WinGate {
+    int inst, la, lb, pad;
+};
+
+// The window form of the forward pass of a 12-qubit column (L = 12) without the matrices: the
+// program's gates cut into 4-qubit windows (window_blocks), per window its gates in order
+// (gates[wstart[w] .. wstart[w + 1])), the LDS offsets of each window (`fp64`: the FP64 MFMA
+// layout of the materialisation, mat_swz; else the FP32 tile kernel's lds_swz) and the window
+// order of the instances (`order`, new -> old).  Returns the number of windows, or -1.
+int forward_window_form(const std::vector<Inst>& inst, bool fp64, std::vector<DWin>* wins, std::vector<int>* wstart,
+                        std::vector<WinGate>* gates, std::vector<int>* order);
+
+// The windows' A fragments on the device, from this evaluation's coefficients (`coef`: FP64,
+// [program instance][kCoefPerInst]): block w forms window w's M (and with `hvp` N) column by
+// column and writes [window][8 or 16][lane] fragments in the FP64 (out64) or FP32 (out32) MFMA
+// result layout, the same values forward_windows makes on the host
+hipError_t launch_window_frags(int nwin, const int* wstart, const WinGate* gates, const cpx<double>* coef, int hvp,
+                               double* out64, float* out32, hipStream_t s);
 
 struct MatArgs {
     int L, ntb;

@@ -77,7 +77,12 @@ struct Context {
     DevBuf rows;  // materialised target rows of one column chunk (program targets beyond 12 qubits)
     DevBuf w_inst, w_coef, w_blk;  // gate-window form of the program (register window kernel)
     DevBuf w_slot, w_bpar;         // tiled kernel: per-block accumulator slots and their parameters
-    DevBuf w_fwin, w_fwaf;         // tiled kernel: forward-pass windows and their MFMA fragments
+    DevBuf w_fwin, w_fwaf;         // forward-pass windows and their MFMA fragments
+    DevBuf w_fgate, w_fcoef;       // L=12 forward-pass windows: their gates, FP64 coefficients
+    // host staging of the L=12 forward-pass window form, kept from one evaluation to the next
+    std::vector<DWin> h_dwin;
+    std::vector<int> h_wstart;
+    std::vector<WinGate> h_wgate;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::vector<double> out_h;
     // batched evaluation: packed descriptor upload, partials, outputs, scratch, counter
@@ -878,9 +883,8 @@ static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, in
     // topological order of the program: the same product, and the reverse sweep runs it backwards)
     if (fp32 && lds_states && !ckpt && !reg_T && !use_win && !use_tile && L == 12 && kn.fwd_mfma != 0 && NW <= 16) {
         std::vector<int> o2;
-        std::vector<FWin> fw;
-        std::vector<float> fa;
-        if (forward_windows(b.inst, b.coef, &o2, &fw, &fa, mode == 2) < 0 || o2.size() != b.inst.size()) {
+        const int nwin = forward_window_form(b.inst, false, &c->h_dwin, &c->h_wstart, &c->h_wgate, &o2);
+        if (nwin < 1 || o2.size() != b.inst.size()) {
             set_error("LDS sweep: forward window blocking failed");
             return -1;
         }
@@ -888,26 +892,33 @@ static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, in
         std::vector<cpx<R>> wc(o2.size() * kCoefPerInst);
         for (size_t k = 0; k < o2.size(); k++) {
             wi[k] = b.inst[o2[k]];
-            for (int e = 0; e < kCoefPerInst; e++) {
-                const cpx<double> v = b.coef[(size_t)o2[k] * kCoefPerInst + e];
-                wc[k * kCoefPerInst + e] = {(R)v.re, (R)v.im};
-            }
+            std::copy(coefR.begin() + (size_t)o2[k] * kCoefPerInst, coefR.begin() + (size_t)(o2[k] + 1) * kCoefPerInst,
+                      wc.begin() + k * kCoefPerInst);
         }
-        void *d_wi = nullptr, *d_wc = nullptr, *d_fw = nullptr, *d_fa = nullptr;
+        void *d_wi = nullptr, *d_wc = nullptr, *d_fw = nullptr, *d_fa = nullptr, *d_fg = nullptr, *d_c64 = nullptr;
+        const size_t ngate_bytes = c->h_wstart.size() * sizeof(int), gate_bytes = c->h_wgate.size() * sizeof(WinGate);
         if (ar.get(c->w_inst, wi.size() * sizeof(Inst), &d_wi) < 0 || ar.get(c->w_coef, wc.size() * sizeof(cpx<R>), &d_wc) < 0 ||
-            ar.get(c->w_fwin, fw.size() * sizeof(FWin), &d_fw) < 0 || ar.get(c->w_fwaf, fa.size() * sizeof(float), &d_fa) < 0)
+            ar.get(c->w_fwin, nwin * sizeof(FWin), &d_fw) < 0 ||
+            ar.get(c->w_fwaf, (size_t)nwin * (mode == 2 ? 16 : 8) * 64 * sizeof(float), &d_fa) < 0 ||
+            ar.get(c->w_fgate, ngate_bytes + gate_bytes, &d_fg) < 0 ||
+            ar.get(c->w_fcoef, b.coef.size() * sizeof(cpx<double>), &d_c64) < 0)
             return -1;
         MG_HIP(hipMemcpyAsync(d_wi, wi.data(), wi.size() * sizeof(Inst), hipMemcpyHostToDevice, s));
         MG_HIP(hipMemcpyAsync(d_wc, wc.data(), wc.size() * sizeof(cpx<R>), hipMemcpyHostToDevice, s));
-        MG_HIP(hipMemcpyAsync(d_fw, fw.data(), fw.size() * sizeof(FWin), hipMemcpyHostToDevice, s));
-        MG_HIP(hipMemcpyAsync(d_fa, fa.data(), fa.size() * sizeof(float), hipMemcpyHostToDevice, s));
-        // the copies read pageable host vectors: complete them before the vectors go away
+        MG_HIP(hipMemcpyAsync(d_fw, c->h_dwin.data(), nwin * sizeof(FWin), hipMemcpyHostToDevice, s));
+        MG_HIP(hipMemcpyAsync(d_fg, c->h_wstart.data(), ngate_bytes, hipMemcpyHostToDevice, s));
+        MG_HIP(hipMemcpyAsync((unsigned char*)d_fg + ngate_bytes, c->h_wgate.data(), gate_bytes, hipMemcpyHostToDevice, s));
+        MG_HIP(hipMemcpyAsync(d_c64, b.coef.data(), b.coef.size() * sizeof(cpx<double>), hipMemcpyHostToDevice, s));
+        // the window matrices of this evaluation's gates (and direction), on the device
+        MG_HIP(launch_window_frags(nwin, (const int*)d_fg, (const WinGate*)((unsigned char*)d_fg + ngate_bytes),
+                                   (const cpx<double>*)d_c64, mode == 2, nullptr, (float*)d_fa, s));
+        // the copies above read pageable host vectors: complete them before the vectors go away
         MG_HIP(hipStreamSynchronize(s));
         a.inst = (const Inst*)d_wi;
         a.coef = (const cpx<R>*)d_wc;
         a.fwin = d_fw;
         a.fwaf = (const float*)d_fa;
-        a.nwin = (int)fw.size();
+        a.nwin = nwin;
     }
 
     WinSweepArgs wa{};
@@ -941,22 +952,30 @@ static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, in
         wa.rs = win_rs;
         wa.slots = win_slots;
         // FP64: the forward pass (psi, and dpsi for the HVP) on the matrix cores over 4-qubit
-        // windows of the column held in the re-layout buffer (the HVP needs two states per round)
-        std::vector<DWin> fw;
-        std::vector<double> fa;
+        // windows of the column held in the re-layout buffer (the HVP needs two states per round);
+        // the windows' matrices are formed on the device from the coefficients in program order
         if (!fp32 && L == 12 && kn.fwd_mfma != 0 && (mode < 2 || win_rs >= 2)) {
-            if (forward_windows64(b.inst, b.coef, &fw, &fa, mode == 2) < 0) {
+            std::vector<int> o2;
+            const int nwin = forward_window_form(b.inst, true, &c->h_dwin, &c->h_wstart, &c->h_wgate, &o2);
+            if (nwin < 1) {
                 set_error("window sweep: forward window blocking failed");
                 return -1;
             }
-            void *d_fw = nullptr, *d_fa = nullptr;
-            if (ar.get(c->w_fwin, fw.size() * sizeof(DWin), &d_fw) < 0 || ar.get(c->w_fwaf, fa.size() * sizeof(double), &d_fa) < 0)
+            void *d_fw = nullptr, *d_fa = nullptr, *d_fg = nullptr;
+            const size_t ngate_bytes = c->h_wstart.size() * sizeof(int), gate_bytes = c->h_wgate.size() * sizeof(WinGate);
+            if (ar.get(c->w_fwin, nwin * sizeof(DWin), &d_fw) < 0 ||
+                ar.get(c->w_fwaf, (size_t)nwin * (mode == 2 ? 16 : 8) * 64 * sizeof(double), &d_fa) < 0 ||
+                ar.get(c->w_fgate, ngate_bytes + gate_bytes, &d_fg) < 0)
                 return -1;
-            MG_HIP(hipMemcpyAsync(d_fw, fw.data(), fw.size() * sizeof(DWin), hipMemcpyHostToDevice, s));
-            MG_HIP(hipMemcpyAsync(d_fa, fa.data(), fa.size() * sizeof(double), hipMemcpyHostToDevice, s));
+            MG_HIP(hipMemcpyAsync(d_fw, c->h_dwin.data(), nwin * sizeof(DWin), hipMemcpyHostToDevice, s));
+            MG_HIP(hipMemcpyAsync(d_fg, c->h_wstart.data(), ngate_bytes, hipMemcpyHostToDevice, s));
+            MG_HIP(hipMemcpyAsync((unsigned char*)d_fg + ngate_bytes, c->h_wgate.data(), gate_bytes, hipMemcpyHostToDevice, s));
+            // the sweep's coefficients in program order (R = double here)
+            MG_HIP(launch_window_frags(nwin, (const int*)d_fg, (const WinGate*)((unsigned char*)d_fg + ngate_bytes),
+                                       (const cpx<double>*)d_coef, mode == 2, (double*)d_fa, nullptr, s));
             wa.fwin = d_fw;
             wa.fwaf = (const double*)d_fa;
-            wa.nfwin = (int)fw.size();
+            wa.nfwin = nwin;
         }
         // the copies above read pageable host vectors: complete them before w goes away
         MG_HIP(hipStreamSynchronize(s));
