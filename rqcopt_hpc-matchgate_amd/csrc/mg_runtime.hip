@@ -1030,12 +1030,11 @@ static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, in
             set_error("tiled sweep: gate blocking failed");
             return -1;
         }
-        // FP32 f and f + gradient: the forward pass of each tile block on the matrix cores, over
-        // 4-qubit windows (the block's instances reordered into window order; the accumulator slots
-        // follow them)
+        // FP32: the forward pass of each tile block on the matrix cores, over 4-qubit windows (the
+        // block's instances reordered into window order; the accumulator slots follow them)
         std::vector<FWin> fwins;
         std::vector<float> fwaf;
-        if (fp32 && mode <= 1 && kn.fwd_mfma != 0) {
+        if (fp32 && kn.fwd_mfma != 0) {
             std::vector<int> order_w(order.size()), slot_w(tb_slot.size());
             for (TBlock& bk : blocks) {
                 std::vector<Inst> li;
@@ -1054,7 +1053,7 @@ static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, in
                 }
                 std::vector<int> o2;
                 bk.w0 = (int)fwins.size();
-                if (forward_windows(li, lc, &o2, &fwins, &fwaf) < 0) {
+                if (forward_windows(li, lc, &o2, &fwins, &fwaf, mode == 2) < 0) {
                     set_error("tiled sweep: forward window blocking failed");
                     return -1;
                 }

@@ -35,12 +35,13 @@ namespace {
 template <typename R>
 constexpr int tile_kw() { return sizeof(R) == 8 ? 11 : 12; }
 
-// threads per workgroup (log2): FP64 512 (256 measured the same); FP32 HVP 1024, one quadruple
-// per thread and gate at 4 waves per SIMD with the coefficients in scalar registers (L=16 HVP 1.75
-// -> 1.52 s against 512 threads with two quadruples per thread and vector coefficients); FP32 f
-// and f + gradient 512, so that two workgroups (two tiles of one or two states) share a CU
+// threads per workgroup (log2): 512.  FP64: 256 measured the same.  FP32 HVP: its forward pass
+// runs on the matrix cores, where 512 threads give each wave two column blocks (L=16 1.49 -> 1.47 s,
+// L=14 77.4 -> 75.6 ms against 1024 threads with the forward pass on the VALU, which was the faster
+// of the two VALU forms: 1.52 s against 1.63 s at 512).  FP32 f and f + gradient: two workgroups
+// (two tiles of one or two states) share a CU.
 template <typename R, int MODE>
-constexpr int tile_lt() { return sizeof(R) == 4 && MODE == 2 ? 10 : 9; }
+constexpr int tile_lt() { return 9; }
 // waves per SIMD the register allocation must allow: FP32 f + gradient 4 (two 512-thread
 // workgroups per CU, at most 128 VGPRs)
 template <typename R, int MODE>
@@ -70,10 +71,9 @@ __device__ inline void tile_map(const TBlock* blk, int tid, TileMap<PER>* m)
 
 }  // namespace
 
-// The forward pass of one FP32 tile over the block's 4-qubit windows on the matrix cores (f and f
-// + gradient: psi only; fwd_windows, mg_fwd_mfma.hpp).  (The HVP's forward pass, psi and dpsi <-
-// M dpsi + N psi, measured L=16 1543 -> 1505 ms but L=14 depth 15 108.7 -> 112.3 ms: compiled into
-// the HVP kernel it raises the reverse pass's register spills, so the HVP stays gate by gate.)
+// The forward pass of one FP32 tile over the block's 4-qubit windows on the matrix cores: psi
+// (fwd_windows) for f and f + gradient, psi and dpsi <- M dpsi + N psi (fwd_windows_hvp) for the
+// HVP; mg_fwd_mfma.hpp.
 
 template <typename R, int MODE>
 __global__ __launch_bounds__((1 << tile_lt<R, MODE>()), (tile_min_waves<R, MODE>())) void sweep_tile_kernel(TileArgs<R> a)
@@ -149,9 +149,13 @@ __global__ __launch_bounds__((1 << tile_lt<R, MODE>()), (tile_min_waves<R, MODE>
                     }
                 }
                 __syncthreads();
-                if constexpr (sizeof(R) == 4 && MODE <= 1) {
+                if constexpr (sizeof(R) == 4) {
                     if (a.fwin) {
-                        fwd_windows<NW>(reinterpret_cast<const FWin*>(a.fwin), a.fwaf, bk->w0, bk->w1, t_psi, lane, wv);
+                        if constexpr (MODE == 2)
+                            fwd_windows_hvp<NW>(reinterpret_cast<const FWin*>(a.fwin), a.fwaf, bk->w0, bk->w1, t_psi,
+                                                t_dpsi, lane, wv);
+                        else
+                            fwd_windows<NW>(reinterpret_cast<const FWin*>(a.fwin), a.fwaf, bk->w0, bk->w1, t_psi, lane, wv);
                         goto forward_done;
                     }
                 }

@@ -391,16 +391,21 @@ def test_tile_kernels_match_oracle(mg, L, layout, depth, prec):
     for r in results:
         assert abs(r[0] - g[0]) / abs(g[0]) < tol_g and rel(r[1], g[1]) < tol_g and rel(r[2], g[2]) < tol_g
     if prec == "fp32" and L > 13:
-        # FP32 f and f + gradient: forward pass gate by gate on the VALU against the default
-        # (4-qubit windows on the matrix cores), both against the oracle
+        # FP32 f, f + gradient and f + gradient + HVP: forward pass gate by gate on the VALU against
+        # the default (4-qubit windows on the matrix cores), both against the oracle
         mg.set_forward_mfma(0)
         try:
+            v2 = mg.evaluate_columns(t, p, V, Z, mg.EVAL_F_GRAD_HVP, precision, c0, c1)
+            assert mg.last_sweep_info()["kernel"] == "sweep_tile_kernel"
             v1 = mg.evaluate_columns(t, p, V, None, mg.EVAL_F_GRAD, precision, c0, c1)
             v0 = mg.evaluate_columns(t, p, V, None, mg.EVAL_F, precision, c0, c1)
             assert mg.last_sweep_info()["kernel"] == "sweep_tile_kernel"
         finally:
             mg.set_forward_mfma(-1)
+        assert abs(v2[0] - fo) / abs(fo) < tol and rel(v2[1], dgo) < tol and rel(v2[2], dho) < tol
         assert abs(v1[0] - fo) / abs(fo) < tol and rel(v1[1], dgo) < tol and abs(v0[0] - fo) / abs(fo) < tol
+        # the default took the matrix cores: the two forward passes round differently
+        assert any(np.any(r[2] != v2[2]) for r in results)
 
 
 def test_register_kernel_brickwall_target(mg):
