@@ -514,9 +514,10 @@ def main():
                 "frac": achieved / PEAK_TFLOPS[dtype], "traffic": measured_traffic(info["kernel"], dtype),
                 "kernel": info["kernel"],
                 "kernel_ms": kernel_ms, "flops_per_launch": flops,
-                "note": "the sweeps are vector-ALU FMA work (no MFMA instructions); peak = MI355X FP64 vector rate "
-                        "(equal to its FP64 matrix rate) or FP32 vector rate; traffic = HBM bytes per launch from "
-                        "rocprofv3 FETCH_SIZE/WRITE_SIZE, profiles/"}
+                "note": "the reverse sweep (most of the kernel) is vector-ALU FMA work, the forward pass runs on the "
+                        "matrix cores over 4-qubit windows; peak = MI355X FP64 vector rate (equal to its FP64 "
+                        "matrix rate) or FP32 vector rate; flops counted per gate as the VALU form; traffic = HBM "
+                        "bytes per launch from rocprofv3 FETCH_SIZE/WRITE_SIZE, profiles/"}
 
     extra = {}
     if prec == mg.MG_FP64 and not args.no_fp32 and not args.only_served:

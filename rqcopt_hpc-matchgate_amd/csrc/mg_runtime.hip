@@ -73,18 +73,25 @@ struct Context {
     hipStream_t s = nullptr;
     int ncu = 0;
     size_t lds_limit = 0;
-    DevBuf inst, coef, partial, out, scratch, ckpt;
-    DevBuf rows;  // materialised target rows of one column chunk (program targets beyond 12 qubits)
-    DevBuf w_inst, w_coef, w_blk;  // gate-window form of the program (register window kernel)
-    DevBuf w_slot, w_bpar;         // tiled kernel: per-block accumulator slots and their parameters
-    DevBuf w_fwin, w_fwaf;         // forward-pass windows and their MFMA fragments
-    DevBuf w_fgate, w_fcoef;       // L=12 forward-pass windows: their gates, FP64 coefficients
+    DevBuf partial, out, scratch, ckpt;
+    DevBuf rows;    // materialised target rows of one column chunk (program targets beyond 12 qubits)
+    DevBuf w_fwaf;  // L=12 forward-pass windows' MFMA fragments, formed on the device
     // host staging of the L=12 forward-pass window form, kept from one evaluation to the next
     std::vector<DWin> h_dwin;
     std::vector<int> h_wstart;
     std::vector<WinGate> h_wgate;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::vector<double> out_h;
+    // single-request evaluation: every descriptor of one evaluation in one device image, sent in
+    // one copy from pinned staging; the event marks that copy's completion (the staging is reused
+    // by the next evaluation); the reduced result comes back into pinned memory
+    DevBuf e_up;
+    void* e_up_h = nullptr;
+    size_t e_up_hn = 0;
+    hipEvent_t e_up_ev = nullptr;
+    bool e_up_pending = false;
+    void* e_out_h = nullptr;
+    size_t e_out_hn = 0;
     // batched evaluation: packed descriptor upload, partials, outputs, scratch, counter
     DevBuf b_up, b_partial, b_out, b_scratch, b_counter;
     void* b_up_h = nullptr;   // pinned staging of the upload
@@ -138,6 +145,8 @@ struct Context {
     ~Context()
     {
         for (Graph& g : graphs) (void)hipGraphExecDestroy(g.exec);
+        for (void* h : {b_up_h, b_out_h, e_up_h, e_out_h})
+            if (h) (void)hipHostFree(h);
     }
 };
 
@@ -152,7 +161,9 @@ static Context* ctx()
         n->ncu = prop.multiProcessorCount;
         n->lds_limit = prop.sharedMemPerBlockOptin ? prop.sharedMemPerBlockOptin : prop.sharedMemPerBlock;
         if (hipStreamCreateWithFlags(&n->s, hipStreamNonBlocking) != hipSuccess) return nullptr;
-        if (hipEventCreate(&n->ev0) != hipSuccess || hipEventCreate(&n->ev1) != hipSuccess) return nullptr;
+        if (hipEventCreate(&n->ev0) != hipSuccess || hipEventCreate(&n->ev1) != hipSuccess ||
+            hipEventCreateWithFlags(&n->e_up_ev, hipEventDisableTiming) != hipSuccess)
+            return nullptr;
         void* sp = nullptr;
         const hipError_t e = hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch));
         if (e != hipSuccess) {
@@ -375,6 +386,37 @@ static int ensure_fp32_target(Target& t, hipStream_t s)
 }
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// host staging of one upload: regions appended at 256-byte alignment
+struct Arena {
+    std::vector<unsigned char> h;
+    size_t add(const void* p, size_t n)
+    {
+        const size_t o = align256(h.size());
+        h.resize(o + n);
+        if (n) std::memcpy(h.data() + o, p, n);
+        return o;
+    }
+    template <typename T>
+    size_t add(const std::vector<T>& v)
+    {
+        return add(v.data(), v.size() * sizeof(T));
+    }
+};
+
+static int pinned_ensure(void** p, size_t* n, size_t bytes)
+{
+    if (*p && *n >= bytes) return 0;
+    if (*p) (void)hipHostFree(*p);
+    *p = nullptr;
+    *n = 0;
+    if (hipHostMalloc(p, bytes ? bytes : 256, hipHostMallocDefault) != hipSuccess) {
+        *p = nullptr;
+        return -1;
+    }
+    *n = bytes;
+    return 0;
+}
 
 // states per column of the tiled kernel: psi (f), + lam (gradient), + dpsi, dlam (HVP)
 static int nstate_tile(int mode) { return mode == 2 ? 4 : mode == 1 ? 2 : 1; }
@@ -652,6 +694,38 @@ struct SweepMem {
     }
 };
 
+// Sends an evaluation's staged descriptors to the device in one copy and returns the device
+// image.  From pinned staging the copy is asynchronous (a pageable copy stages through the driver
+// and blocks the host, per region); the allocation-free evaluation, which allocates no pinned
+// memory, copies from the host image and waits for it.
+static int upload_flush(Context* c, SweepMem& ar, const Arena& up, hipStream_t s, unsigned char** dev)
+{
+    void* d = nullptr;
+    const size_t n = std::max<size_t>(up.h.size(), 256);
+    if (ar.get(c->e_up, n, &d) < 0) return -1;
+    if (!up.h.empty()) {
+        if (ar.base) {
+            MG_HIP(hipMemcpyAsync(d, up.h.data(), up.h.size(), hipMemcpyHostToDevice, s));
+            MG_HIP(hipStreamSynchronize(s));
+        } else {
+            // the staging is reused: the previous evaluation's copy out of it has completed (it
+            // synchronised at its end, unless it failed in between)
+            if (c->e_up_pending) MG_HIP(hipEventSynchronize(c->e_up_ev));
+            c->e_up_pending = false;
+            if (pinned_ensure(&c->e_up_h, &c->e_up_hn, n) < 0) {
+                set_error("pinned staging allocation failed");
+                return -1;
+            }
+            std::memcpy(c->e_up_h, up.h.data(), up.h.size());
+            MG_HIP(hipMemcpyAsync(d, c->e_up_h, up.h.size(), hipMemcpyHostToDevice, s));
+            MG_HIP(hipEventRecord(c->e_up_ev, s));
+            c->e_up_pending = true;
+        }
+    }
+    *dev = (unsigned char*)d;
+    return 0;
+}
+
 template <typename R>
 static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, int mode, long b0, long b1, EvalOut* out,
                      EvalStats* st, SweepMem& ar)
@@ -677,11 +751,10 @@ static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, in
     // coefficients in precision R
     std::vector<cpx<R>> coefR(b.coef.size());
     for (size_t k = 0; k < coefR.size(); k++) coefR[k] = {(R)b.coef[k].re, (R)b.coef[k].im};
-    void *d_inst = nullptr, *d_coef = nullptr;
-    if (ar.get(c->inst, b.inst.size() * sizeof(Inst), &d_inst) < 0 || ar.get(c->coef, coefR.size() * sizeof(cpx<R>), &d_coef) < 0)
-        return -1;
-    MG_HIP(hipMemcpyAsync(d_inst, b.inst.data(), b.inst.size() * sizeof(Inst), hipMemcpyHostToDevice, s));
-    MG_HIP(hipMemcpyAsync(d_coef, coefR.data(), coefR.size() * sizeof(cpx<R>), hipMemcpyHostToDevice, s));
+    // every descriptor of the evaluation is staged here (u_* offsets) and sent in one copy before
+    // the first kernel
+    Arena up;
+    const size_t u_inst = up.add(b.inst), u_coef = up.add(coefR);
 
     // checkpointing: needed when psi_g cannot be recovered by V^{-1} (non-unitary gates)
     bool ckpt = false;
@@ -861,8 +934,7 @@ static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, in
     a.L = L;
     a.G = p.G;
     a.P = p.P;
-    a.inst = (const Inst*)d_inst;
-    a.coef = (const cpx<R>*)d_coef;
+    size_t ua_inst = u_inst, ua_coef = u_coef;  // a.inst, a.coef in the upload
     a.target_kind = t.kind;
     if (t.kind == 0) {
         a.seed = std::is_same<R, double>::value ? (const cpx<R>*)t.seed64.p : (const cpx<R>*)t.seed32.p;
@@ -878,9 +950,19 @@ static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, in
     a.ckpt = (cpx<R>*)d_ckpt;
     a.nstate = reg_T ? reg_nst : nstate;
     a.lds_state_offset = (int)(reg_T ? head_reg : head);
+    // the L=12 forward-pass windows' matrices, formed on the device from this evaluation's
+    // coefficients (launch_window_frags) once the upload is there: window gates at u_start and
+    // u_gate, FP64 coefficients in program order at u_c64
+    struct {
+        int nwin = 0;
+        size_t u_start = 0, u_gate = 0, u_c64 = 0;
+        double* out64 = nullptr;
+        float* out32 = nullptr;
+    } frag;
     // FP32 sweeps with the states in LDS at L = 12: the forward pass (psi, and dpsi for the HVP) on
     // the matrix cores over 4-qubit windows; the instances go to the kernel in the windows' order (a
     // topological order of the program: the same product, and the reverse sweep runs it backwards)
+    size_t ua_fwin = 0;
     if (fp32 && lds_states && !ckpt && !reg_T && !use_win && !use_tile && L == 12 && kn.fwd_mfma != 0 && NW <= 16) {
         std::vector<int> o2;
         const int nwin = forward_window_form(b.inst, false, &c->h_dwin, &c->h_wstart, &c->h_wgate, &o2);
@@ -895,54 +977,34 @@ static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, in
             std::copy(coefR.begin() + (size_t)o2[k] * kCoefPerInst, coefR.begin() + (size_t)(o2[k] + 1) * kCoefPerInst,
                       wc.begin() + k * kCoefPerInst);
         }
-        void *d_wi = nullptr, *d_wc = nullptr, *d_fw = nullptr, *d_fa = nullptr, *d_fg = nullptr, *d_c64 = nullptr;
-        const size_t ngate_bytes = c->h_wstart.size() * sizeof(int), gate_bytes = c->h_wgate.size() * sizeof(WinGate);
-        if (ar.get(c->w_inst, wi.size() * sizeof(Inst), &d_wi) < 0 || ar.get(c->w_coef, wc.size() * sizeof(cpx<R>), &d_wc) < 0 ||
-            ar.get(c->w_fwin, nwin * sizeof(FWin), &d_fw) < 0 ||
-            ar.get(c->w_fwaf, (size_t)nwin * (mode == 2 ? 16 : 8) * 64 * sizeof(float), &d_fa) < 0 ||
-            ar.get(c->w_fgate, ngate_bytes + gate_bytes, &d_fg) < 0 ||
-            ar.get(c->w_fcoef, b.coef.size() * sizeof(cpx<double>), &d_c64) < 0)
-            return -1;
-        MG_HIP(hipMemcpyAsync(d_wi, wi.data(), wi.size() * sizeof(Inst), hipMemcpyHostToDevice, s));
-        MG_HIP(hipMemcpyAsync(d_wc, wc.data(), wc.size() * sizeof(cpx<R>), hipMemcpyHostToDevice, s));
-        MG_HIP(hipMemcpyAsync(d_fw, c->h_dwin.data(), nwin * sizeof(FWin), hipMemcpyHostToDevice, s));
-        MG_HIP(hipMemcpyAsync(d_fg, c->h_wstart.data(), ngate_bytes, hipMemcpyHostToDevice, s));
-        MG_HIP(hipMemcpyAsync((unsigned char*)d_fg + ngate_bytes, c->h_wgate.data(), gate_bytes, hipMemcpyHostToDevice, s));
-        MG_HIP(hipMemcpyAsync(d_c64, b.coef.data(), b.coef.size() * sizeof(cpx<double>), hipMemcpyHostToDevice, s));
-        // the window matrices of this evaluation's gates (and direction), on the device
-        MG_HIP(launch_window_frags(nwin, (const int*)d_fg, (const WinGate*)((unsigned char*)d_fg + ngate_bytes),
-                                   (const cpx<double>*)d_c64, mode == 2, nullptr, (float*)d_fa, s));
-        // the copies above read pageable host vectors: complete them before the vectors go away
-        MG_HIP(hipStreamSynchronize(s));
-        a.inst = (const Inst*)d_wi;
-        a.coef = (const cpx<R>*)d_wc;
-        a.fwin = d_fw;
+        void* d_fa = nullptr;
+        if (ar.get(c->w_fwaf, (size_t)nwin * (mode == 2 ? 16 : 8) * 64 * sizeof(float), &d_fa) < 0) return -1;
+        ua_inst = up.add(wi);
+        ua_coef = up.add(wc);
+        ua_fwin = up.add(c->h_dwin.data(), nwin * sizeof(FWin));
+        frag.nwin = nwin;
+        frag.u_start = up.add(c->h_wstart);
+        frag.u_gate = up.add(c->h_wgate);
+        frag.u_c64 = up.add(b.coef);
+        frag.out32 = (float*)d_fa;
         a.fwaf = (const float*)d_fa;
         a.nwin = nwin;
     }
 
     WinSweepArgs wa{};
+    size_t uw_inst = 0, uw_coef = 0, uw_blk = 0, uw_glob = 0, uw_fwin = 0;  // wa's descriptors in the upload
     if (use_win) {
         const Windowed& w = w_win;
         std::vector<cpx<R>> wc(w.coef.size());
         for (size_t k = 0; k < wc.size(); k++) wc[k] = {(R)w.coef[k].re, (R)w.coef[k].im};
-        void *d_wi = nullptr, *d_wc = nullptr, *d_wb = nullptr;
-        if (ar.get(c->w_inst, w.inst.size() * sizeof(Inst), &d_wi) < 0 || ar.get(c->w_coef, wc.size() * sizeof(cpx<R>), &d_wc) < 0 ||
-            ar.get(c->w_blk, 2 * w.blk_lds.size() * sizeof(WBlock), &d_wb) < 0)
-            return -1;
-        MG_HIP(hipMemcpyAsync(d_wi, w.inst.data(), w.inst.size() * sizeof(Inst), hipMemcpyHostToDevice, s));
-        MG_HIP(hipMemcpyAsync(d_wc, wc.data(), wc.size() * sizeof(cpx<R>), hipMemcpyHostToDevice, s));
-        MG_HIP(hipMemcpyAsync(d_wb, w.blk_lds.data(), w.blk_lds.size() * sizeof(WBlock), hipMemcpyHostToDevice, s));
-        MG_HIP(hipMemcpyAsync((WBlock*)d_wb + w.blk_lds.size(), w.blk_glob.data(), w.blk_glob.size() * sizeof(WBlock),
-                              hipMemcpyHostToDevice, s));
+        uw_inst = up.add(w.inst);
+        uw_coef = up.add(wc);
+        uw_blk = up.add(w.blk_lds);
+        uw_glob = up.add(w.blk_glob);
         wa.L = L;
         wa.G = p.G;
         wa.P = p.P;
         wa.nblocks = (int)w.blk_lds.size();
-        wa.inst = (const Inst*)d_wi;
-        wa.coef = d_wc;
-        wa.blk_lds = (const WBlock*)d_wb;
-        wa.blk_glob = (const WBlock*)d_wb + w.blk_lds.size();
         wa.seed = fp32 ? t.seed32.p : t.seed64.p;
         wa.b0 = b0;
         wa.b1 = b1;
@@ -961,50 +1023,31 @@ static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, in
                 set_error("window sweep: forward window blocking failed");
                 return -1;
             }
-            void *d_fw = nullptr, *d_fa = nullptr, *d_fg = nullptr;
-            const size_t ngate_bytes = c->h_wstart.size() * sizeof(int), gate_bytes = c->h_wgate.size() * sizeof(WinGate);
-            if (ar.get(c->w_fwin, nwin * sizeof(DWin), &d_fw) < 0 ||
-                ar.get(c->w_fwaf, (size_t)nwin * (mode == 2 ? 16 : 8) * 64 * sizeof(double), &d_fa) < 0 ||
-                ar.get(c->w_fgate, ngate_bytes + gate_bytes, &d_fg) < 0)
-                return -1;
-            MG_HIP(hipMemcpyAsync(d_fw, c->h_dwin.data(), nwin * sizeof(DWin), hipMemcpyHostToDevice, s));
-            MG_HIP(hipMemcpyAsync(d_fg, c->h_wstart.data(), ngate_bytes, hipMemcpyHostToDevice, s));
-            MG_HIP(hipMemcpyAsync((unsigned char*)d_fg + ngate_bytes, c->h_wgate.data(), gate_bytes, hipMemcpyHostToDevice, s));
-            // the sweep's coefficients in program order (R = double here)
-            MG_HIP(launch_window_frags(nwin, (const int*)d_fg, (const WinGate*)((unsigned char*)d_fg + ngate_bytes),
-                                       (const cpx<double>*)d_coef, mode == 2, (double*)d_fa, nullptr, s));
-            wa.fwin = d_fw;
+            void* d_fa = nullptr;
+            if (ar.get(c->w_fwaf, (size_t)nwin * (mode == 2 ? 16 : 8) * 64 * sizeof(double), &d_fa) < 0) return -1;
+            uw_fwin = up.add(c->h_dwin.data(), nwin * sizeof(DWin));
+            frag.nwin = nwin;
+            frag.u_start = up.add(c->h_wstart);
+            frag.u_gate = up.add(c->h_wgate);
+            frag.u_c64 = u_coef;  // the sweep's coefficients in program order (R = double here)
+            frag.out64 = (double*)d_fa;
             wa.fwaf = (const double*)d_fa;
             wa.nfwin = nwin;
         }
-        // the copies above read pageable host vectors: complete them before w goes away
-        MG_HIP(hipStreamSynchronize(s));
     }
     WinTileArgs wta{};
+    size_t ut_inst = 0, ut_coef = 0, ut_lds = 0, ut_glob = 0, ut_tb = 0;  // wta's descriptors in the upload
     if (use_wtile) {
         const WindowedTiles& w = w_wt;
-        void *d_wi = nullptr, *d_wc = nullptr, *d_wb = nullptr;
-        if (ar.get(c->w_inst, w.inst.size() * sizeof(Inst), &d_wi) < 0 ||
-            ar.get(c->w_coef, w.coef.size() * sizeof(cpx<double>), &d_wc) < 0 ||
-            ar.get(c->w_blk, 2 * w.blk_lds.size() * sizeof(WBlock) + w.tblocks.size() * sizeof(WTBlock), &d_wb) < 0)
-            return -1;
-        unsigned char* blk = (unsigned char*)d_wb;
-        MG_HIP(hipMemcpyAsync(d_wi, w.inst.data(), w.inst.size() * sizeof(Inst), hipMemcpyHostToDevice, s));
-        MG_HIP(hipMemcpyAsync(d_wc, w.coef.data(), w.coef.size() * sizeof(cpx<double>), hipMemcpyHostToDevice, s));
-        MG_HIP(hipMemcpyAsync(blk, w.blk_lds.data(), w.blk_lds.size() * sizeof(WBlock), hipMemcpyHostToDevice, s));
-        MG_HIP(hipMemcpyAsync(blk + w.blk_lds.size() * sizeof(WBlock), w.blk_glob.data(), w.blk_glob.size() * sizeof(WBlock),
-                              hipMemcpyHostToDevice, s));
-        MG_HIP(hipMemcpyAsync(blk + 2 * w.blk_lds.size() * sizeof(WBlock), w.tblocks.data(),
-                              w.tblocks.size() * sizeof(WTBlock), hipMemcpyHostToDevice, s));
+        ut_inst = up.add(w.inst);
+        ut_coef = up.add(w.coef);
+        ut_lds = up.add(w.blk_lds);
+        ut_glob = up.add(w.blk_glob);
+        ut_tb = up.add(w.tblocks);
         wta.L = L;
         wta.G = p.G;
         wta.P = p.P;
         wta.ntb = (int)w.tblocks.size();
-        wta.inst = (const Inst*)d_wi;
-        wta.coef = d_wc;
-        wta.tblocks = (const WTBlock*)(blk + 2 * w.blk_lds.size() * sizeof(WBlock));
-        wta.blk_lds = (const WBlock*)blk;
-        wta.blk_glob = (const WBlock*)(blk + w.blk_lds.size() * sizeof(WBlock));
         wta.seed = t.seed64.p;
         wta.b0 = b0;
         wta.b1 = b1;
@@ -1015,9 +1058,10 @@ static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, in
         wta.buf_off = (int)wt_buf;
         wta.rs = wt_rs;
         wta.slots = wt_slots;
-        MG_HIP(hipStreamSynchronize(s));
     }
     TileArgs<R> ta{};
+    size_t ut_fwaf = 0, ut_slot = 0, ut_bpar = 0;  // ta's descriptors in the upload (with ut_inst ...)
+    bool ut_fw = false;
     if (use_tile && !use_wtile) {
         std::vector<int> order;
         std::vector<TBlock> blocks;
@@ -1086,32 +1130,17 @@ static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, in
                     tc[(size_t)k * kCoefPerInst + e] = {(R)v.re, (R)v.im};
                 }
             }
-        void *d_wi = nullptr, *d_wc = nullptr, *d_wb = nullptr;
-        if (ar.get(c->w_inst, ti.size() * sizeof(Inst), &d_wi) < 0 || ar.get(c->w_coef, tc.size() * sizeof(cpx<R>), &d_wc) < 0 ||
-            ar.get(c->w_blk, blocks.size() * sizeof(TBlock), &d_wb) < 0)
-            return -1;
-        MG_HIP(hipMemcpyAsync(d_wi, ti.data(), ti.size() * sizeof(Inst), hipMemcpyHostToDevice, s));
-        MG_HIP(hipMemcpyAsync(d_wc, tc.data(), tc.size() * sizeof(cpx<R>), hipMemcpyHostToDevice, s));
-        MG_HIP(hipMemcpyAsync(d_wb, blocks.data(), blocks.size() * sizeof(TBlock), hipMemcpyHostToDevice, s));
+        ut_inst = up.add(ti);
+        ut_coef = up.add(tc);
+        ut_tb = up.add(blocks);
         if (!fwins.empty()) {
-            void *d_fw = nullptr, *d_fa = nullptr;
-            if (ar.get(c->w_fwin, fwins.size() * sizeof(FWin), &d_fw) < 0 ||
-                ar.get(c->w_fwaf, fwaf.size() * sizeof(float), &d_fa) < 0)
-                return -1;
-            MG_HIP(hipMemcpyAsync(d_fw, fwins.data(), fwins.size() * sizeof(FWin), hipMemcpyHostToDevice, s));
-            MG_HIP(hipMemcpyAsync(d_fa, fwaf.data(), fwaf.size() * sizeof(float), hipMemcpyHostToDevice, s));
-            ta.fwin = d_fw;
-            ta.fwaf = (const float*)d_fa;
+            uw_fwin = up.add(fwins);
+            ut_fwaf = up.add(fwaf);
+            ut_fw = true;
         }
         if (tile_slots) {
-            void *d_sl = nullptr, *d_bp = nullptr;
-            if (ar.get(c->w_slot, tb_slot.size() * sizeof(int), &d_sl) < 0 ||
-                ar.get(c->w_bpar, tb_param.size() * sizeof(int), &d_bp) < 0)
-                return -1;
-            MG_HIP(hipMemcpyAsync(d_sl, tb_slot.data(), tb_slot.size() * sizeof(int), hipMemcpyHostToDevice, s));
-            MG_HIP(hipMemcpyAsync(d_bp, tb_param.data(), tb_param.size() * sizeof(int), hipMemcpyHostToDevice, s));
-            ta.slot = (const int*)d_sl;
-            ta.bparam = (const int*)d_bp;
+            ut_slot = up.add(tb_slot);
+            ut_bpar = up.add(tb_param);
             int most = 0;
             for (const TBlock& bk : blocks) most = std::max(most, bk.ns);
             ta.nacc = most;
@@ -1121,9 +1150,6 @@ static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, in
         ta.P = p.P;
         ta.kw = tile_kw;
         ta.nblocks = (int)blocks.size();
-        ta.inst = (const Inst*)d_wi;
-        ta.coef = (const cpx<R>*)d_wc;
-        ta.blocks = (const TBlock*)d_wb;
         ta.seed = fp32 ? nullptr : (const cpx<R>*)t.seed64.p;
         ta.seed64 = fp32 ? (const cpx<double>*)t.seed64.p : nullptr;
         ta.b0 = b0;
@@ -1132,9 +1158,43 @@ static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, in
         ta.scratch = (cpx<R>*)d_scratch;
         ta.nstate = nstate_tile(mode);
         ta.tile_off = (int)tile_off;
-        // the copies above read pageable host vectors: complete them before they go away
-        MG_HIP(hipStreamSynchronize(s));
     }
+
+    // the upload, then the device pointers into it, and the forward windows' matrices
+    unsigned char* dev = nullptr;
+    if (upload_flush(c, ar, up, s, &dev) < 0) return -1;
+    a.inst = (const Inst*)(dev + ua_inst);
+    a.coef = (const cpx<R>*)(dev + ua_coef);
+    if (frag.out32) a.fwin = dev + ua_fwin;
+    if (use_win) {
+        wa.inst = (const Inst*)(dev + uw_inst);
+        wa.coef = dev + uw_coef;
+        wa.blk_lds = (const WBlock*)(dev + uw_blk);
+        wa.blk_glob = (const WBlock*)(dev + uw_glob);
+        if (frag.out64) wa.fwin = dev + uw_fwin;
+    }
+    if (use_wtile) {
+        wta.inst = (const Inst*)(dev + ut_inst);
+        wta.coef = dev + ut_coef;
+        wta.blk_lds = (const WBlock*)(dev + ut_lds);
+        wta.blk_glob = (const WBlock*)(dev + ut_glob);
+        wta.tblocks = (const WTBlock*)(dev + ut_tb);
+    } else if (use_tile) {
+        ta.inst = (const Inst*)(dev + ut_inst);
+        ta.coef = (const cpx<R>*)(dev + ut_coef);
+        ta.blocks = (const TBlock*)(dev + ut_tb);
+        if (ut_fw) {
+            ta.fwin = dev + uw_fwin;
+            ta.fwaf = (const float*)(dev + ut_fwaf);
+        }
+        if (tile_slots) {
+            ta.slot = (const int*)(dev + ut_slot);
+            ta.bparam = (const int*)(dev + ut_bpar);
+        }
+    }
+    if (frag.nwin)
+        MG_HIP(launch_window_frags(frag.nwin, (const int*)(dev + frag.u_start), (const WinGate*)(dev + frag.u_gate),
+                                   (const cpx<double>*)(dev + frag.u_c64), mode == 2, frag.out64, frag.out32, s));
     if (kn.time_kernels) MG_HIP(hipEventRecord(c->ev0, s));
     if (chunk_rows) {
         for (long ci = 0; ci < nchunks; ci++) {
@@ -1173,11 +1233,19 @@ static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, in
         MG_HIP(launch_sweep<R>(a, mode, lds_states, ckpt, lds_T, (int)grid, lds, s));
     if (kn.time_kernels) MG_HIP(hipEventRecord(c->ev1, s));
     MG_HIP(launch_reduce((const double*)d_partial, (int)(grid * nchunks), stride, (double*)d_out, s));
-    c->out_h.resize(stride);
-    MG_HIP(hipMemcpyAsync(c->out_h.data(), d_out, stride * sizeof(double), hipMemcpyDeviceToHost, s));
+    // the result into pinned memory (the allocation-free evaluation: pageable)
+    double* oh = nullptr;
+    if (!ar.base && pinned_ensure(&c->e_out_h, &c->e_out_hn, stride * sizeof(double)) == 0) {
+        oh = (double*)c->e_out_h;
+    } else {
+        c->out_h.resize(stride);
+        oh = c->out_h.data();
+    }
+    MG_HIP(hipMemcpyAsync(oh, d_out, stride * sizeof(double), hipMemcpyDeviceToHost, s));
     MG_HIP(hipStreamSynchronize(s));
+    c->e_up_pending = false;
 
-    const double* o = c->out_h.data();
+    const double* o = oh;
     out->f = zd(o[0], o[1]);
     out->grad.assign((size_t)8 * p.P, zd(0.0));
     out->dhz.assign((size_t)8 * p.P, zd(0.0));
@@ -1265,20 +1333,6 @@ BatchStats& last_batch_stats()
 {
     static thread_local BatchStats s;
     return s;
-}
-
-static int pinned_ensure(void** p, size_t* n, size_t bytes)
-{
-    if (*p && *n >= bytes) return 0;
-    if (*p) (void)hipHostFree(*p);
-    *p = nullptr;
-    *n = 0;
-    if (hipHostMalloc(p, bytes ? bytes : 256, hipHostMallocDefault) != hipSuccess) {
-        *p = nullptr;
-        return -1;
-    }
-    *n = bytes;
-    return 0;
 }
 
 void batch_shape(int L, int precision, int* cgw, int* cpi)
@@ -1369,18 +1423,6 @@ int target_densify(Target& t)
     t.seed32_ready = false;
     return 0;
 }
-
-// host staging of one upload: regions appended at 256-byte alignment
-struct Arena {
-    std::vector<unsigned char> h;
-    size_t add(const void* p, size_t n)
-    {
-        const size_t o = align256(h.size());
-        h.resize(o + n);
-        if (n) std::memcpy(h.data() + o, p, n);
-        return o;
-    }
-};
 
 template <typename R>
 static int run_batch(Context* c, const std::vector<int>& idx, BatchReq* reqs, BatchStats* st)
