@@ -12,9 +12,10 @@
 // the tile back.  A brickwall needs a handful of such passes (4 for L=14, depth 11, FP64)
 // where one-gate or 3/4-qubit-window passes need one per gate or per 2-4 gates.
 //
-// Per column: forward blocks (psi, dpsi), f = seed . psi_G while the last forward block is
-// stored, then the reverse blocks (lam seeded from the dense target row in the first one, psi
-// and dpsi uncomputed with V^{-1}).  Partial sums per workgroup as the other sweep kernels.
+// Per column: forward blocks (psi, dpsi); each tile of the last forward block goes on at once
+// through the first reverse block (the same gates backwards), with f = seed . psi_G and lam seeded
+// from the dense target row read once for both; then the other reverse blocks (psi and dpsi
+// uncomputed with V^{-1}).  Partial sums per workgroup as the other sweep kernels.
 //
 // Reference: per-column loop src/matchgate_target.c:788-837, forward
 // src/matchgate_brickwall.c:52-78, backward kernel src/gate.c:659-772.
@@ -129,18 +130,99 @@ __global__ __launch_bounds__((1 << tile_lt<R, MODE>()), (tile_min_waves<R, MODE>
             return a.seed[rb + i];
         };
         double fr = 0.0, fi = 0.0;
-        // ---- forward: psi, dpsi ----------------------------------------------------------
-        for (int k = 0; k < nb; k++) {
+        // the reverse sweep of one tile through block bk's gates (lam, psi, dlam, dpsi in LDS)
+        auto reverse_tile = [&](const TBlock* bk) {
+            for (int g = bk->g1 - 1; g >= bk->g0; g--) {
+                const Inst in = a.inst[g];
+                cpx<R> cf[24];
+                load_coef_c<R, MODE == 2 ? 24 : 16>(a.coef + (size_t)g * kCoefPerInst, cf);
+                cpx<R> ga[8], ha[8];
+#pragma unroll
+                for (int e = 0; e < 8; e++) {
+                    ga[e] = {R(0), R(0)};
+                    ha[e] = {R(0), R(0)};
+                }
+                const uint32_t olo = swz_idx<R, true>(1u << in.lo), ohi = swz_idx<R, true>(1u << in.hi);
+                uint32_t q = tid;
+                if constexpr (2 * T <= Qt)
+                    for (; q + T < Qt; q += 2 * T)
+                        rev_quads<R, MODE, true, 2>(q, T, in.lo, in.hi, olo, ohi, cf, t_lam, t_psi, t_dlam, t_dpsi, ga, ha);
+                if (q < Qt)
+                    rev_quads<R, MODE, true, 1>(q, T, in.lo, in.hi, olo, ohi, cf, t_lam, t_psi, t_dlam, t_dpsi, ga, ha);
+                R v[NV];
+#pragma unroll
+                for (int e = 0; e < 8; e++) {
+                    v[2 * e] = ga[e].re;
+                    v[2 * e + 1] = ga[e].im;
+                    if constexpr (MODE == 2) {
+                        v[16 + 2 * e] = ha[e].re;
+                        v[16 + 2 * e + 1] = ha[e].im;
+                    }
+                }
+                wave_halve<NV, 32, R>(v, lane);
+                const int buf = g & 1;
+                if ((lane & ((1 << SHIFT) - 1)) == 0) stage[(buf * NW + wv) * NV + (lane >> SHIFT)] = (double)v[0];
+                __syncthreads();
+                if (wv == g % NW && lane < NV) {
+                    double s = 0.0;
+                    for (int w = 0; w < NW; w++) s += stage[(buf * NW + w) * NV + lane];
+                    const int ci = lane >> 1;
+                    int e = ci & 7;
+                    if (in.flip && e < 4) e = 3 - e;
+                    acc[NV * (a.slot ? a.slot[g] : in.param) + 2 * ((ci & 8) | e) + (lane & 1)] += s;
+                }
+            }
+        };
+        // the tile's states back to the column's slices (each thread its own positions)
+        auto store_reverse = [&](const TileMap<PER>& mp, uint32_t ot) {
+#pragma unroll
+            for (int m = 0; m < PER; m++) {
+                const uint32_t j = tid + m * T, gi = ot | mp.win_tid | mp.wm[m];
+                g_psi[gi] = sel<R, true>(t_psi, j);
+                g_lam[gi] = sel<R, true>(t_lam, j);
+                if constexpr (MODE == 2) {
+                    g_dpsi[gi] = sel<R, true>(t_dpsi, j);
+                    g_dlam[gi] = sel<R, true>(t_dlam, j);
+                }
+            }
+        };
+        // after a reverse block (per-block slots): this block's sums of this column into the
+        // workgroup's partial sums
+        auto flush_slots = [&](const TBlock* bk) {
+            if (a.slot) {
+                const int* bp = a.bparam + bk->s0;
+                for (int k2 = tid; k2 < NV * bk->ns; k2 += T) {
+                    out[2 + 32 * bp[k2 / NV] + (k2 % NV)] += acc[k2];
+                    acc[k2] = 0.0;
+                }
+                __syncthreads();
+            }
+        };
+        // ---- passes over the blocks: forward 0 .. nb-2 (psi, dpsi), then the last block forward
+        // and at once backwards, tile by tile (f = s_b . psi_G, lam seeded from the target row read
+        // once for both, no store and reload of psi between the two sweeps), then reverse nb-2 .. 0
+        // (lam, psi uncomputed, dlam, dpsi).  One loop, so that the forward and the reverse gate
+        // code each appear once (their registers are not live together).
+        const int npass = MODE >= 1 ? 2 * nb - 1 : nb;
+        for (int ps = 0; ps < npass; ps++) {
+            const int k = ps < nb ? ps : 2 * nb - 2 - ps;
+            const bool fwd = ps < nb, rev = MODE >= 1 && ps >= nb - 1;
             const TBlock* bk = a.blocks + k;
             TileMap<PER> mp;
             tile_map<KW, LT, PER>(bk, tid, &mp);
-            const int g0 = bk->g0, g1 = bk->g1;
             for (uint32_t t = 0; t < ntiles; t++) {
                 const uint32_t ot = scatter_bits(t, bk->other, L - KW);
 #pragma unroll
                 for (int m = 0; m < PER; m++) {
                     const uint32_t j = tid + m * T, gi = ot | mp.win_tid | mp.wm[m];
-                    if (k == 0) {
+                    if (!fwd) {
+                        sel<R, true>(t_psi, j) = g_psi[gi];
+                        if constexpr (MODE >= 1) sel<R, true>(t_lam, j) = g_lam[gi];
+                        if constexpr (MODE == 2) {
+                            sel<R, true>(t_dpsi, j) = g_dpsi[gi];
+                            sel<R, true>(t_dlam, j) = g_dlam[gi];
+                        }
+                    } else if (k == 0) {
                         sel<R, true>(t_psi, j) = {R(gi == (uint32_t)b), R(0)};
                         if constexpr (MODE == 2) sel<R, true>(t_dpsi, j) = {R(0), R(0)};
                     } else {
@@ -149,49 +231,69 @@ __global__ __launch_bounds__((1 << tile_lt<R, MODE>()), (tile_min_waves<R, MODE>
                     }
                 }
                 __syncthreads();
-                if constexpr (sizeof(R) == 4) {
-                    if (a.fwin) {
-                        if constexpr (MODE == 2)
-                            fwd_windows_hvp<NW>(reinterpret_cast<const FWin*>(a.fwin), a.fwaf, bk->w0, bk->w1, t_psi,
-                                                t_dpsi, lane, wv);
-                        else
-                            fwd_windows<NW>(reinterpret_cast<const FWin*>(a.fwin), a.fwaf, bk->w0, bk->w1, t_psi, lane, wv);
-                        goto forward_done;
+                if (fwd) {
+                    if constexpr (sizeof(R) == 4) {
+                        if (a.fwin) {
+                            if constexpr (MODE == 2)
+                                fwd_windows_hvp<NW>(reinterpret_cast<const FWin*>(a.fwin), a.fwaf, bk->w0, bk->w1, t_psi,
+                                                    t_dpsi, lane, wv);
+                            else
+                                fwd_windows<NW>(reinterpret_cast<const FWin*>(a.fwin), a.fwaf, bk->w0, bk->w1, t_psi, lane,
+                                                wv);
+                            goto forward_done;
+                        }
                     }
-                }
-                for (int g = g0; g < g1; g++) {
-                    const Inst in = a.inst[g];
-                    cpx<R> cf[24];
-                    load_coef_c<R, 8>(a.coef + (size_t)g * kCoefPerInst, cf);
-                    if constexpr (MODE == 2) load_coef_c<R, 8>(a.coef + (size_t)g * kCoefPerInst + 16, cf + 16);
-                    const uint32_t olo = swz_idx<R, true>(1u << in.lo), ohi = swz_idx<R, true>(1u << in.hi);
-                    uint32_t q = tid;
-                    if constexpr (MODE == 2 && 2 * T <= Qt)
-                        for (; q + T < Qt; q += 2 * T)
-                            fwd_quads<R, MODE, true, 2>(q, T, in.lo, in.hi, olo, ohi, cf, t_psi, t_psi, t_dpsi, t_dpsi);
-                    for (; q < Qt; q += T)
-                        fwd_quads<R, MODE, true, 1>(q, T, in.lo, in.hi, olo, ohi, cf, t_psi, t_psi, t_dpsi, t_dpsi);
-                    __syncthreads();
-                }
-            forward_done:
-                // store (each thread its own positions, so the next tile's load needs no barrier);
-                // f_b = s_b . psi_G over the last block's tiles
+                    for (int g = bk->g0; g < bk->g1; g++) {
+                        const Inst in = a.inst[g];
+                        cpx<R> cf[24];
+                        load_coef_c<R, 8>(a.coef + (size_t)g * kCoefPerInst, cf);
+                        if constexpr (MODE == 2) load_coef_c<R, 8>(a.coef + (size_t)g * kCoefPerInst + 16, cf + 16);
+                        const uint32_t olo = swz_idx<R, true>(1u << in.lo), ohi = swz_idx<R, true>(1u << in.hi);
+                        uint32_t q = tid;
+                        if constexpr (MODE == 2 && 2 * T <= Qt)
+                            for (; q + T < Qt; q += 2 * T)
+                                fwd_quads<R, MODE, true, 2>(q, T, in.lo, in.hi, olo, ohi, cf, t_psi, t_psi, t_dpsi, t_dpsi);
+                        for (; q < Qt; q += T)
+                            fwd_quads<R, MODE, true, 1>(q, T, in.lo, in.hi, olo, ohi, cf, t_psi, t_psi, t_dpsi, t_dpsi);
+                        __syncthreads();
+                    }
+                forward_done:
+                    if (k < nb - 1) {
+                        // store (each thread its own positions, so the next tile's load needs no barrier)
 #pragma unroll
-                for (int m = 0; m < PER; m++) {
-                    const uint32_t j = tid + m * T, gi = ot | mp.win_tid | mp.wm[m];
-                    const cpx<R> p = sel<R, true>(t_psi, j);
-                    if (k == nb - 1) {
-                        const cpx<R> tt = cmul(row(gi), p);
+                        for (int m = 0; m < PER; m++) {
+                            const uint32_t j = tid + m * T, gi = ot | mp.win_tid | mp.wm[m];
+                            g_psi[gi] = sel<R, true>(t_psi, j);
+                            if constexpr (MODE == 2) g_dpsi[gi] = sel<R, true>(t_dpsi, j);
+                        }
+                        continue;
+                    }
+                    // the last block: f_b = s_b . psi_G, and lam = s_b, dlam = 0 for the reverse sweep
+#pragma unroll
+                    for (int m = 0; m < PER; m++) {
+                        const uint32_t j = tid + m * T, gi = ot | mp.win_tid | mp.wm[m];
+                        const cpx<R> r = row(gi);
+                        const cpx<R> tt = cmul(r, sel<R, true>(t_psi, j));
                         fr += (double)tt.re;
                         fi += (double)tt.im;
-                        if constexpr (MODE == 0) continue;
+                        if constexpr (MODE >= 1) sel<R, true>(t_lam, j) = r;
+                        if constexpr (MODE == 2) sel<R, true>(t_dlam, j) = {R(0), R(0)};
                     }
-                    g_psi[gi] = p;
-                    if constexpr (MODE == 2) g_dpsi[gi] = sel<R, true>(t_dpsi, j);
+                    if constexpr (MODE == 0) continue;
+                    __syncthreads();
+                }
+                if constexpr (MODE >= 1) {
+                    if (rev) {
+                        reverse_tile(bk);
+                        if (k > 0) store_reverse(mp, ot);
+                    }
                 }
             }
             // the next block reads, in its own mapping, what other threads stored here
             __syncthreads();
+            if constexpr (MODE >= 1) {
+                if (rev) flush_slots(bk);
+            }
         }
         {
             const double sr = wave_sum(fr), si = wave_sum(fi);
@@ -208,92 +310,6 @@ __global__ __launch_bounds__((1 << tile_lt<R, MODE>()), (tile_min_waves<R, MODE>
                 }
                 fre += tr;
                 fim += ti;
-            }
-        }
-        if constexpr (MODE >= 1) {
-            // ---- reverse: lam (seeded), psi uncomputed, dlam, dpsi ---------------------------
-            for (int k = nb - 1; k >= 0; k--) {
-                const TBlock* bk = a.blocks + k;
-                TileMap<PER> mp;
-                tile_map<KW, LT, PER>(bk, tid, &mp);
-                const int g0 = bk->g0, g1 = bk->g1;
-                for (uint32_t t = 0; t < ntiles; t++) {
-                    const uint32_t ot = scatter_bits(t, bk->other, L - KW);
-#pragma unroll
-                    for (int m = 0; m < PER; m++) {
-                        const uint32_t j = tid + m * T, gi = ot | mp.win_tid | mp.wm[m];
-                        sel<R, true>(t_psi, j) = g_psi[gi];
-                        sel<R, true>(t_lam, j) = k == nb - 1 ? row(gi) : g_lam[gi];
-                        if constexpr (MODE == 2) {
-                            sel<R, true>(t_dpsi, j) = g_dpsi[gi];
-                            sel<R, true>(t_dlam, j) = k == nb - 1 ? cpx<R>{R(0), R(0)} : g_dlam[gi];
-                        }
-                    }
-                    __syncthreads();
-                    for (int g = g1 - 1; g >= g0; g--) {
-                        const Inst in = a.inst[g];
-                        cpx<R> cf[24];
-                        load_coef_c<R, MODE == 2 ? 24 : 16>(a.coef + (size_t)g * kCoefPerInst, cf);
-                        cpx<R> ga[8], ha[8];
-#pragma unroll
-                        for (int e = 0; e < 8; e++) {
-                            ga[e] = {R(0), R(0)};
-                            ha[e] = {R(0), R(0)};
-                        }
-                        const uint32_t olo = swz_idx<R, true>(1u << in.lo), ohi = swz_idx<R, true>(1u << in.hi);
-                        uint32_t q = tid;
-                        if constexpr (2 * T <= Qt)
-                            for (; q + T < Qt; q += 2 * T)
-                                rev_quads<R, MODE, true, 2>(q, T, in.lo, in.hi, olo, ohi, cf, t_lam, t_psi, t_dlam, t_dpsi, ga, ha);
-                        if (q < Qt)
-                            rev_quads<R, MODE, true, 1>(q, T, in.lo, in.hi, olo, ohi, cf, t_lam, t_psi, t_dlam, t_dpsi, ga, ha);
-                        R v[NV];
-#pragma unroll
-                        for (int e = 0; e < 8; e++) {
-                            v[2 * e] = ga[e].re;
-                            v[2 * e + 1] = ga[e].im;
-                            if constexpr (MODE == 2) {
-                                v[16 + 2 * e] = ha[e].re;
-                                v[16 + 2 * e + 1] = ha[e].im;
-                            }
-                        }
-                        wave_halve<NV, 32, R>(v, lane);
-                        const int buf = g & 1;
-                        if ((lane & ((1 << SHIFT) - 1)) == 0) stage[(buf * NW + wv) * NV + (lane >> SHIFT)] = (double)v[0];
-                        __syncthreads();
-                        if (wv == g % NW && lane < NV) {
-                            double s = 0.0;
-                            for (int w = 0; w < NW; w++) s += stage[(buf * NW + w) * NV + lane];
-                            const int ci = lane >> 1;
-                            int e = ci & 7;
-                            if (in.flip && e < 4) e = 3 - e;
-                            acc[NV * (a.slot ? a.slot[g] : in.param) + 2 * ((ci & 8) | e) + (lane & 1)] += s;
-                        }
-                    }
-                    if (k > 0) {
-#pragma unroll
-                        for (int m = 0; m < PER; m++) {
-                            const uint32_t j = tid + m * T, gi = ot | mp.win_tid | mp.wm[m];
-                            g_psi[gi] = sel<R, true>(t_psi, j);
-                            g_lam[gi] = sel<R, true>(t_lam, j);
-                            if constexpr (MODE == 2) {
-                                g_dpsi[gi] = sel<R, true>(t_dpsi, j);
-                                g_dlam[gi] = sel<R, true>(t_dlam, j);
-                            }
-                        }
-                    }
-                }
-                // the next block reads, in its own mapping, what other threads stored here
-                __syncthreads();
-                if (a.slot) {
-                    // this block's sums of this column into the workgroup's partial sums
-                    const int* bp = a.bparam + bk->s0;
-                    for (int k2 = tid; k2 < NV * bk->ns; k2 += T) {
-                        out[2 + 32 * bp[k2 / NV] + (k2 % NV)] += acc[k2];
-                        acc[k2] = 0.0;
-                    }
-                    __syncthreads();
-                }
             }
         }
         __syncthreads();
