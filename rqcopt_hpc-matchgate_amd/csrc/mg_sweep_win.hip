@@ -458,57 +458,66 @@ __global__ __launch_bounds__(512) void sweep_win_tile_kernel(WinTileArgs a)
         double fr = 0.0, fi = 0.0;
         const cpx<R>* row = MODE == 3 ? nullptr : seed + (size_t)(b - a.row0) * S;
         cpx<R>* f_psi = MODE == 3 ? reinterpret_cast<cpx<R>*>(a.rows) + (size_t)(b - a.row0) * S : g_psi;
-        // ---- forward: psi, dpsi, tile block by tile block ------------------------------------
-        for (int tb = 0; tb < ntb; tb++) {
+        // ---- passes over the tile blocks: forward 0 .. ntb-2 (psi, dpsi); the last tile block
+        // forward and at once backwards, tile by tile (f_b = s_b . psi_G, lam seeded from the target
+        // row read once for both, the states handed over in registers); then reverse ntb-2 .. 0 (lam,
+        // psi uncomputed, dlam, dpsi).  One loop, so that the forward and the reverse code each
+        // appear once.
+        constexpr bool kRev = MODE == 1 || MODE == 2;
+        const int npass = kRev ? 2 * ntb - 1 : ntb;
+        for (int ps = 0; ps < npass; ps++) {
+            const bool fwd = ps < ntb;
+            const int tb = fwd ? ps : 2 * ntb - 2 - ps;
             const WTBlock* tbk = a.tblocks + tb;
             const int k0 = cptr(tbk)->k0, k1 = cptr(tbk)->k1;
             for (uint32_t t = 0; t < ntiles; t++) {
                 const uint32_t ot = scatter_c(t, tbk->other, L - TB);
-                {
-                    const BlockRegs<kW> bf = load_block<kW>(a.blk_glob + k0);
-                    const uint32_t gb = ot | scatter_c(deposit<kW>((uint32_t)tid, bf.pos), tbk->tpos, TB);
-                    if (tb == 0) {
+                if (fwd) {
+                    {
+                        const BlockRegs<kW> bf = load_block<kW>(a.blk_glob + k0);
+                        const uint32_t gb = ot | scatter_c(deposit<kW>((uint32_t)tid, bf.pos), tbk->tpos, TB);
+                        if (tb == 0) {
 #pragma unroll
-                        for (int j = 0; j < (1 << kW); j++) {
-                            P.a[j] = {(gb ^ bf.off[j]) == (uint32_t)b ? R(1) : R(0), R(0)};
-                            D.a[j] = {R(0), R(0)};
+                            for (int j = 0; j < (1 << kW); j++) {
+                                P.a[j] = {(gb ^ bf.off[j]) == (uint32_t)b ? R(1) : R(0), R(0)};
+                                D.a[j] = {R(0), R(0)};
+                            }
+                        } else {
+                            win_load<R, kW, false>(f_psi, gb, bf.off, P);
+                            if constexpr (MODE == 2) win_load<R, kW, false>(g_dpsi, gb, bf.off, D);
                         }
-                    } else {
-                        win_load<R, kW, false>(f_psi, gb, bf.off, P);
-                        if constexpr (MODE == 2) win_load<R, kW, false>(g_dpsi, gb, bf.off, D);
                     }
-                }
-                for (int k = k0; k < k1; k++) {
-                    const BlockRegs<kW> bk = load_block<kW>(a.blk_lds + k);
-                    for (int g = bk.g0; g < bk.g1; g++) {
-                        const int wcase = cptr(a.inst)[g].lo;
-                        cpx<R> cf[24];
-                        load_coef_c<R, 8>(coef + (size_t)g * kCoefPerInst, cf);
-                        if constexpr (MODE == 2) load_coef_c<R, 8>(coef + (size_t)g * kCoefPerInst + 16, cf + 16);
+                    for (int k = k0; k < k1; k++) {
+                        const BlockRegs<kW> bk = load_block<kW>(a.blk_lds + k);
+                        for (int g = bk.g0; g < bk.g1; g++) {
+                            const int wcase = cptr(a.inst)[g].lo;
+                            cpx<R> cf[24];
+                            load_coef_c<R, 8>(coef + (size_t)g * kCoefPerInst, cf);
+                            if constexpr (MODE == 2) load_coef_c<R, 8>(coef + (size_t)g * kCoefPerInst + 16, cf + 16);
 #define MG_FWD(WW, A, B) wg_forward<R, WW, A, B, MODE == 2 ? 2 : 0>(cf, P, D)
-                        MG_WCASES(kW, MG_FWD)
+                            MG_WCASES(kW, MG_FWD)
 #undef MG_FWD
-                    }
-                    if (k + 1 < k1) {
-                        const BlockRegs<kW> nx = load_block<kW>(a.blk_lds + k + 1);
-                        const uint32_t sb = lin<R, true>(deposit<kW>((uint32_t)tid, bk.pos));
-                        const uint32_t sn = lin<R, true>(deposit<kW>((uint32_t)tid, nx.pos));
-                        win_store<R, kW, true>(buf, sb, bk.off, P);
-                        if constexpr (MODE == 2 && RS >= 2) win_store<R, kW, true>(buf + TSZ, sb, bk.off, D);
-                        __syncthreads();
-                        win_load<R, kW, true>(buf, sn, nx.off, P);
-                        if constexpr (MODE == 2 && RS >= 2) win_load<R, kW, true>(buf + TSZ, sn, nx.off, D);
-                        __syncthreads();
-                        if constexpr (MODE == 2 && RS == 1) {
-                            win_store<R, kW, true>(buf, sb, bk.off, D);
+                        }
+                        if (k + 1 < k1) {
+                            const BlockRegs<kW> nx = load_block<kW>(a.blk_lds + k + 1);
+                            const uint32_t sb = lin<R, true>(deposit<kW>((uint32_t)tid, bk.pos));
+                            const uint32_t sn = lin<R, true>(deposit<kW>((uint32_t)tid, nx.pos));
+                            win_store<R, kW, true>(buf, sb, bk.off, P);
+                            if constexpr (MODE == 2 && RS >= 2) win_store<R, kW, true>(buf + TSZ, sb, bk.off, D);
                             __syncthreads();
-                            win_load<R, kW, true>(buf, sn, nx.off, D);
+                            win_load<R, kW, true>(buf, sn, nx.off, P);
+                            if constexpr (MODE == 2 && RS >= 2) win_load<R, kW, true>(buf + TSZ, sn, nx.off, D);
                             __syncthreads();
+                            if constexpr (MODE == 2 && RS == 1) {
+                                win_store<R, kW, true>(buf, sb, bk.off, D);
+                                __syncthreads();
+                                win_load<R, kW, true>(buf, sn, nx.off, D);
+                                __syncthreads();
+                            }
                         }
                     }
-                }
-                // back to global memory (the last tile block also forms f_b = s_b . psi_G)
-                {
+                    // the last window block's layout: back to global memory, or (last tile block)
+                    // f_b = s_b . psi_G and the reverse sweep's seed lam = s_b, dlam = 0
                     const BlockRegs<kW> bl = load_block<kW>(a.blk_glob + k1 - 1);
                     const uint32_t gb = ot | scatter_c(deposit<kW>((uint32_t)tid, bl.pos), tbk->tpos, TB);
                     if constexpr (MODE == 3) {
@@ -517,64 +526,35 @@ __global__ __launch_bounds__(512) void sweep_win_tile_kernel(WinTileArgs a)
                             for (int j = 0; j < (1 << kW); j++) P.a[j] = {-P.a[j].re, P.a[j].im};
                         }
                         win_store<R, kW, false>(f_psi, gb, bl.off, P);
-                    } else if (tb == ntb - 1) {
+                        continue;
+                    } else {
+                        if (tb < ntb - 1) {
+                            win_store<R, kW, false>(g_psi, gb, bl.off, P);
+                            if constexpr (MODE == 2) win_store<R, kW, false>(g_dpsi, gb, bl.off, D);
+                            continue;
+                        }
 #pragma unroll
                         for (int j = 0; j < (1 << kW); j++) {
-                            const cpx<R> tt = cmul(row[gb ^ bl.off[j]], P.a[j]);
+                            const cpx<R> r = row[gb ^ bl.off[j]];
+                            const cpx<R> tt = cmul(r, P.a[j]);
                             fr += tt.re;
                             fi += tt.im;
+                            Lm.a[j] = r;
+                            DL.a[j] = {R(0), R(0)};
                         }
+                        if constexpr (!kRev) continue;
                     }
-                    if (MODE == 1 || MODE == 2 || (MODE == 0 && tb < ntb - 1)) {
-                        win_store<R, kW, false>(g_psi, gb, bl.off, P);
-                        if constexpr (MODE == 2) win_store<R, kW, false>(g_dpsi, gb, bl.off, D);
+                } else {
+                    const BlockRegs<kW> bl = load_block<kW>(a.blk_glob + k1 - 1);
+                    const uint32_t gb = ot | scatter_c(deposit<kW>((uint32_t)tid, bl.pos), tbk->tpos, TB);
+                    win_load<R, kW, false>(g_psi, gb, bl.off, P);
+                    win_load<R, kW, false>(g_lam, gb, bl.off, Lm);
+                    if constexpr (MODE == 2) {
+                        win_load<R, kW, false>(g_dpsi, gb, bl.off, D);
+                        win_load<R, kW, false>(g_dlam, gb, bl.off, DL);
                     }
                 }
-            }
-            // the next tile block reads, in its own mapping, what other threads stored here
-            __syncthreads();
-        }
-        if constexpr (MODE == 3) continue;
-        {
-            const double sr = wave_sum(fr), si = wave_sum(fi);
-            if (lane == 0) {
-                fst[2 * wv] = sr;
-                fst[2 * wv + 1] = si;
-            }
-            __syncthreads();
-            if (tid == 0) {
-                double tr = 0.0, ti = 0.0;
-                for (int w = 0; w < kNW; w++) {
-                    tr += fst[2 * w];
-                    ti += fst[2 * w + 1];
-                }
-                fre += tr;
-                fim += ti;
-            }
-        }
-        if constexpr (MODE == 1 || MODE == 2) {
-            // ---- reverse, tile block by tile block ------------------------------------------
-            for (int tb = ntb - 1; tb >= 0; tb--) {
-                const WTBlock* tbk = a.tblocks + tb;
-                const int k0 = cptr(tbk)->k0, k1 = cptr(tbk)->k1;
-                for (uint32_t t = 0; t < ntiles; t++) {
-                    const uint32_t ot = scatter_c(t, tbk->other, L - TB);
-                    {
-                        const BlockRegs<kW> bl = load_block<kW>(a.blk_glob + k1 - 1);
-                        const uint32_t gb = ot | scatter_c(deposit<kW>((uint32_t)tid, bl.pos), tbk->tpos, TB);
-                        win_load<R, kW, false>(g_psi, gb, bl.off, P);
-                        if constexpr (MODE == 2) win_load<R, kW, false>(g_dpsi, gb, bl.off, D);
-                        if (tb == ntb - 1) {
-#pragma unroll
-                            for (int j = 0; j < (1 << kW); j++) {
-                                Lm.a[j] = row[gb ^ bl.off[j]];
-                                DL.a[j] = {R(0), R(0)};
-                            }
-                        } else {
-                            win_load<R, kW, false>(g_lam, gb, bl.off, Lm);
-                            if constexpr (MODE == 2) win_load<R, kW, false>(g_dlam, gb, bl.off, DL);
-                        }
-                    }
+                if constexpr (kRev) {
                     for (int k = k1 - 1; k >= k0; k--) {
                         const BlockRegs<kW> bk = load_block<kW>(a.blk_lds + k);
                         const uint32_t sb = lin<R, true>(deposit<kW>((uint32_t)tid, bk.pos));
@@ -691,8 +671,26 @@ __global__ __launch_bounds__(512) void sweep_win_tile_kernel(WinTileArgs a)
                         }
                     }
                 }
-                // the next (lower) tile block reads, in its own mapping, what other threads stored
-                __syncthreads();
+            }
+            // the next tile block reads, in its own mapping, what other threads stored here
+            __syncthreads();
+        }
+        if constexpr (MODE == 3) continue;
+        {
+            const double sr = wave_sum(fr), si = wave_sum(fi);
+            if (lane == 0) {
+                fst[2 * wv] = sr;
+                fst[2 * wv + 1] = si;
+            }
+            __syncthreads();
+            if (tid == 0) {
+                double tr = 0.0, ti = 0.0;
+                for (int w = 0; w < kNW; w++) {
+                    tr += fst[2 * w];
+                    ti += fst[2 * w + 1];
+                }
+                fre += tr;
+                fim += ti;
             }
         }
         __syncthreads();
