@@ -543,7 +543,9 @@ def main():
         extra["rtr"] = {"iters": args.rtr_iters, "iters_per_s": args.rtr_iters / rt, "hvp_sweeps": nh,
                         "f_first": float(f_iter[0]), "f_last": float(f_iter[-1]),
                         "precision": "fp64 sweeps, reference control flow"}
-        job = {"program": p, "gates": V, "niter": args.rtr_iters, "U": W, "arrival": 0.0}
+        # the job names the resident target (as the FP64 run above uses it): no host-side target
+        # setup inside the timed call
+        job = {"program": p, "gates": V, "niter": args.rtr_iters, "target": t, "arrival": 0.0}
         r, st = mg.serve([job], batched=False, precision=mg.MG_FP32, precision_switch=3, realtime=False,
                          tr_reuse=True)
         extra["rtr_fp32_sweeps"] = {

@@ -176,7 +176,9 @@ int mg_batch_autotune(mg_request* reqs, int n, int precision, int reps, int* thr
  * on its circuit and target (with tr_reuse, the variant described at mg_serve_options).  batched = 1: jobs are admitted as they arrive (and as slots and
  * device memory allow) and all admitted jobs' evaluations are merged into shared launches;
  * batched = 0: jobs run one at a time, one launch per evaluation.  The target is created on
- * the device at admission and freed when the job ends. */
+ * the device at admission and freed when the job ends, unless the job names a resident target
+ * (`target`), which the caller keeps: several jobs may share one, and it costs no admission time
+ * or job footprint. */
 typedef struct mg_job {
     int nqubits;
     const mg_program* program;           /* circuit to optimise */
@@ -192,6 +194,8 @@ typedef struct mg_job {
     double t_admit, t_done;              /* out: seconds after the start */
     int nhvp;                            /* out: HVP evaluations */
     int status;                          /* out: 0 ok, -1 failed, -2 stopped or not run (time_limit_s) */
+    const mg_target* target;             /* a resident target of nqubits (mg_target_*), used instead of U or
+                                            the circuit and left to the caller; NULL: U or the circuit */
 } mg_job;
 
 typedef struct mg_serve_options {

@@ -65,7 +65,8 @@ struct Posted {
 struct JobRt {
     Server* srv = nullptr;
     mg_job* spec = nullptr;
-    std::unique_ptr<mg::Target> target;
+    mg::Target* target = nullptr;             // the job's target: its own, or the caller's resident one
+    std::unique_ptr<mg::Target> own_target;
     size_t footprint = 0;
     int precision = MG_FP64, switch_last = 0, niter = 0, iter = -1, nhvp = 0, failed = 0;
     bool reuse = false;
@@ -131,7 +132,7 @@ int job_eval(JobRt* j, const zd* gates, const zd* Z, int mode, mg::EvalOut* out)
         return rc;
     }
     Posted p;
-    p.req.t = j->target.get();
+    p.req.t = j->target;
     p.req.p = &prog;
     p.req.gates = gates;
     p.req.Z = Z;
@@ -294,8 +295,13 @@ size_t job_footprint(const mg_job& s, int densify_qubits)
     size_t b = 0;
     // dense rows: FP64, plus an FP32 copy up to 12 qubits (the tiled FP32 kernel of larger
     // targets rounds the FP64 rows on load)
-    if (s.U || L <= densify_qubits) b += S * S * (sizeof(double) * 2 + (L <= 12 ? sizeof(float) * 2 : 0));
-    else b += (size_t)s.target_ngates * (mg::kCoefPerInst * 24 + sizeof(mg::Inst));
+    if (s.target) {
+        // the caller's resident target: not the job's to hold
+    } else if (s.U || L <= densify_qubits) {
+        b += S * S * (sizeof(double) * 2 + (L <= 12 ? sizeof(float) * 2 : 0));
+    } else {
+        b += (size_t)s.target_ngates * (mg::kCoefPerInst * 24 + sizeof(mg::Inst));
+    }
     b += (size_t)p.G * (mg::kCoefPerInst * 16 + sizeof(mg::Inst));
     int cgw, cpi;
     mg::batch_shape(L, mg::kFP32, &cgw, &cpi);
@@ -322,6 +328,7 @@ bool valid_job(const mg_job& s)
 {
     if (!s.program || !s.gates_start || !s.f_iter || !s.gates_opt || s.niter < 0) return false;
     if (s.nqubits != s.program->p.L || s.nqubits > 16) return false;
+    if (s.target) return s.target->t && s.target->t->L == s.nqubits;
     if (!s.U && (s.target_ngates < 1 || !s.target_gates || !s.target_wires)) return false;
     return true;
 }
@@ -545,7 +552,12 @@ int mg_serve(mg_job* jobs, int njobs, const mg_serve_options* opt, mg_serve_stat
         j->reuse = opt->tr_reuse != 0;
         j->footprint = job_footprint(jobs[i], opt->densify_qubits);
         jobs[i].t_admit = now_s();
-        j->target.reset(make_target(jobs[i], opt->densify_qubits));
+        if (jobs[i].target) {
+            j->target = jobs[i].target->t.get();
+        } else {
+            j->own_target.reset(make_target(jobs[i], opt->densify_qubits));
+            j->target = j->own_target.get();
+        }
         if (!j->target) {
             jobs[i].status = -1;
             return false;
@@ -570,7 +582,7 @@ int mg_serve(mg_job* jobs, int njobs, const mg_serve_options* opt, mg_serve_stat
             peak_active = 1;
             peak_held = std::max(peak_held, rt[i]->footprint);
             run_job(rt[i].get());
-            rt[i]->target.reset();
+            rt[i]->own_target.reset();
             jobs[i].t_done = now_s();
             if (jobs[i].status == -1) rc = -1;
             done++;
@@ -655,7 +667,7 @@ int mg_serve(mg_job* jobs, int njobs, const mg_serve_options* opt, mg_serve_stat
                 JobRt* j = rt[order[k]].get();
                 if (j && j->finished && j->th.joinable()) {
                     j->th.join();
-                    j->target.reset();
+                    j->own_target.reset();
                     held -= j->footprint;
                     j->spec->t_done = now_s();
                     if (j->spec->status == -1) rc = -1;

@@ -66,7 +66,8 @@ class Job(ctypes.Structure):
     _fields_ = [("nqubits", ctypes.c_int), ("program", _P), ("gates_start", _P), ("U", _P),
                 ("target_ngates", ctypes.c_int), ("target_gates", _P), ("target_wires", _P),
                 ("niter", ctypes.c_int), ("arrival", _D), ("f_iter", _P), ("gates_opt", _P),
-                ("t_admit", _D), ("t_done", _D), ("nhvp", ctypes.c_int), ("status", ctypes.c_int)]
+                ("t_admit", _D), ("t_done", _D), ("nhvp", ctypes.c_int), ("status", ctypes.c_int),
+                ("target", _P)]
 
 
 class ServeOptions(ctypes.Structure):
@@ -476,7 +477,8 @@ def serve(jobs, batched=True, precision=MG_FP32, precision_switch=3, max_active=
     """Run a stream of trust-region optimisation jobs (mg_serve).
 
     jobs: list of dicts with keys program (Program), gates (P, 8), niter, arrival (s),
-    and either U (dense target) or target_gates + target_wires (circuit target).
+    and either target (a resident Target, shared and kept by the caller), U (dense target)
+    or target_gates + target_wires (circuit target).
     tr_reuse: H eta of each ratio test from the truncated CG's own products and f + gradient
     kept across rejected steps (mg_serve_options.tr_reuse); False keeps the control flow of
     riemannian_trust_region_optimize_hvp.
@@ -493,14 +495,16 @@ def serve(jobs, batched=True, precision=MG_FP32, precision_switch=3, max_active=
         niter = int(j["niter"])
         f_iter = np.zeros(niter + 1)
         gout = np.zeros_like(g0)
-        U = _c(j["U"]) if j.get("U") is not None else None
-        tg = _c(j["target_gates"]) if U is None else None
-        tw = _i(j["target_wires"]) if U is None else None
-        keep += [prog, g0, U, tg, tw]
+        res_t = j.get("target")
+        U = _c(j["U"]) if res_t is None and j.get("U") is not None else None
+        tg = _c(j["target_gates"]) if res_t is None and U is None else None
+        tw = _i(j["target_wires"]) if res_t is None and U is None else None
+        keep += [prog, g0, U, tg, tw, res_t]
         arr[k] = Job(prog.L, prog.h, g0.ctypes.data, U.ctypes.data if U is not None else None,
                      tg.shape[0] if tg is not None else 0, tg.ctypes.data if tg is not None else None,
                      tw.ctypes.data if tw is not None else None, niter, float(j.get("arrival", 0.0)),
-                     f_iter.ctypes.data, gout.ctypes.data, 0.0, 0.0, 0, 0)
+                     f_iter.ctypes.data, gout.ctypes.data, 0.0, 0.0, 0, 0,
+                     res_t.h if res_t is not None else None)
         outs.append((f_iter, gout))
     opt = ServeOptions(int(bool(batched)), int(precision), int(precision_switch), int(max_active), float(hbm_budget),
                        int(bool(realtime)), int(densify_qubits), float(progress_s), int(bool(tr_reuse)),

@@ -230,6 +230,40 @@ def test_serve_batched_equals_alone_and_single_path(mg):
         assert np.max(np.abs(a["f_iter"] - b["f_iter"])) < 1e-9 and np.max(np.abs(a["gates"] - b["gates"])) < 1e-9
 
 
+def test_serve_jobs_sharing_a_resident_target(mg):
+    """Jobs that name a resident target (mg_job.target) run exactly as jobs that hand over the
+    same dense matrix: the same bits batched and one job at a time, on the batched kernel (L=8)
+    and on the single-request kernels (L=12); the target is the caller's, so it adds nothing to
+    the jobs' footprint and stays usable after the call."""
+    from rqcopt_mg.workloads import brickwall_job
+
+    rng = np.random.default_rng(29)
+    for L, specs in ((8, [(3, 5), (5, 4)]), (12, [(3, 3), (5, 2)])):
+        jobs = []
+        W = None
+        for d, it in specs:
+            j = brickwall_job(rng, L, d, it)
+            if W is None:
+                W = program_matrix(j["target_gates"], np.arange(j["Gt"]), j["target_wires"], L)
+            jobs.append({"program": mg.Program.create(L, j["G"], j["param"], j["wires"]), "gates": j["gates"],
+                         "niter": it, "arrival": 0.0})
+        t = mg.Target.dense(W)
+        with_u = [dict(j, U=W) for j in jobs]
+        with_t = [dict(j, target=t) for j in jobs]
+        for batched in (True, False):
+            say(f"case: L={L} serve 2 jobs batched={batched}, dense U vs a shared resident target")
+            ru, su = mg.serve(with_u, batched=batched, precision=mg.MG_FP32, precision_switch=2, realtime=False,
+                              tr_reuse=True)
+            rt, st = mg.serve(with_t, batched=batched, precision=mg.MG_FP32, precision_switch=2, realtime=False,
+                              tr_reuse=True)
+            for a, b in zip(rt, ru):
+                assert a["status"] == 0 and b["status"] == 0
+                assert np.array_equal(a["f_iter"], b["f_iter"]) and np.array_equal(a["gates"], b["gates"])
+            assert st["peak_footprint"] < su["peak_footprint"] - 0.5 * W.size * 16
+        f, _, _ = mg.riemannian_gradient_hvp(t, jobs[0]["program"], jobs[0]["gates"], None, mg.MG_FP64)
+        assert np.isfinite(f)
+
+
 def test_serve_tr_reuse_matches_reference_control_flow(mg):
     """tr_reuse: H eta of the ratio test summed from the truncated CG's products, f + gradient
     kept across rejected steps.  Same trajectories as the reference control flow (1e-9, FP64),
