@@ -322,20 +322,42 @@ Target* target_dense_seed(std::vector<cpx<double>>&& seed, int L)
     return t.release();
 }
 
+// in place: m holds U (row-major S x S) and receives the seed rows m[b][a] = -conj(U[a][b]); each
+// pair of mirrored entries is swapped by the thread of its upper-triangle entry
+__global__ void seed_rows_in_place(cpx<double>* m, size_t S)
+{
+    const size_t n = S * S;
+    for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x) {
+        const size_t a = k / S, b = k % S;
+        if (b < a) continue;
+        const cpx<double> u = m[a * S + b], v = m[b * S + a];
+        m[b * S + a] = {-u.re, u.im};
+        if (b != a) m[a * S + b] = {-v.re, v.im};
+    }
+}
+
 Target* target_dense(const zd* U, int L)
 {
+    Context* c = ctx();
+    if (!c) { set_error("no HIP device"); return nullptr; }
     const size_t S = (size_t)1 << L;
-    std::vector<cpx<double>> seed(S * S);
-    // seed[b][a] = -conj(U[a][b]): blocked transpose
-    const size_t B = 64;
-    for (size_t a0 = 0; a0 < S; a0 += B)
-        for (size_t b0 = 0; b0 < S; b0 += B)
-            for (size_t a = a0; a < std::min(S, a0 + B); a++)
-                for (size_t b = b0; b < std::min(S, b0 + B); b++) {
-                    const zd u = U[a * S + b];
-                    seed[b * S + a] = {-u.real(), u.imag()};
-                }
-    return target_dense_seed(std::move(seed), L);
+    auto t = std::make_unique<Target>();
+    t->kind = 0;
+    t->L = L;
+    // the matrix as given, then its seed rows formed on the device (a host transpose of a 2^12
+    // target took longer than the upload)
+    if (t->seed64.ensure(S * S * sizeof(cpx<double>)) != hipSuccess) { set_error("out of device memory for the target"); return nullptr; }
+    if (hipMemcpyAsync(t->seed64.p, U, S * S * sizeof(cpx<double>), hipMemcpyHostToDevice, c->s) != hipSuccess) {
+        set_error("copying the target to the device failed");
+        return nullptr;
+    }
+    const long grid = std::min<long>((long)c->ncu * 8, std::max<long>(1, (long)((S * S + 255) / 256)));
+    hipLaunchKernelGGL(seed_rows_in_place, dim3((unsigned)grid), dim3(256), 0, c->s, (cpx<double>*)t->seed64.p, S);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(c->s) != hipSuccess) {
+        set_error("forming the target's seed rows failed");
+        return nullptr;
+    }
+    return t.release();
 }
 
 Target* target_program(const Program& prog, const zd* gates)
