@@ -9,6 +9,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 
 #include "mg_batch.hpp"
 #include "mg_mat_mfma.hpp"
@@ -36,9 +37,28 @@ const char* last_error() { return g_err.c_str(); }
         }                                                                                    \
     } while (0)
 
+// ---- graph capture and the calls that may not overlap it ---------------------------------
+// Several host threads use the device at once (the server's lanes, each with its own stream).
+// While one of them captures a HIP graph, another thread's allocation, free or synchronous copy
+// (which orders itself against every stream of the device) fails the capture or itself; such
+// calls and the capture sections hold this lock.
+std::recursive_mutex& capture_mutex()
+{
+    static std::recursive_mutex m;
+    return m;
+}
+
+// synchronous host-to-device copy, outside any other thread's graph capture
+static hipError_t h2d(void* dst, const void* src, size_t bytes)
+{
+    std::lock_guard<std::recursive_mutex> g(capture_mutex());
+    return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
+}
+
 hipError_t DevBuf::ensure(size_t bytes)
 {
     if (bytes <= n && p) return hipSuccess;
+    std::lock_guard<std::recursive_mutex> g(capture_mutex());
     release();
     hipError_t e = hipMalloc(&p, bytes ? bytes : 256);
     if (e != hipSuccess) {
@@ -52,6 +72,7 @@ hipError_t DevBuf::ensure(size_t bytes)
 
 void DevBuf::release()
 {
+    std::lock_guard<std::recursive_mutex> g(capture_mutex());
     if (p) (void)hipFree(p);
     p = nullptr;
     n = 0;
@@ -145,6 +166,7 @@ struct Context {
     ~Context()
     {
         for (Graph& g : graphs) (void)hipGraphExecDestroy(g.exec);
+        std::lock_guard<std::recursive_mutex> lk(capture_mutex());
         for (void* h : {b_up_h, b_out_h, e_up_h, e_out_h})
             if (h) (void)hipHostFree(h);
     }
@@ -315,7 +337,7 @@ Target* target_dense_seed(std::vector<cpx<double>>&& seed, int L)
     t->kind = 0;
     t->L = L;
     if (t->seed64.ensure(S * S * sizeof(cpx<double>)) != hipSuccess) { set_error("out of device memory for the target"); return nullptr; }
-    if (hipMemcpy(t->seed64.p, seed.data(), S * S * sizeof(cpx<double>), hipMemcpyHostToDevice) != hipSuccess) {
+    if (h2d(t->seed64.p, seed.data(), S * S * sizeof(cpx<double>)) != hipSuccess) {
         set_error("copying the target to the device failed");
         return nullptr;
     }
@@ -379,8 +401,8 @@ Target* target_program(const Program& prog, const zd* gates)
         set_error("out of device memory for the target program");
         return nullptr;
     }
-    if (hipMemcpy(t->tinst.p, b.inst.data(), b.inst.size() * sizeof(Inst), hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(t->tcoef64.p, b.coef.data(), b.coef.size() * sizeof(cpx<double>), hipMemcpyHostToDevice) != hipSuccess) {
+    if (h2d(t->tinst.p, b.inst.data(), b.inst.size() * sizeof(Inst)) != hipSuccess ||
+        h2d(t->tcoef64.p, b.coef.data(), b.coef.size() * sizeof(cpx<double>)) != hipSuccess) {
         set_error("copying the target program failed");
         return nullptr;
     }
@@ -429,6 +451,7 @@ struct Arena {
 static int pinned_ensure(void** p, size_t* n, size_t bytes)
 {
     if (*p && *n >= bytes) return 0;
+    std::lock_guard<std::recursive_mutex> g(capture_mutex());
     if (*p) (void)hipHostFree(*p);
     *p = nullptr;
     *n = 0;
@@ -577,11 +600,11 @@ static int prepare_tile_form(Context* c, Target& t)
     const size_t nb = wt.blk_lds.size() * sizeof(WBlock);
     MG_HIP(f.blk.ensure(2 * nb + wt.tblocks.size() * sizeof(WTBlock)));
     unsigned char* blk = (unsigned char*)f.blk.p;
-    MG_HIP(hipMemcpy(f.inst.p, wt.inst.data(), wt.inst.size() * sizeof(Inst), hipMemcpyHostToDevice));
-    MG_HIP(hipMemcpy(f.coef.p, wt.coef.data(), wt.coef.size() * sizeof(cpx<double>), hipMemcpyHostToDevice));
-    MG_HIP(hipMemcpy(blk, wt.blk_lds.data(), nb, hipMemcpyHostToDevice));
-    MG_HIP(hipMemcpy(blk + nb, wt.blk_glob.data(), nb, hipMemcpyHostToDevice));
-    MG_HIP(hipMemcpy(blk + 2 * nb, wt.tblocks.data(), wt.tblocks.size() * sizeof(WTBlock), hipMemcpyHostToDevice));
+    MG_HIP(h2d(f.inst.p, wt.inst.data(), wt.inst.size() * sizeof(Inst)));
+    MG_HIP(h2d(f.coef.p, wt.coef.data(), wt.coef.size() * sizeof(cpx<double>)));
+    MG_HIP(h2d(blk, wt.blk_lds.data(), nb));
+    MG_HIP(h2d(blk + nb, wt.blk_glob.data(), nb));
+    MG_HIP(h2d(blk + 2 * nb, wt.tblocks.data(), wt.tblocks.size() * sizeof(WTBlock)));
     f.G = (int)wt.inst.size();
     f.ntb = (int)wt.tblocks.size();
     f.nblk = (int)wt.blk_lds.size();
@@ -641,10 +664,10 @@ static int prepare_mat_form(Target& t)
     MG_HIP(f.win.ensure(d.wins.size() * sizeof(DWin)));
     MG_HIP(f.af.ensure(d.afrag.size() * sizeof(double)));
     MG_HIP(f.af32.ensure(d.afrag32.size() * sizeof(float)));
-    MG_HIP(hipMemcpy(f.tb.p, d.tblocks.data(), d.tblocks.size() * sizeof(WTBlock), hipMemcpyHostToDevice));
-    MG_HIP(hipMemcpy(f.win.p, d.wins.data(), d.wins.size() * sizeof(DWin), hipMemcpyHostToDevice));
-    MG_HIP(hipMemcpy(f.af.p, d.afrag.data(), d.afrag.size() * sizeof(double), hipMemcpyHostToDevice));
-    MG_HIP(hipMemcpy(f.af32.p, d.afrag32.data(), d.afrag32.size() * sizeof(float), hipMemcpyHostToDevice));
+    MG_HIP(h2d(f.tb.p, d.tblocks.data(), d.tblocks.size() * sizeof(WTBlock)));
+    MG_HIP(h2d(f.win.p, d.wins.data(), d.wins.size() * sizeof(DWin)));
+    MG_HIP(h2d(f.af.p, d.afrag.data(), d.afrag.size() * sizeof(double)));
+    MG_HIP(h2d(f.af32.p, d.afrag32.data(), d.afrag32.size() * sizeof(float)));
     f.ntb = (int)d.tblocks.size();
     f.ready = true;
     return 0;
@@ -1390,10 +1413,10 @@ static int ensure_windowed_target(Target& t)
     MG_HIP(wc.coef.ensure(cR.size() * sizeof(cpx<R>)));
     MG_HIP(wc.blk_lds.ensure(w.blk_lds.size() * sizeof(WBlock)));
     MG_HIP(wc.blk_glob.ensure(w.blk_glob.size() * sizeof(WBlock)));
-    MG_HIP(hipMemcpy(wc.inst.p, w.inst.data(), w.inst.size() * sizeof(Inst), hipMemcpyHostToDevice));
-    MG_HIP(hipMemcpy(wc.coef.p, cR.data(), cR.size() * sizeof(cpx<R>), hipMemcpyHostToDevice));
-    MG_HIP(hipMemcpy(wc.blk_lds.p, w.blk_lds.data(), w.blk_lds.size() * sizeof(WBlock), hipMemcpyHostToDevice));
-    MG_HIP(hipMemcpy(wc.blk_glob.p, w.blk_glob.data(), w.blk_glob.size() * sizeof(WBlock), hipMemcpyHostToDevice));
+    MG_HIP(h2d(wc.inst.p, w.inst.data(), w.inst.size() * sizeof(Inst)));
+    MG_HIP(h2d(wc.coef.p, cR.data(), cR.size() * sizeof(cpx<R>)));
+    MG_HIP(h2d(wc.blk_lds.p, w.blk_lds.data(), w.blk_lds.size() * sizeof(WBlock)));
+    MG_HIP(h2d(wc.blk_glob.p, w.blk_glob.data(), w.blk_glob.size() * sizeof(WBlock)));
     wc.nblocks = (int)w.blk_lds.size();
     wc.wbits = W;
     return 0;
@@ -1413,14 +1436,14 @@ int target_densify(Target& t)
     MG_HIP(di.ensure(w.inst.size() * sizeof(Inst)));
     MG_HIP(dc.ensure(w.coef.size() * sizeof(cpx<double>)));
     MG_HIP(db.ensure(w.blk_lds.size() * sizeof(WBlock)));
-    MG_HIP(hipMemcpy(di.p, w.inst.data(), w.inst.size() * sizeof(Inst), hipMemcpyHostToDevice));
-    MG_HIP(hipMemcpy(dc.p, w.coef.data(), w.coef.size() * sizeof(cpx<double>), hipMemcpyHostToDevice));
+    MG_HIP(h2d(di.p, w.inst.data(), w.inst.size() * sizeof(Inst)));
+    MG_HIP(h2d(dc.p, w.coef.data(), w.coef.size() * sizeof(cpx<double>)));
     const long cgw = std::min<long>(256, std::max<long>(1, (long)(S >> 3)));
     const size_t lds = (size_t)(256 / cgw) * S * sizeof(cpx<double>);
     const bool in_lds = lds <= c->lds_limit - 256;
     // LDS-resident columns use the swizzled block offsets, in-place rows the plain ones
     const std::vector<WBlock>& bl = in_lds ? w.blk_lds : w.blk_glob;
-    MG_HIP(hipMemcpy(db.p, bl.data(), bl.size() * sizeof(WBlock), hipMemcpyHostToDevice));
+    MG_HIP(h2d(db.p, bl.data(), bl.size() * sizeof(WBlock)));
     if (t.seed64.ensure(S * S * sizeof(cpx<double>)) != hipSuccess) { set_error("out of device memory for the target"); return -1; }
     // beyond the LDS: 12-qubit tiles of each row in registers (the tiled window kernel's
     // forward sweep), unless the knob asks for the windowed in-place rows
@@ -1631,10 +1654,12 @@ static int run_batch(Context* c, const std::vector<int>& idx, BatchReq* reqs, Ba
         if (!graph && c->graph_seen(key)) {
             // the dynamic-LDS attribute is a function attribute, set outside the capture
             MG_HIP(launch_batch<R>(nullptr, nullptr, 0, nullptr, nullptr, nullptr, 0, T, W, -1, lds, s));
+            std::unique_lock<std::recursive_mutex> cap(capture_mutex());
             MG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
             const hipError_t e = enqueue(false);
             hipGraph_t g = nullptr;
             const hipError_t ee = hipStreamEndCapture(s, &g);
+            cap.unlock();
             if (e != hipSuccess || ee != hipSuccess) {
                 if (g) (void)hipGraphDestroy(g);
                 set_error(std::string("batched evaluation: graph capture failed: ") +

@@ -3,6 +3,7 @@
 #pragma once
 #include <complex>
 #include <cstdint>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -17,6 +18,10 @@ void set_error(const std::string& msg);
 const char* last_error();
 
 // grow-only device allocation owned by the context
+// held by HIP graph captures and by the calls that may not overlap one on another thread
+// (allocations, frees, synchronous copies)
+std::recursive_mutex& capture_mutex();
+
 struct DevBuf {
     void* p = nullptr;
     size_t n = 0;

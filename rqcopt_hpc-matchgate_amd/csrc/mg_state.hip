@@ -158,7 +158,11 @@ __global__ void k_gate_placeholder(int lo, int hi, int flip, const C* __restrict
 
 struct Dev {
     void* p = nullptr;
-    ~Dev() { if (p) (void)hipFree(p); }
+    ~Dev()
+    {
+        std::lock_guard<std::recursive_mutex> g(capture_mutex());
+        if (p) (void)hipFree(p);
+    }
 };
 
 #define ST_HIP(call)                                                                   \
@@ -170,7 +174,11 @@ struct Dev {
         }                                                                              \
     } while (0)
 
-int dalloc(Dev& d, size_t bytes) { return hipMalloc(&d.p, std::max<size_t>(bytes, 256)) == hipSuccess ? 0 : -1; }
+int dalloc(Dev& d, size_t bytes)
+{
+    std::lock_guard<std::recursive_mutex> g(capture_mutex());
+    return hipMalloc(&d.p, std::max<size_t>(bytes, 256)) == hipSuccess ? 0 : -1;
+}
 
 int grid_for(size_t n) { return (int)std::min<size_t>((n + 255) / 256, 4096); }
 
