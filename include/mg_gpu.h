@@ -274,6 +274,13 @@ void mg_set_materialize_kernel(int mode);
  * forward pass over 4-qubit windows, each window's gates as two 8 x 8 parity blocks, FP32 MFMA;
  * for the HVP also dpsi <- M dpsi + N psi), 0 gate by gate on the VALU */
 void mg_set_forward_mfma(int mode);
+/* mg_optimize with HVPs (and the drivers over it): mode -1 automatic (default: against a dense
+ * target, the trust region with its truncated CG on the device, each iteration one HIP graph
+ * launch, where the single-request sweeps have plans: LDS-resident states or the L=12 register
+ * window kernel), 0 the host loop.  Both follow the reference control flow. */
+void mg_set_device_trust_region(int mode);
+/* graph launches of the last mg_optimize on the device (0: it ran on the host loop) */
+long mg_last_trust_region_graphs(void);
 /* record HIP events around every sweep kernel on the library's stream */
 void mg_set_kernel_timing(int on);
 /* duration of the last sweep kernel in ms (needs mg_set_kernel_timing(1)) */

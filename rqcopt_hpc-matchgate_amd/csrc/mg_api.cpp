@@ -19,6 +19,7 @@
 #include "../../include/mg_brickwall_opt.h"
 #include "../../include/mg_gpu.h"
 #include "../../include/trust_region.h"
+#include "mg_device_tr.hpp"
 #include "mg_host.hpp"
 #include "mg_runtime.hpp"
 
@@ -655,6 +656,9 @@ void mg_set_tile_kernel(int mode) { mg::knobs().force_tile = mode != 0 ? -1 : 0;
 void mg_set_row_chunk_bytes(long bytes) { mg::knobs().row_chunk_bytes = bytes > 0 ? (size_t)bytes : 0; }
 void mg_set_materialize_kernel(int mode) { mg::knobs().mat_kernel = mode; }
 void mg_set_forward_mfma(int mode) { mg::knobs().fwd_mfma = mode; }
+void mg_set_device_trust_region(int mode) { mg::knobs().device_tr = mode; }
+static thread_local long g_tr_graphs = 0;
+long mg_last_trust_region_graphs(void) { return g_tr_graphs; }
 float mg_last_kernel_ms(void) { return g_last_stats.kernel_ms; }
 void mg_last_sweep_info(int* lds_states, int* ckpt, int* grid, long* lds_bytes)
 {
@@ -768,6 +772,19 @@ int mg_optimize(const mg_target* t, const mg_program* p, const struct matchgate 
     opt_ctx c{t->t.get(), &p->p, precision, std::max(0, precision_switch), niter, -1, 0, 0, hvp ? f_citer : nullptr};
     int P = p->p.P;
     const int n = 8 * P, m = 16 * P;
+    g_tr_graphs = 0;
+    if (hvp && mg::knobs().device_tr != 0) {
+        // the trust region on the device, one graph per iteration, where its sweeps have plans
+        mg::DevTRStats ds;
+        const int r = mg::optimize_device(*t->t, p->p, reinterpret_cast<const zd*>(gates_start), params, niter, precision,
+                                          precision_switch, f_iter, reinterpret_cast<zd*>(f_citer),
+                                          reinterpret_cast<zd*>(gates_opt), nhvp, &ds);
+        if (r < 0) return -1;
+        if (r > 0) {
+            g_tr_graphs = ds.graph_launches;
+            return 0;
+        }
+    }
     if (hvp) {
         riemannian_trust_region_optimize_hvp(opt_f, opt_f_deriv, opt_hvp, &c, opt_retract, &P, n,
                                              reinterpret_cast<const double*>(gates_start), m, params, niter, f_iter,

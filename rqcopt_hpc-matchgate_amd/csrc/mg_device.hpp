@@ -44,6 +44,10 @@ __host__ __device__ inline cpx<R> csub(cpx<R> a, cpx<R> b)
     return {a.re - b.re, a.im - b.im};
 }
 
+// Raises a kernel's dynamic-LDS limit to `bytes` the first time it is needed and never lowers it,
+// so that a launch recorded in a graph stays valid and a launch inside a capture sets nothing.
+hipError_t ensure_lds_limit(const void* kernel, size_t bytes);
+
 // One gate instance of a program, in the sorted basis.
 struct Inst {
     int32_t hi;     // bit position of the first (smaller) wire: L-1-i

@@ -155,9 +155,10 @@ hipError_t launch_materialize_mfma(const MatArgs& a, int fp32, int grid, hipStre
 // (r % 4) / 2, part (r % 4) % 2).
 __global__ __launch_bounds__(64) void window_frags_kernel(const int* wstart, const WinGate* gates,
                                                           const cpx<double>* coef, int hvp, double* out64,
-                                                          float* out32)
+                                                          float* out32, const int* skip)
 {
     __shared__ double col[16][4][8];  // per thread: M re, M im, N re, N im of its column
+    if (skip && *skip) return;
     const int w = blockIdx.x, t = threadIdx.x;
     if (t >= 16) return;  // no barrier below: every thread works on its own column
     const int p = t >> 3, n = t & 7;
@@ -220,10 +221,10 @@ __global__ __launch_bounds__(64) void window_frags_kernel(const int* wstart, con
 }
 
 hipError_t launch_window_frags(int nwin, const int* wstart, const WinGate* gates, const cpx<double>* coef, int hvp,
-                               double* out64, float* out32, hipStream_t s)
+                               double* out64, float* out32, hipStream_t s, const int* skip)
 {
     if (nwin < 1 || (!out64) == (!out32)) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(window_frags_kernel, dim3(nwin), dim3(64), 0, s, wstart, gates, coef, hvp, out64, out32);
+    hipLaunchKernelGGL(window_frags_kernel, dim3(nwin), dim3(64), 0, s, wstart, gates, coef, hvp, out64, out32, skip);
     return hipGetLastError();
 }
 

@@ -72,9 +72,10 @@ int forward_window_form(const std::vector<Inst>& inst, bool fp64, std::vector<DW
 // The windows' A fragments on the device, from this evaluation's coefficients (`coef`: FP64,
 // [program instance][kCoefPerInst]): block w forms window w's M (and with `hvp` N) column by
 // column and writes [window][8 or 16][lane] fragments in the FP64 (out64) or FP32 (out32) MFMA
-// result layout, the same values forward_windows makes on the host
+// result layout, the same values forward_windows makes on the host.  Nonzero *skip: the launch
+// returns at once (a launch inside a captured trust-region step whose truncated CG has ended)
 hipError_t launch_window_frags(int nwin, const int* wstart, const WinGate* gates, const cpx<double>* coef, int hvp,
-                               double* out64, float* out32, hipStream_t s);
+                               double* out64, float* out32, hipStream_t s, const int* skip = nullptr);
 
 struct MatArgs {
     int L, ntb;

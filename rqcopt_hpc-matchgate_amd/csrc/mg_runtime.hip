@@ -13,6 +13,7 @@
 
 #include "mg_batch.hpp"
 #include "mg_mat_mfma.hpp"
+#include "mg_plan.hpp"
 #include "mg_sweep.hpp"
 #include "mg_sweep_tile.hpp"
 #include "mg_sweep_win.hpp"
@@ -46,6 +47,18 @@ std::recursive_mutex& capture_mutex()
 {
     static std::recursive_mutex m;
     return m;
+}
+
+hipError_t ensure_lds_limit(const void* kernel, size_t bytes)
+{
+    static std::mutex m;
+    static std::map<const void*, size_t> set;
+    std::lock_guard<std::mutex> g(m);
+    size_t& cur = set[kernel];
+    if (bytes <= cur) return hipSuccess;
+    const hipError_t e = hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e == hipSuccess) cur = bytes;
+    return e;
 }
 
 // synchronous host-to-device copy, outside any other thread's graph capture
@@ -476,7 +489,7 @@ struct Windowed {
 
 template <typename R>
 static void make_windowed(const std::vector<Inst>& inst, const std::vector<cpx<double>>& coef, int L, int W,
-                          Windowed* w, int kmax = kWinMaxGates)
+                          Windowed* w, int kmax = kWinMaxGates, std::vector<int>* order_out = nullptr)
 {
     std::vector<int> order;
     std::vector<WBlock> blocks;
@@ -502,6 +515,7 @@ static void make_windowed(const std::vector<Inst>& inst, const std::vector<cpx<d
             w->blk_glob[k].off[j] = o;
             w->blk_lds[k].off[j] = win_swz<batch_bank_bits<R>()>(o);
         }
+    if (order_out) order_out->swap(order);
 }
 
 // tile blocks (12-qubit windows) and, per tile block, its 3-qubit window blocks, for the tiled
@@ -771,9 +785,10 @@ static int upload_flush(Context* c, SweepMem& ar, const Arena& up, hipStream_t s
     return 0;
 }
 
+// plan non-null: lay the sweep out in plan->mem (as ar) and record it instead of launching
 template <typename R>
 static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, int mode, long b0, long b1, EvalOut* out,
-                     EvalStats* st, SweepMem& ar)
+                     EvalStats* st, SweepMem& ar, SweepPlan* plan = nullptr)
 {
     hipStream_t s = c->s;
     const int L = p.L;
@@ -837,11 +852,12 @@ static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, in
     int win_rs = 0, win_slots = kWinRegGates;
     bool use_win = false;
     Windowed w_win;
+    std::vector<int> win_order;  // the window form's instance order (program instance of each position)
     constexpr int fp32 = std::is_same<R, float>::value ? 1 : 0;
     if (!ckpt && b.defect <= 1e-12 && t.kind == 0 && win_sweep_fits(L, fp32) && kn.force_win != 0 && kn.force_reg != 0 &&
         kn.force_lds != 0 && (!fp32 || kn.force_win == 1)) {
         // the window form first: its largest block sizes the staging area
-        make_windowed<R>(b.inst, b.coef, L, win_sweep_window_bits(fp32), &w_win, kWinRegGates);
+        make_windowed<R>(b.inst, b.coef, L, win_sweep_window_bits(fp32), &w_win, kWinRegGates, &win_order);
         win_slots = max_block_gates(w_win.blk_lds);
         win_lds = win_sweep_lds_bytes(L, p.P, fp32, c->lds_limit, &win_acc, &win_buf, &win_rs, mode, win_slots);
         use_win = win_lds <= c->lds_limit;
@@ -1034,6 +1050,7 @@ static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, in
         frag.out32 = (float*)d_fa;
         a.fwaf = (const float*)d_fa;
         a.nwin = nwin;
+        if (plan) plan->korder = o2;
     }
 
     WinSweepArgs wa{};
@@ -1237,6 +1254,45 @@ static int run_sweep(Context* c, Target& t, const Program& p, const Built& b, in
             ta.bparam = (const int*)(dev + ut_bpar);
         }
     }
+    if (plan) {
+        // the sweeps with a plan: LDS-resident states (no checkpoints, registers or tiles), or the
+        // register gate-window kernel
+        const bool lds_kernel = lds_states && !ckpt && !reg_T && !use_win && !use_tile;
+        plan->kind = use_win ? 2 : lds_kernel ? 1 : 0;
+        if (plan->kind == 0) return 0;
+        plan->mode = mode;
+        plan->fp32 = fp32;
+        plan->grid = (int)grid;
+        plan->threads = use_win ? 512 : lds_T;
+        plan->lds = lds;
+        plan->stride = 2 + 32 * p.P;
+        plan->partial = (double*)d_partial;
+        plan->out = (double*)d_out;
+        plan->nwin = frag.nwin;
+        plan->wstart = frag.nwin ? (const int*)(dev + frag.u_start) : nullptr;
+        plan->wgate = frag.nwin ? (const WinGate*)(dev + frag.u_gate) : nullptr;
+        plan->frag64 = frag.out64;
+        plan->frag32 = frag.out32;
+        plan->prog64 = frag.nwin ? (cpx<double>*)(dev + frag.u_c64) : nullptr;
+        if (use_win) {
+            plan->wa = wa;
+            plan->kcoef = const_cast<void*>(wa.coef);
+            plan->korder = win_order;
+        } else {
+            if constexpr (std::is_same<R, double>::value) plan->a64 = a;
+            else plan->a32 = a;
+            plan->kcoef = const_cast<cpx<R>*>(a.coef);
+            if (!frag.out32) {
+                plan->korder.resize(p.G);
+                for (int g = 0; g < p.G; g++) plan->korder[g] = g;
+            }
+        }
+        if ((int)plan->korder.size() != p.G) {
+            set_error("sweep plan: instance order incomplete");
+            return -1;
+        }
+        return 0;
+    }
     if (frag.nwin)
         MG_HIP(launch_window_frags(frag.nwin, (const int*)(dev + frag.u_start), (const WinGate*)(dev + frag.u_gate),
                                    (const cpx<double>*)(dev + frag.u_c64), mode == 2, frag.out64, frag.out32, s));
@@ -1354,6 +1410,59 @@ int evaluate(const Target& tc, const Program& p, const zd* gates, const zd* Z, i
         return run_sweep<float>(c, t, p, b, mode, b0, b1, out, stats, ar);
     }
     return run_sweep<double>(c, t, p, b, mode, b0, b1, out, stats, ar);
+}
+
+int plan_sweep(const Target& tc, const Program& p, const zd* gates, int mode, int precision, SweepPlan* plan)
+{
+    Context* c = ctx();
+    if (!c) { set_error("no HIP device available"); return -1; }
+    Target& t = const_cast<Target&>(tc);
+    std::string why;
+    if (!p.validate(&why)) { set_error("program: " + why); return -1; }
+    if (t.L != p.L || t.kind != 0) { set_error("sweep plan: needs a dense target of the program's qubit count"); return -1; }
+    *plan = SweepPlan{};
+    // the FP32 copy of the rows now: the plan's layout takes no allocation of its own
+    if (precision == kFP32 && !t.seed32_ready && ensure_fp32_target(t, c->s) < 0) return -1;
+    // directions of zero in the layout; the plan's users rewrite all coefficients before a launch
+    std::vector<zd> zero((size_t)8 * p.P, zd(0.0));
+    Built b;
+    build_instances(p, gates, zero.data(), &b);
+    // everything in one allocation of the plan's own, laid out as the allocation-free evaluation
+    // does: the workgroup partial sums (at most 4 workgroups per CU) and 8 MiB for the descriptors
+    const size_t bytes = (size_t)c->ncu * 4 * (2 + 32 * (size_t)p.P) * sizeof(double) + ((size_t)8 << 20);
+    MG_HIP(plan->mem.ensure(bytes));
+    SweepMem ar;
+    ar.base = (unsigned char*)plan->mem.p;
+    ar.cap = bytes;
+    const long S = 1L << p.L;
+    if (precision == kFP32) return run_sweep<float>(c, t, p, b, mode, 0, S, nullptr, nullptr, ar, plan);
+    return run_sweep<double>(c, t, p, b, mode, 0, S, nullptr, nullptr, ar, plan);
+}
+
+hipError_t launch_plan(const SweepPlan& plan, const int* skip, hipStream_t s)
+{
+    hipError_t e = hipSuccess;
+    if (plan.nwin)
+        e = launch_window_frags(plan.nwin, plan.wstart, plan.wgate, plan.prog64, plan.mode == 2, plan.frag64, plan.frag32,
+                                s, skip);
+    if (e != hipSuccess) return e;
+    if (plan.kind == 2) {
+        WinSweepArgs wa = plan.wa;
+        wa.skip = skip;
+        e = launch_sweep_win(wa, plan.mode, plan.fp32, plan.grid, plan.lds, s);
+    } else if (plan.kind == 1 && plan.fp32) {
+        SweepArgs<float> a = plan.a32;
+        a.skip = skip;
+        e = launch_sweep<float>(a, plan.mode, true, false, plan.threads, plan.grid, plan.lds, s);
+    } else if (plan.kind == 1) {
+        SweepArgs<double> a = plan.a64;
+        a.skip = skip;
+        e = launch_sweep<double>(a, plan.mode, true, false, plan.threads, plan.grid, plan.lds, s);
+    } else {
+        return hipErrorInvalidValue;
+    }
+    if (e != hipSuccess) return e;
+    return launch_reduce(plan.partial, plan.grid, plan.stride, plan.out, s, skip);
 }
 
 }  // namespace mg

@@ -131,6 +131,8 @@ struct Knobs {
     size_t row_chunk_bytes = 0;  // materialised target rows per column chunk (0: 4 GiB)
     int mat_kernel = -1;         // target rows beyond 12 qubits: -1 auto, 0 register tiles, 1 matrix cores
     int fwd_mfma = -1;           // FP32 forward passes on the matrix cores (tiled f, f + gradient; L=12 HVP in LDS): -1 auto, 0 gate by gate
+    int device_tr = -1;          // mg_optimize with HVPs: -1 auto (trust region on the device, one graph per
+                                 // iteration, where its sweeps have plans), 0 the host loop
 };
 Knobs& knobs();
 

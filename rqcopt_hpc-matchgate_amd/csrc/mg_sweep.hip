@@ -31,6 +31,7 @@ namespace mg {
 template <typename R, int MODE, bool LDS, bool CKPT, int T>
 __global__ __launch_bounds__(T) void sweep_kernel(SweepArgs<R> a)
 {
+    if (a.skip && *a.skip) return;
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr int NW = T / 64;
     constexpr int NV = MODE == 2 ? 32 : 16;  // doubles reduced per gate (and kept per parameter)
@@ -237,8 +238,9 @@ __global__ __launch_bounds__(T) void sweep_kernel(SweepArgs<R> a)
 // out[k] = sum over workgroups w of partial[w][k], in a fixed order: thread row ty
 // sums w = ty, ty + 16, ... and row 0 then adds the 16 row sums in order.
 __global__ __launch_bounds__(1024) void reduce_partials(const double* __restrict__ partial, int nblk, int stride,
-                                                        double* __restrict__ out)
+                                                        double* __restrict__ out, const int* skip)
 {
+    if (skip && *skip) return;
     __shared__ double rows[16][64];
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
     const int k = blockIdx.x * 64 + tx;
@@ -260,7 +262,7 @@ static hipError_t launch_t(const SweepArgs<R>& a, int grid, size_t lds, hipStrea
 {
     auto k = sweep_kernel<R, MODE, LDS, CKPT, T>;
     if (lds > 65536) {
-        hipError_t e = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        hipError_t e = ensure_lds_limit((const void*)k, lds);
         if (e != hipSuccess) return e;
     }
     hipLaunchKernelGGL(k, dim3(grid), dim3(T), lds, s, a);
@@ -292,9 +294,9 @@ hipError_t launch_sweep(const SweepArgs<R>& a, int mode, bool lds_states, bool c
 template hipError_t launch_sweep<double>(const SweepArgs<double>&, int, bool, bool, int, int, size_t, hipStream_t);
 template hipError_t launch_sweep<float>(const SweepArgs<float>&, int, bool, bool, int, int, size_t, hipStream_t);
 
-hipError_t launch_reduce(const double* partial, int nblk, int stride, double* out, hipStream_t s)
+hipError_t launch_reduce(const double* partial, int nblk, int stride, double* out, hipStream_t s, const int* skip)
 {
-    hipLaunchKernelGGL(reduce_partials, dim3((stride + 63) / 64), dim3(1024), 0, s, partial, nblk, stride, out);
+    hipLaunchKernelGGL(reduce_partials, dim3((stride + 63) / 64), dim3(1024), 0, s, partial, nblk, stride, out, skip);
     return hipGetLastError();
 }
 

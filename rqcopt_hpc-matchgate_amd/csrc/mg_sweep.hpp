@@ -28,6 +28,9 @@ struct SweepArgs {
     const void* fwin;
     const float* fwaf;
     int nwin;
+    // a launch inside a captured trust-region step: nonzero *skip makes the launch return at once
+    // (the step's truncated CG has already ended); null: always run
+    const int* skip;
 };
 
 // doubles of LDS bookkeeping in front of the states: acc [P][32] (sweep_kernel: [P][16] below
@@ -47,7 +50,9 @@ hipError_t launch_sweep(const SweepArgs<R>& a, int mode, bool lds_states, bool c
 template <typename R>
 hipError_t launch_sweep_reg(const SweepArgs<R>& a, int mode, int threads, int qpt, int grid, size_t lds, hipStream_t s);
 
-hipError_t launch_reduce(const double* partial, int nblk, int stride, double* out, hipStream_t s);
+// skip as SweepArgs::skip
+hipError_t launch_reduce(const double* partial, int nblk, int stride, double* out, hipStream_t s,
+                         const int* skip = nullptr);
 
 
 }  // namespace mg

@@ -123,6 +123,7 @@ __device__ inline void win_rev_adds(const WinSweepArgs& a, int g0, int g1, const
 template <typename R, int MODE, int RS>
 __global__ __launch_bounds__(WinShape<R>::T) void sweep_win_kernel(WinSweepArgs a)
 {
+    if (a.skip && *a.skip) return;
     constexpr int kT = WinShape<R>::T, kW = WinShape<R>::W, kNW = kT / 64;
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr int NV = MODE == 2 ? 32 : 16;
@@ -775,7 +776,7 @@ static hipError_t launch_win_r(const WinSweepArgs& a, int mode, int grid, size_t
     const void* k = mode == 2 ? (const void*)sweep_win_kernel<R, 2, RS>
                   : mode == 1 ? (const void*)sweep_win_kernel<R, 1, RS> : (const void*)sweep_win_kernel<R, 0, RS>;
     if (lds > 65536) {
-        hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        hipError_t e = ensure_lds_limit(k, lds);
         if (e != hipSuccess) return e;
     }
     constexpr int T = WinShape<R>::T;

@@ -25,6 +25,7 @@ struct WinSweepArgs {
     const void* fwin;
     const double* fwaf;
     int nfwin;
+    const int* skip;  // nonzero *skip: the launch returns at once (as SweepArgs::skip); null: run
 };
 
 // Tiled variant (FP64, L > 12): the program is cut into tile blocks acting inside 12-qubit

@@ -130,6 +130,8 @@ _sig("mg_set_tile_kernel", [_I], None)
 _sig("mg_set_row_chunk_bytes", [_L], None)
 _sig("mg_set_materialize_kernel", [_I], None)
 _sig("mg_set_forward_mfma", [_I], None)
+_sig("mg_set_device_trust_region", [_I], None)
+_sig("mg_last_trust_region_graphs", [], ctypes.c_long)
 _sig("mg_last_kernel_ms", [], ctypes.c_float)
 _sig("mg_last_sweep_info", [_P, _P, _P, _P], None)
 _sig("mg_target_cache_clear", [], None)
@@ -545,6 +547,17 @@ def set_tile_kernel(mode):
 def set_forward_mfma(mode):
     """FP32 tiled sweeps: -1 forward pass on the matrix cores (default), 0 gate by gate"""
     lib.mg_set_forward_mfma(int(mode))
+
+
+def set_device_trust_region(mode):
+    """optimize with hvp=True: -1 trust region on the device, one graph per iteration (default,
+    where it applies), 0 the host loop"""
+    lib.mg_set_device_trust_region(int(mode))
+
+
+def last_trust_region_graphs():
+    """graph launches of the last optimize on the device (0: it ran on the host loop)"""
+    return int(lib.mg_last_trust_region_graphs())
 
 
 def set_materialize_kernel(mode):

@@ -667,6 +667,45 @@ def test_rtr_hvp_matches_oracle(mg):
     assert abs(f32[-1] - fo[-1]) < 1e-4 * abs(fo[-1])
 
 
+def test_device_trust_region_matches_host_loop(mg):
+    """mg_optimize with HVPs runs its trust region on the device, one HIP graph per iteration
+    (truncated CG, retraction and ratio test in device kernels between the sweeps): the same
+    iterates as the host loop over the same sweeps up to rounding, the same number of HVP sweeps,
+    one graph launch per iteration (and one per two further steps of a longer truncated CG).  L=8 tied brickwall (the reference driver's inputs, FP64, LDS
+    sweeps), L=12 open brickwall against a materialised circuit target (FP64 register-window
+    sweeps; FP32 LDS sweeps with the last two iterations in FP64)."""
+    def both(t, p, V, niter, prec=mg.MG_FP64, sw=0):
+        res = []
+        for mode in (0, -1):
+            mg.set_device_trust_region(mode)
+            res.append(mg.optimize(t, p, V, niter, hvp=True, precision=prec, precision_switch=sw) +
+                       (mg.last_trust_region_graphs(),))
+        mg.set_device_trust_region(-1)
+        return res
+
+    say("case: device trust region, Hubbard L=8 n5 (tied), FP64, 10 iterations")
+    U, V, pm = _hubbard_q8(5)
+    host, dev = both(mg.Target.dense(U), mg.Program.brickwall(8, pm), V, 10)
+    # one graph per iteration, and one more for each truncated CG longer than a graph's two steps
+    assert host[4] == 0 and 10 <= dev[4] <= 10 + dev[3] // 2
+    assert np.max(np.abs(dev[0] - host[0])) < 1e-12 and np.max(np.abs(dev[2] - host[2])) < 1e-12
+    assert np.max(np.abs(dev[1] - host[1])) < 1e-12 and dev[3] == host[3]
+    rng = np.random.default_rng(41)
+    L = 12
+    param, wires, G = open_brickwall(L, 7)
+    V = haar_matchgates(rng, G)
+    tparam, twires, Gt = open_brickwall(L, 13)
+    t = mg.Target.circuit(haar_matchgates(rng, Gt), twires, L)
+    t.densify()
+    p = mg.Program.create(L, G, param, wires)
+    for prec, sw, tol in ((mg.MG_FP64, 0, 1e-11), (mg.MG_FP32, 2, 1e-9)):
+        say(f"case: device trust region, L=12 open depth 7, precision {prec}, switch {sw}, 6 iterations")
+        host, dev = both(t, p, V, 6, prec, sw)
+        assert host[4] == 0 and 6 <= dev[4] <= 6 + dev[3] // 2
+        assert np.max(np.abs(dev[0] - host[0])) < tol * np.max(np.abs(host[0]))
+        assert np.max(np.abs(dev[2] - host[2])) < tol and dev[3] == host[3]
+
+
 def test_ti_hessian_matches_oracle(mg):
     say("case: translation-invariant Hessian shortcut")
     rng = np.random.default_rng(2)
