@@ -541,8 +541,10 @@ def main():
         f_iter, _, _, nh = mg.optimize(t, p, V, args.rtr_iters, hvp=True, precision=prec)
         rt = time.perf_counter() - t0
         extra["rtr"] = {"iters": args.rtr_iters, "iters_per_s": args.rtr_iters / rt, "hvp_sweeps": nh,
+                        "graph_launches": mg.last_trust_region_graphs(),
                         "f_first": float(f_iter[0]), "f_last": float(f_iter[-1]),
-                        "precision": "fp64 sweeps, reference control flow"}
+                        "precision": "fp64 sweeps, reference control flow; trust region and truncated CG on the "
+                                     "device, one HIP graph launch per iteration"}
         # the job names the resident target (as the FP64 run above uses it): no host-side target
         # setup inside the timed call
         job = {"program": p, "gates": V, "niter": args.rtr_iters, "target": t, "arrival": 0.0}
