@@ -555,8 +555,24 @@ def main():
             "evaluations": st["requests"], "f_first": float(r[0]["f_iter"][0]), "f_last": float(r[0]["f_iter"][-1]),
             "f_last_rel_diff_vs_fp64": float(abs(r[0]["f_iter"][-1] - f_iter[-1]) / abs(f_iter[-1])),
             "precision": "fp32 sweeps with fp64 accumulation, last 3 iterations fp64; tr_reuse"}
+        # the device trust region with reuse (H eta from the truncated CG, f + gradient at the trial
+        # point kept): FP64, and FP32 sweeps with the last 3 iterations in FP64
+        mg.set_device_trust_region(2)
+        for key, pr, sw in (("rtr_reuse", mg.MG_FP64, 0), ("rtr_fp32_reuse", mg.MG_FP32, 3)):
+            t0 = time.perf_counter()
+            fr, _, _, nhr = mg.optimize(t, p, V, args.rtr_iters, hvp=True, precision=pr, precision_switch=sw)
+            rt = time.perf_counter() - t0
+            extra[key] = {"iters": args.rtr_iters, "iters_per_s": args.rtr_iters / rt, "hvp_sweeps": nhr,
+                          "graph_launches": mg.last_trust_region_graphs(), "f_last": float(fr[-1]),
+                          "f_last_rel_diff_vs_fp64": float(abs(fr[-1] - f_iter[-1]) / abs(f_iter[-1])),
+                          "precision": ("fp64 sweeps" if pr == mg.MG_FP64 else
+                                        "fp32 sweeps with fp64 accumulation, last 3 iterations fp64") +
+                                       "; device trust region with reuse, one HIP graph launch per iteration"}
+        mg.set_device_trust_region(-1)
         log(f"trust region: {extra['rtr']['iters_per_s']:.1f} it/s FP64, "
-            f"{extra['rtr_fp32_sweeps']['iters_per_s']:.1f} it/s FP32 sweeps + tr_reuse")
+            f"{extra['rtr_reuse']['iters_per_s']:.1f} it/s FP64 device reuse, "
+            f"{extra['rtr_fp32_reuse']['iters_per_s']:.1f} it/s FP32 device reuse, "
+            f"{extra['rtr_fp32_sweeps']['iters_per_s']:.1f} it/s FP32 sweeps + tr_reuse (served)")
 
     if args.only_large:
         mg.set_kernel_timing(True)

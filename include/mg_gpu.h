@@ -277,7 +277,10 @@ void mg_set_forward_mfma(int mode);
 /* mg_optimize with HVPs (and the drivers over it): mode -1 automatic (default: against a dense
  * target, the trust region with its truncated CG on the device, each iteration one HIP graph
  * launch, where the single-request sweeps have plans: LDS-resident states or the L=12 register
- * window kernel), 0 the host loop.  Both follow the reference control flow. */
+ * window kernel), 0 the host loop.  Both follow the reference control flow.  Mode 2: on the device
+ * with reuse (as mg_serve_options.tr_reuse): H eta of each ratio test from the truncated CG's
+ * products, and the f + gradient sweep at an accepted trial point kept for the next iteration
+ * (one sweep per iteration fewer, up to two without a rejection; iterates equal up to rounding). */
 void mg_set_device_trust_region(int mode);
 /* graph launches of the last mg_optimize on the device (0: it ran on the host loop) */
 long mg_last_trust_region_graphs(void);

@@ -777,7 +777,8 @@ int mg_optimize(const mg_target* t, const mg_program* p, const struct matchgate 
         // the trust region on the device, one graph per iteration, where its sweeps have plans
         mg::DevTRStats ds;
         const int r = mg::optimize_device(*t->t, p->p, reinterpret_cast<const zd*>(gates_start), params, niter, precision,
-                                          precision_switch, f_iter, reinterpret_cast<zd*>(f_citer),
+                                          precision_switch, mg::knobs().device_tr == 2 ? 1 : 0, f_iter,
+                                          reinterpret_cast<zd*>(f_citer),
                                           reinterpret_cast<zd*>(gates_opt), nhvp, &ds);
         if (r < 0) return -1;
         if (r > 0) {

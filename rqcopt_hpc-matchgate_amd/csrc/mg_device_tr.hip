@@ -282,29 +282,36 @@ __global__ void k_coefs(const double* x, const double* v, const int2* pm, const 
         for (int k = 0; k < 24; k++) prog64[(size_t)g * kCoefPerInst + k] = c[k];
 }
 
-// f and the Riemannian gradient at x from an f + gradient sweep; the truncated CG's start
-// (r = grad, d = -grad, z = 0); f_citer[k] = f
+// f and the Riemannian gradient at x from an f + gradient sweep (with reuse, once kept: those of
+// the last trial point or of the rejected step's iterate); the truncated CG's start (r = grad,
+// d = -grad, z = 0, H z = 0); f_citer[k] = f
 __global__ __launch_bounds__(kBT) void k_grad_init(const double* out, const double* x, int P, double* grad, double* r,
-                                                   double* d, double* z, DevTRState* st, double* f_citer)
+                                                   double* d, double* z, double* Hz, DevTRState* st, double* f_citer)
 {
     __shared__ double sh[kBT];
     const int t = threadIdx.x, n = 8 * P;
-    for (int q = t; q < P; q += kBT) {
-        const MG V = load_mg(x + 16 * (size_t)q);
-        tangent_to_real(V, conj_mg(load_out(out, q, 0)), grad + 8 * (size_t)q);
-    }
+    const bool have = st->reuse && st->kept;
+    if (!have)
+        for (int q = t; q < P; q += kBT) {
+            const MG V = load_mg(x + 16 * (size_t)q);
+            tangent_to_real(V, conj_mg(load_out(out, q, 0)), grad + 8 * (size_t)q);
+        }
     __syncthreads();
     for (int k = t; k < n; k += kBT) {
         r[k] = grad[k];
         d[k] = -grad[k];
         z[k] = 0.0;
+        Hz[k] = 0.0;
     }
     const double rsq = block_dot(grad, grad, n, sh);
     if (t == 0) {
-        st->fx = out[0];
+        if (!have) {
+            st->fx = out[0];
+            st->fx_im = out[1];
+        }
         if (f_citer) {
-            f_citer[2 * st->k] = out[0];
-            f_citer[2 * st->k + 1] = out[1];
+            f_citer[2 * st->k] = st->fx;
+            f_citer[2 * st->k + 1] = st->fx_im;
         }
         st->rsq = rsq;
         st->stoptol = fmax(st->abstol, st->reltol * sqrt(rsq));
@@ -318,7 +325,7 @@ __global__ __launch_bounds__(kBT) void k_grad_init(const double* out, const doub
 // H d from an HVP sweep along d, then one step of the truncated CG (mg_trust_region.cpp
 // steihaug; the roots as quadratic_roots / boundary_step)
 __global__ __launch_bounds__(kBT) void k_hvp_tcg(const double* out, const double* x, int P, double* Hd, double* r,
-                                                 double* d, double* z, DevTRState* st)
+                                                 double* d, double* z, double* Hz, DevTRState* st)
 {
     __shared__ double sh[kBT];
     if (st->done) return;
@@ -349,7 +356,10 @@ __global__ __launch_bounds__(kBT) void k_hvp_tcg(const double* out, const double
     const double alpha = rsq / dHd;
     const bool boundary = dHd <= 0.0 || alpha > tb;
     if (boundary) {
-        for (int k = t; k < n; k += kBT) z[k] += tb * d[k];
+        for (int k = t; k < n; k += kBT) {
+            z[k] += tb * d[k];
+            Hz[k] += tb * Hd[k];
+        }
         if (t == 0) {
             st->bnd = 1;
             st->done = 1;
@@ -358,6 +368,7 @@ __global__ __launch_bounds__(kBT) void k_hvp_tcg(const double* out, const double
         for (int k = t; k < n; k += kBT) {
             r[k] += alpha * Hd[k];
             z[k] += alpha * d[k];
+            Hz[k] += alpha * Hd[k];
         }
         const double rsq_next = block_dot(r, r, n, sh);
         const bool conv = sqrt(rsq_next) <= st->stoptol;
@@ -431,6 +442,60 @@ __global__ __launch_bounds__(kBT) void k_heta_update(const double* out, double* 
         for (int k = t; k < 16 * P; k += kBT) x[k] = xn[k];
 }
 
+// (reuse) f and the Riemannian gradient at the trial point from an f + gradient sweep there
+__global__ __launch_bounds__(kBT) void k_trial_grad(const double* out, const double* xn, int P, double* grad_n,
+                                                    DevTRState* st)
+{
+    if (st->skip_tail) return;
+    const int t = threadIdx.x;
+    for (int q = t; q < P; q += kBT) {
+        const MG V = load_mg(xn + 16 * (size_t)q);
+        tangent_to_real(V, conj_mg(load_out(out, q, 0)), grad_n + 8 * (size_t)q);
+    }
+    if (t == 0) {
+        st->fxn = out[0];
+        st->fxn_im = out[1];
+    }
+}
+
+// (reuse) the ratio test with H eta = the CG's sum of its steps' products; on acceptance the
+// trial point's f and gradient become the iterate's
+__global__ __launch_bounds__(kBT) void k_update_reuse(double* x, const double* xn, const double* z, double* grad,
+                                                      const double* grad_n, const double* Hz, int P, DevTRState* st,
+                                                      double* f_iter)
+{
+    __shared__ double sh[kBT];
+    __shared__ int accept_sh;
+    if (st->skip_tail) return;
+    const int t = threadIdx.x, n = 8 * P;
+    const double gz = block_dot(grad, z, n, sh);
+    const double zhz = block_dot(z, Hz, n, sh);
+    if (t == 0) {
+        const double rho = (st->fxn - st->fx) / (gz + 0.5 * zhz);
+        if (rho < 0.25) st->radius *= 0.25;
+        else if (rho > 0.75 && st->bnd) st->radius = fmin(2 * st->radius, st->maxradius);
+        accept_sh = rho > st->rho_trust;
+        f_iter[st->k] = st->fx;
+        st->k++;
+        st->kept = 1;
+        if (accept_sh) {
+            st->fx = st->fxn;
+            st->fx_im = st->fxn_im;
+        }
+    }
+    __syncthreads();
+    if (accept_sh) {
+        for (int k = t; k < 16 * P; k += kBT) x[k] = xn[k];
+        for (int k = t; k < n; k += kBT) grad[k] = grad_n[k];
+    }
+}
+
+// (reuse) f and gradient held from sweeps of the other precision: the next head sweeps again
+__global__ void k_drop_kept(DevTRState* st)
+{
+    if (threadIdx.x == 0) st->kept = 0;
+}
+
 // every device buffer of one optimisation
 struct Plans {
     SweepPlan p[3];  // modes f, f + gradient, f + gradient + HVP
@@ -439,7 +504,8 @@ struct Plans {
 
 struct Run {
     int P = 0, G = 0;
-    DevBuf x, xn, grad, r, d, z, Hd, Heta, st, f_iter, f_citer, pm;
+    bool reuse = false;
+    DevBuf x, xn, grad, grad_n, r, d, z, Hd, Hz, Heta, st, f_iter, f_citer, pm;
     std::unique_ptr<Plans> plans[2];       // [0] FP64, [1] FP32
     hipGraphExec_t step[2] = {nullptr, nullptr}, cont[2] = {nullptr, nullptr};
     DevTRState* host = nullptr;             // pinned mirror of the loop state
@@ -488,7 +554,8 @@ hipError_t enqueue_cg(const Run& run, const Plans& pl, hipStream_t s)
     if (e == hipSuccess) e = launch_plan(pl.p[2], done, s);
     if (e == hipSuccess) {
         hipLaunchKernelGGL(k_hvp_tcg, dim3(1), dim3(kBT), 0, s, (const double*)pl.p[2].out, (const double*)run.x.p,
-                           run.P, (double*)run.Hd.p, (double*)run.r.p, (double*)run.d.p, (double*)run.z.p, st);
+                           run.P, (double*)run.Hd.p, (double*)run.r.p, (double*)run.d.p, (double*)run.z.p,
+                           (double*)run.Hz.p, st);
         e = hipGetLastError();
     }
     return e;
@@ -502,6 +569,24 @@ hipError_t enqueue_tail(const Run& run, const Plans& pl, hipStream_t s)
     hipLaunchKernelGGL(k_retract, dim3((run.P + 63) / 64), dim3(64), 0, s, (const double*)run.x.p, (const double*)run.z.p,
                        (double*)run.xn.p, run.P, skip);
     hipError_t e = hipGetLastError();
+    if (run.reuse) {
+        // f + gradient at the trial point; H eta from the CG's sum
+        if (e == hipSuccess) e = enqueue_coefs_any(run, pl.p[1], pl.kpos[1], (const double*)run.xn.p, nullptr, skip, s);
+        if (e == hipSuccess) e = launch_plan(pl.p[1], skip, s);
+        if (e == hipSuccess) {
+            hipLaunchKernelGGL(k_trial_grad, dim3(1), dim3(kBT), 0, s, (const double*)pl.p[1].out, (const double*)run.xn.p,
+                               run.P, (double*)run.grad_n.p, st);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess) {
+            hipLaunchKernelGGL(k_update_reuse, dim3(1), dim3(kBT), 0, s, (double*)run.x.p, (const double*)run.xn.p,
+                               (const double*)run.z.p, (double*)run.grad.p, (const double*)run.grad_n.p,
+                               (const double*)run.Hz.p, run.P, st, (double*)run.f_iter.p);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess) e = hipMemcpyAsync(run.host, run.st.p, sizeof(DevTRState), hipMemcpyDeviceToHost, s);
+        return e;
+    }
     if (e == hipSuccess) e = enqueue_coefs_any(run, pl.p[0], pl.kpos[0], (const double*)run.xn.p, nullptr, skip, s);
     if (e == hipSuccess) e = launch_plan(pl.p[0], skip, s);
     if (e == hipSuccess) {
@@ -524,12 +609,14 @@ hipError_t enqueue_tail(const Run& run, const Plans& pl, hipStream_t s)
 hipError_t enqueue_head(const Run& run, const Plans& pl, hipStream_t s)
 {
     DevTRState* st = (DevTRState*)run.st.p;
-    hipError_t e = enqueue_coefs_any(run, pl.p[1], pl.kpos[1], (const double*)run.x.p, nullptr, nullptr, s);
-    if (e == hipSuccess) e = launch_plan(pl.p[1], nullptr, s);
+    // with reuse the sweep runs only while nothing is kept (the first iteration, a precision switch)
+    const int* skip = run.reuse ? &st->kept : nullptr;
+    hipError_t e = enqueue_coefs_any(run, pl.p[1], pl.kpos[1], (const double*)run.x.p, nullptr, skip, s);
+    if (e == hipSuccess) e = launch_plan(pl.p[1], skip, s);
     if (e == hipSuccess) {
         hipLaunchKernelGGL(k_grad_init, dim3(1), dim3(kBT), 0, s, (const double*)pl.p[1].out, (const double*)run.x.p,
-                           run.P, (double*)run.grad.p, (double*)run.r.p, (double*)run.d.p, (double*)run.z.p, st,
-                           (double*)run.f_citer.p);
+                           run.P, (double*)run.grad.p, (double*)run.r.p, (double*)run.d.p, (double*)run.z.p,
+                           (double*)run.Hz.p, st, (double*)run.f_citer.p);
         e = hipGetLastError();
     }
     return e;
@@ -596,8 +683,8 @@ int prepare(Run& run, int pi, const Target& t, const Program& p, const zd* gates
 }  // namespace
 
 int optimize_device(const Target& t, const Program& p, const zd* gates_start, const rtr_params* prm, int niter,
-                    int precision, int precision_switch, double* f_iter, zd* f_citer, zd* gates_opt, int* nhvp,
-                    DevTRStats* stats)
+                    int precision, int precision_switch, int reuse, double* f_iter, zd* f_citer, zd* gates_opt,
+                    int* nhvp, DevTRStats* stats)
 {
     if (prm->g_func || t.kind != 0 || niter < 1) return 0;
     const int P = p.P, n = 8 * P;
@@ -605,9 +692,11 @@ int optimize_device(const Target& t, const Program& p, const zd* gates_start, co
     Run run;
     run.P = P;
     run.G = p.G;
+    run.reuse = reuse != 0;
     TR_HIP(run.x.ensure(16 * (size_t)P * sizeof(double)));
     TR_HIP(run.xn.ensure(16 * (size_t)P * sizeof(double)));
-    for (DevBuf* b : {&run.grad, &run.r, &run.d, &run.z, &run.Hd, &run.Heta}) TR_HIP(b->ensure((size_t)n * sizeof(double)));
+    for (DevBuf* b : {&run.grad, &run.grad_n, &run.r, &run.d, &run.z, &run.Hd, &run.Hz, &run.Heta})
+        TR_HIP(b->ensure((size_t)n * sizeof(double)));
     TR_HIP(run.st.ensure(sizeof(DevTRState)));
     TR_HIP(run.f_iter.ensure((size_t)(niter + 1) * sizeof(double)));
     TR_HIP(run.f_citer.ensure((size_t)(niter + 1) * 2 * sizeof(double)));
@@ -627,6 +716,7 @@ int optimize_device(const Target& t, const Program& p, const zd* gates_start, co
     st0.reltol = prm->tcg_params.reltol;
     st0.maxiter = prm->tcg_params.maxiter;
     st0.skip_tail = 1;
+    st0.reuse = run.reuse ? 1 : 0;
     TR_HIP(hipMemcpyAsync(run.pm.p, pm.data(), pm.size() * sizeof(int2), hipMemcpyHostToDevice, s));
     TR_HIP(hipMemcpyAsync(run.x.p, gates_start, 16 * (size_t)P * sizeof(double), hipMemcpyHostToDevice, s));
     TR_HIP(hipMemcpyAsync(run.st.p, &st0, sizeof(DevTRState), hipMemcpyHostToDevice, s));
@@ -641,6 +731,11 @@ int optimize_device(const Target& t, const Program& p, const zd* gates_start, co
     long launches = 0;
     for (int k = 0; k < niter; k++) {
         const int pi = (precision == kFP64 || k >= niter - sw) ? 0 : 1;
+        // kept f and gradient of FP32 sweeps do not start the FP64 iterations
+        if (run.reuse && k > 0 && pi == 0 && precision == kFP32 && k == niter - sw) {
+            hipLaunchKernelGGL(k_drop_kept, dim3(1), dim3(64), 0, s, (DevTRState*)run.st.p);
+            TR_HIP(hipGetLastError());
+        }
         TR_HIP(hipGraphLaunch(run.step[pi], s));
         TR_HIP(hipStreamSynchronize(s));
         launches++;

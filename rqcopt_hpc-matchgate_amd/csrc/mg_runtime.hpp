@@ -132,7 +132,8 @@ struct Knobs {
     int mat_kernel = -1;         // target rows beyond 12 qubits: -1 auto, 0 register tiles, 1 matrix cores
     int fwd_mfma = -1;           // FP32 forward passes on the matrix cores (tiled f, f + gradient; L=12 HVP in LDS): -1 auto, 0 gate by gate
     int device_tr = -1;          // mg_optimize with HVPs: -1 auto (trust region on the device, one graph per
-                                 // iteration, where its sweeps have plans), 0 the host loop
+                                 // iteration, where its sweeps have plans), 0 the host loop, 2 on the
+                                 // device with reuse (H eta from the CG, f + gradient at the trial kept)
 };
 Knobs& knobs();
 

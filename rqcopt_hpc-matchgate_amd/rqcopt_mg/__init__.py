@@ -551,7 +551,8 @@ def set_forward_mfma(mode):
 
 def set_device_trust_region(mode):
     """optimize with hvp=True: -1 trust region on the device, one graph per iteration (default,
-    where it applies), 0 the host loop"""
+    where it applies), 0 the host loop, 2 on the device with reuse (H eta of the ratio test from
+    the truncated CG's products, f + gradient at an accepted trial point kept)"""
     lib.mg_set_device_trust_region(int(mode))
 
 

@@ -706,6 +706,44 @@ def test_device_trust_region_matches_host_loop(mg):
         assert np.max(np.abs(dev[2] - host[2])) < tol and dev[3] == host[3]
 
 
+def test_device_trust_region_reuse(mg):
+    """Mode 2 of the device trust region reuses what it evaluates: H eta of each ratio test is the
+    truncated CG's sum of its steps' products (no HVP sweep along eta) and f at the trial point
+    comes with its gradient, which starts the next iteration when the step is accepted (and is
+    kept after a rejection).  The iterates equal the reference control flow's up to rounding,
+    with exactly one HVP sweep per iteration fewer.  L=8 tied Hubbard (FP64) and L=12 open
+    brickwall against a materialised circuit target (FP64; FP32 with a two-iteration switch)."""
+    def both(t, p, V, niter, prec=mg.MG_FP64, sw=0):
+        res = []
+        for mode in (0, 2):
+            mg.set_device_trust_region(mode)
+            res.append(mg.optimize(t, p, V, niter, hvp=True, precision=prec, precision_switch=sw) +
+                       (mg.last_trust_region_graphs(),))
+        mg.set_device_trust_region(-1)
+        return res
+
+    say("case: device trust region with reuse, Hubbard L=8 n5 (tied), FP64, 10 iterations")
+    U, V, pm = _hubbard_q8(5)
+    host, dev = both(mg.Target.dense(U), mg.Program.brickwall(8, pm), V, 10)
+    assert host[4] == 0 and dev[4] >= 10
+    assert np.max(np.abs(dev[0] - host[0])) < 1e-10 and np.max(np.abs(dev[2] - host[2])) < 1e-10
+    assert dev[3] == host[3] - 10
+    rng = np.random.default_rng(41)
+    L = 12
+    param, wires, G = open_brickwall(L, 7)
+    V = haar_matchgates(rng, G)
+    tparam, twires, Gt = open_brickwall(L, 13)
+    t = mg.Target.circuit(haar_matchgates(rng, Gt), twires, L)
+    t.densify()
+    p = mg.Program.create(L, G, param, wires)
+    for prec, sw, tol in ((mg.MG_FP64, 0, 1e-9), (mg.MG_FP32, 2, 1e-4)):
+        say(f"case: device trust region with reuse, L=12 open depth 7, precision {prec}, switch {sw}, 6 iterations")
+        host, dev = both(t, p, V, 6, prec, sw)
+        assert host[4] == 0 and dev[4] >= 6
+        assert np.max(np.abs(dev[0] - host[0])) < tol * np.max(np.abs(host[0]))
+        assert np.max(np.abs(dev[2] - host[2])) < tol and dev[3] == host[3] - 6
+
+
 def test_ti_hessian_matches_oracle(mg):
     say("case: translation-invariant Hessian shortcut")
     rng = np.random.default_rng(2)
