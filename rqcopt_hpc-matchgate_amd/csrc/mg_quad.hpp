@@ -318,18 +318,37 @@ __device__ inline Quad<R> gapply_sub(const cpx<R>* g, const Quad<R>& q, const Qu
             cfms(g[2], q.y, cfms(g[3], q.z, a.z)), cfms(g[6], q.x, cfms(g[7], q.w, a.w))};
 }
 
-// acc[0..7] += d (x) p  (reference src/gate.c:753-757): center then corner entries
+// a*b as the first term of a sum: the operations of cfma(a, b, 0) without the zero operands
 template <typename R>
+__device__ inline cpx<R> cfma0(cpx<R> a, cpx<R> b)
+{
+    return {rfma(a.re, b.re, -a.im * b.im), rfma(a.re, b.im, a.im * b.re)};
+}
+
+// acc[0..7] += d (x) p  (reference src/gate.c:753-757): center then corner entries.
+// INIT: acc = d (x) p, the first term (the sums need no zeroed registers)
+template <typename R, bool INIT = false>
 __device__ inline void outer_acc(cpx<R>* acc, const Quad<R>& d, const Quad<R>& p)
 {
-    acc[0] = cfma(d.y, p.y, acc[0]);
-    acc[1] = cfma(d.y, p.z, acc[1]);
-    acc[2] = cfma(d.z, p.y, acc[2]);
-    acc[3] = cfma(d.z, p.z, acc[3]);
-    acc[4] = cfma(d.x, p.x, acc[4]);
-    acc[5] = cfma(d.x, p.w, acc[5]);
-    acc[6] = cfma(d.w, p.x, acc[6]);
-    acc[7] = cfma(d.w, p.w, acc[7]);
+    if constexpr (INIT) {
+        acc[0] = cfma0(d.y, p.y);
+        acc[1] = cfma0(d.y, p.z);
+        acc[2] = cfma0(d.z, p.y);
+        acc[3] = cfma0(d.z, p.z);
+        acc[4] = cfma0(d.x, p.x);
+        acc[5] = cfma0(d.x, p.w);
+        acc[6] = cfma0(d.w, p.x);
+        acc[7] = cfma0(d.w, p.w);
+    } else {
+        acc[0] = cfma(d.y, p.y, acc[0]);
+        acc[1] = cfma(d.y, p.z, acc[1]);
+        acc[2] = cfma(d.z, p.y, acc[2]);
+        acc[3] = cfma(d.z, p.z, acc[3]);
+        acc[4] = cfma(d.x, p.x, acc[4]);
+        acc[5] = cfma(d.x, p.w, acc[5]);
+        acc[6] = cfma(d.w, p.x, acc[6]);
+        acc[7] = cfma(d.w, p.w, acc[7]);
+    }
 }
 
 // One gate on N of a thread's quadruples (q0, q0 + stride, ...), which are disjoint: all loads
@@ -354,8 +373,9 @@ __device__ inline void fwd_quads(uint32_t q0, uint32_t stride, int lo, int hi, u
     }
 }
 
-// the reverse step (states resident, psi recovered with V^{-1}) on N disjoint quadruples
-template <typename R, int MODE, bool SW, int N>
+// the reverse step (states resident, psi recovered with V^{-1}) on N disjoint quadruples;
+// INIT: the first quadruple's outer products start the sums ga, ha
+template <typename R, int MODE, bool SW, int N, bool INIT = false>
 __device__ inline void rev_quads(uint32_t q0, uint32_t stride, int lo, int hi, uint32_t olo, uint32_t ohi,
                                  const cpx<R>* cf, cpx<R>* lam, cpx<R>* psi, cpx<R>* dlam, cpx<R>* dpsi, cpx<R>* ga,
                                  cpx<R>* ha)
@@ -376,15 +396,41 @@ __device__ inline void rev_quads(uint32_t q0, uint32_t stride, int lo, int hi, u
     for (int i = 0; i < N; i++) {
         const Quad<R> p = gapply(cf + 8, pn[i]);
         store_quad_o(psi, sb[i], olo, ohi, p);
-        outer_acc(ga, l[i], p);
+        if (INIT && i == 0) outer_acc<R, true>(ga, l[i], p);
+        else outer_acc(ga, l[i], p);
         if constexpr (MODE == 2) {
             const Quad<R> dp = gapply(cf + 8, gapply_sub(cf + 16, p, dn[i]));
             store_quad_o(dpsi, sb[i], olo, ohi, dp);
-            outer_acc(ha, dl[i], p);
+            if (INIT && i == 0) outer_acc<R, true>(ha, dl[i], p);
+            else outer_acc(ha, dl[i], p);
             outer_acc(ha, l[i], dp);
             store_quad_o(dlam, sb[i], olo, ohi, gapplyT_add(cf, dl[i], gapplyT(cf + 16, l[i])));
         }
         store_quad_o(lam, sb[i], olo, ohi, gapplyT(cf, l[i]));
+    }
+}
+
+// One gate's reverse step over a thread's quadruples q, q + T, ... < Q, two at a time (TWO: the
+// thread may have more than one), in increasing q; the first sets the sums ga, ha, which stay zero
+// for a thread without quadruples
+template <typename R, int MODE, bool SW, bool TWO>
+__device__ inline void rev_gate(uint32_t q, uint32_t T, uint32_t Q, int lo, int hi, uint32_t olo, uint32_t ohi,
+                                const cpx<R>* cf, cpx<R>* lam, cpx<R>* psi, cpx<R>* dlam, cpx<R>* dpsi, cpx<R>* ga,
+                                cpx<R>* ha)
+{
+    if (TWO && q + T < Q) {
+        rev_quads<R, MODE, SW, 2, true>(q, T, lo, hi, olo, ohi, cf, lam, psi, dlam, dpsi, ga, ha);
+        for (q += 2 * T; q + T < Q; q += 2 * T)
+            rev_quads<R, MODE, SW, 2>(q, T, lo, hi, olo, ohi, cf, lam, psi, dlam, dpsi, ga, ha);
+        if (q < Q) rev_quads<R, MODE, SW, 1>(q, T, lo, hi, olo, ohi, cf, lam, psi, dlam, dpsi, ga, ha);
+    } else if (q < Q) {
+        rev_quads<R, MODE, SW, 1, true>(q, T, lo, hi, olo, ohi, cf, lam, psi, dlam, dpsi, ga, ha);
+    } else {
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            ga[e] = {R(0), R(0)};
+            ha[e] = {R(0), R(0)};
+        }
     }
 }
 

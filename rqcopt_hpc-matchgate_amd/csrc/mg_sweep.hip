@@ -161,18 +161,16 @@ __global__ __launch_bounds__(T) void sweep_kernel(SweepArgs<R> a)
                 cpx<R> cf[24];
                 load_coef<R, MODE == 2 ? 24 : 16>(a.coef + (size_t)g * kCoefPerInst, cf);
                 cpx<R> ga[8], ha[8];
+                if constexpr (CKPT) {
 #pragma unroll
-                for (int k = 0; k < 8; k++) {
-                    ga[k] = {R(0), R(0)};
-                    ha[k] = {R(0), R(0)};
+                    for (int k = 0; k < 8; k++) {
+                        ga[k] = {R(0), R(0)};
+                        ha[k] = {R(0), R(0)};
+                    }
                 }
                 const uint32_t olo = swz_idx<R, LDS>(1u << in.lo), ohi = swz_idx<R, LDS>(1u << in.hi);
-                if constexpr (!CKPT) {
-                    uint32_t q = tid;
-                    for (; q + T < Q; q += 2 * T)
-                        rev_quads<R, MODE, LDS, 2>(q, T, in.lo, in.hi, olo, ohi, cf, lam, psi, dlam, dpsi, ga, ha);
-                    if (q < Q) rev_quads<R, MODE, LDS, 1>(q, T, in.lo, in.hi, olo, ohi, cf, lam, psi, dlam, dpsi, ga, ha);
-                }
+                if constexpr (!CKPT)
+                    rev_gate<R, MODE, LDS, true>(tid, T, Q, in.lo, in.hi, olo, ohi, cf, lam, psi, dlam, dpsi, ga, ha);
                 for (uint32_t q = tid; CKPT && q < Q; q += T) {
                     const uint32_t base = quad_base(q, in.lo, in.hi);
                     const uint32_t sb = swz_idx<R, LDS>(base);

@@ -137,18 +137,9 @@ __global__ __launch_bounds__((1 << tile_lt<R, MODE>()), (tile_min_waves<R, MODE>
                 cpx<R> cf[24];
                 load_coef_c<R, MODE == 2 ? 24 : 16>(a.coef + (size_t)g * kCoefPerInst, cf);
                 cpx<R> ga[8], ha[8];
-#pragma unroll
-                for (int e = 0; e < 8; e++) {
-                    ga[e] = {R(0), R(0)};
-                    ha[e] = {R(0), R(0)};
-                }
                 const uint32_t olo = swz_idx<R, true>(1u << in.lo), ohi = swz_idx<R, true>(1u << in.hi);
-                uint32_t q = tid;
-                if constexpr (2 * T <= Qt)
-                    for (; q + T < Qt; q += 2 * T)
-                        rev_quads<R, MODE, true, 2>(q, T, in.lo, in.hi, olo, ohi, cf, t_lam, t_psi, t_dlam, t_dpsi, ga, ha);
-                if (q < Qt)
-                    rev_quads<R, MODE, true, 1>(q, T, in.lo, in.hi, olo, ohi, cf, t_lam, t_psi, t_dlam, t_dpsi, ga, ha);
+                rev_gate<R, MODE, true, (2 * T <= Qt)>(tid, T, Qt, in.lo, in.hi, olo, ohi, cf, t_lam, t_psi, t_dlam,
+                                                       t_dpsi, ga, ha);
                 R v[NV];
 #pragma unroll
                 for (int e = 0; e < 8; e++) {

@@ -52,7 +52,9 @@ __device__ inline void pass_a(const cpx<R>* V, const cpx<R>* Z, const Win<R, kW>
         wq_put<R, kW, LA, LB>(P, r, p);
         if constexpr (MODE == 2)
             wq_put<R, kW, LA, LB>(D, r, gapplyH(V, gapply_sub(Z, p, wq_get<R, kW, LA, LB>(D, r))));
-        outer_acc(ga, wq_get<R, kW, LA, LB>(Lm, r), p);
+        // the first quadruple starts the sums
+        if (r == 0) outer_acc<R, true>(ga, wq_get<R, kW, LA, LB>(Lm, r), p);
+        else outer_acc(ga, wq_get<R, kW, LA, LB>(Lm, r), p);
         __builtin_amdgcn_sched_barrier(0);  // one quadruple's temporaries at a time
     }
 }
@@ -64,7 +66,8 @@ __device__ inline void pass_b(const Win<R, kW>& Lm, const Win<R, kW>& P, const W
 {
 #pragma unroll
     for (int r = 0; r < WQuads<kW, LA, LB>::N; r++) {
-        outer_acc(ha, wq_get<R, kW, LA, LB>(DL, r), wq_get<R, kW, LA, LB>(P, r));
+        if (r == 0) outer_acc<R, true>(ha, wq_get<R, kW, LA, LB>(DL, r), wq_get<R, kW, LA, LB>(P, r));
+        else outer_acc(ha, wq_get<R, kW, LA, LB>(DL, r), wq_get<R, kW, LA, LB>(P, r));
         outer_acc(ha, wq_get<R, kW, LA, LB>(Lm, r), wq_get<R, kW, LA, LB>(D, r));
     }
 }
@@ -266,9 +269,7 @@ __global__ __launch_bounds__(WinShape<R>::T) void sweep_win_kernel(WinSweepArgs 
                     load_coef_c<R, 8>(coef + (size_t)g * kCoefPerInst, V);
                     if constexpr (MODE == 2) load_coef_c<R, 8>(coef + (size_t)g * kCoefPerInst + 16, Z);
                     {
-                        cpx<R> ga[8];
-#pragma unroll
-                        for (int e = 0; e < 8; e++) ga[e] = {R(0), R(0)};
+                        cpx<R> ga[8];  // set by the gate's first quadruple
 #define MG_PA(WW, A, B) pass_a<R, WW, A, B, MODE>(V, Z, Lm, P, D, ga)
                         MG_WCASES(kW, MG_PA)
 #undef MG_PA
@@ -276,8 +277,6 @@ __global__ __launch_bounds__(WinShape<R>::T) void sweep_win_kernel(WinSweepArgs 
                     }
                     if constexpr (MODE == 2) {
                         cpx<R> ha[8];
-#pragma unroll
-                        for (int e = 0; e < 8; e++) ha[e] = {R(0), R(0)};
 #define MG_PB(WW, A, B) pass_b<R, WW, A, B>(Lm, P, DL, D, ha)
                         MG_WCASES(kW, MG_PB)
 #undef MG_PB
@@ -566,9 +565,7 @@ __global__ __launch_bounds__(512) void sweep_win_tile_kernel(WinTileArgs a)
                             load_coef_c<R, 8>(coef + (size_t)g * kCoefPerInst, V);
                             if constexpr (MODE == 2) load_coef_c<R, 8>(coef + (size_t)g * kCoefPerInst + 16, Z);
                             {
-                                cpx<R> ga[8];
-#pragma unroll
-                                for (int e = 0; e < 8; e++) ga[e] = {R(0), R(0)};
+                                cpx<R> ga[8];  // set by the gate's first quadruple
 #define MG_PA(WW, A, B) pass_a<R, WW, A, B, MODE>(V, Z, Lm, P, D, ga)
                                 MG_WCASES(kW, MG_PA)
 #undef MG_PA
@@ -576,8 +573,6 @@ __global__ __launch_bounds__(512) void sweep_win_tile_kernel(WinTileArgs a)
                             }
                             if constexpr (MODE == 2) {
                                 cpx<R> ha[8];
-#pragma unroll
-                                for (int e = 0; e < 8; e++) ha[e] = {R(0), R(0)};
 #define MG_PB(WW, A, B) pass_b<R, WW, A, B>(Lm, P, DL, D, ha)
                                 MG_WCASES(kW, MG_PB)
 #undef MG_PB
