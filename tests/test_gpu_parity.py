@@ -744,6 +744,35 @@ def test_device_trust_region_reuse(mg):
         assert np.max(np.abs(dev[2] - host[2])) < tol and dev[3] == host[3] - 6
 
 
+def test_device_reuse_matches_served_reuse(mg):
+    """Two implementations of the reuse trust region side by side: mg_optimize on the device
+    (mode 2: loop state, truncated CG and ratio test in device kernels, one graph per iteration)
+    and a served job with tr_reuse (trust region on a host thread of the server, its evaluations
+    through the server's lanes), both with FP32 sweeps and the last 3 iterations in FP64, on the
+    same resident target: the same iterates to 1e-9 and the same HVP count.  L=12 open depth 7
+    against a materialised circuit target, 8 iterations."""
+    say("case: device reuse vs served tr_reuse, L=12 open depth 7, FP32 sweeps, switch 3, 8 iterations")
+    rng = np.random.default_rng(43)
+    L = 12
+    param, wires, G = open_brickwall(L, 7)
+    V = haar_matchgates(rng, G)
+    tparam, twires, Gt = open_brickwall(L, 13)
+    t = mg.Target.circuit(haar_matchgates(rng, Gt), twires, L)
+    t.densify()
+    p = mg.Program.create(L, G, param, wires)
+    mg.set_device_trust_region(2)
+    try:
+        f_dev, _, v_dev, nh_dev = mg.optimize(t, p, V, 8, hvp=True, precision=mg.MG_FP32, precision_switch=3)
+        assert mg.last_trust_region_graphs() >= 8
+    finally:
+        mg.set_device_trust_region(-1)
+    job = {"program": p, "gates": V, "niter": 8, "target": t, "arrival": 0.0}
+    r, _ = mg.serve([job], batched=False, precision=mg.MG_FP32, precision_switch=3, realtime=False, tr_reuse=True)
+    assert r[0]["status"] == 0
+    assert np.max(np.abs(f_dev - r[0]["f_iter"])) < 1e-9 * np.max(np.abs(f_dev))
+    assert np.max(np.abs(v_dev - r[0]["gates"])) < 1e-9 and nh_dev == r[0]["nhvp"]
+
+
 def test_ti_hessian_matches_oracle(mg):
     say("case: translation-invariant Hessian shortcut")
     rng = np.random.default_rng(2)
